@@ -1,0 +1,166 @@
+"""ctypes binding of the HIP library (include/pb_capi.h).
+
+This is the host-side mirror of the reference's time-step interface
+(SOURCE_PASSOPB/time_march.f90:36, wavespace.f90:22, dft_x_realspace.f90:24):
+``Channel.substep`` is ``time_march`` for one RK substep, ``Channel.step``
+runs the three substeps of main.f90:134-156.  Arrays cross the boundary in
+the reference's y-pencil layout (numpy complex128, shape (slot, nz, nx/2, ny),
+i.e. Fortran ``U(ny, nx/2, nz, slot)``).
+
+There is no CPU fallback: when ``libpoongback_hip.so`` is missing or no GPU
+is present every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .config import ChannelConfig, PbConfig
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpoongback_hip.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class PbSubstepOut(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("dt", "velmax", "phi_u", "phi_t", "tau_w", "q_w")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class PbError(RuntimeError):
+    pass
+
+
+# name -> (restype, argtypes); every symbol of include/pb_capi.h
+SIGNATURES = {
+    "pb_create": (ctypes.c_int, [ctypes.POINTER(PbConfig), ctypes.POINTER(ctypes.c_void_p)]),
+    "pb_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "pb_last_error": (ctypes.c_char_p, []),
+    "pb_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]),
+    "pb_get_labs": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 6),
+    "pb_set_state": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "pb_get_state": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "pb_init_stores": (ctypes.c_int, [ctypes.c_void_p]),
+    "pb_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(PbSubstepOut)]),
+    "pb_step": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(PbSubstepOut)]),
+    "pb_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "pb_wavespace_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp,
+                                         ctypes.POINTER(PbSubstepOut)]),
+    "pb_nonlinear_host": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_int32, _dp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PbError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise PbError(f"{load().pb_last_error().decode()} (status {rc})")
+
+
+def _ptr(a: np.ndarray):
+    if a is None:
+        return None
+    if a.dtype != np.complex128 or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays must be C-contiguous complex128")
+    return a.ctypes.data_as(_dp)
+
+
+class Channel:
+    """One channel on the current GPU, state resident in HBM."""
+
+    def __init__(self, cfg: ChannelConfig):
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        h = ctypes.c_void_p()
+        _check(load().pb_create(ctypes.byref(self._c), ctypes.byref(h)))
+        self.h = h
+        bw, nbytes = ctypes.c_int32(0), ctypes.c_int64(0)
+        _check(load().pb_get_info(self.h, ctypes.byref(bw), ctypes.byref(nbytes)))
+        self.bw, self.device_bytes = bw.value, nbytes.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().pb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, nslot):
+        c = self.cfg
+        return (nslot, c.nz, c.nxh, c.ny)
+
+    def labs(self):
+        n, W = self.cfg.ny, 2 * self.bw + 1
+        out = {k: np.zeros((n, W)) for k in ("d0", "d1", "d2", "d0lu")}
+        y, integ = np.zeros(n), np.zeros(n)
+        _check(load().pb_get_labs(self.h, y.ctypes.data_as(_dp), *[out[k].ctypes.data_as(_dp) for k in
+                                  ("d0", "d1", "d2", "d0lu")], integ.ctypes.data_as(_dp)))
+        out["y"], out["integ"] = y, integ
+        return out
+
+    def set_state(self, S: np.ndarray, RK: np.ndarray | None = None):
+        assert S.shape == self.shape(4)
+        if RK is not None:
+            assert RK.shape == self.shape(3)
+        _check(load().pb_set_state(self.h, _ptr(S), _ptr(RK)))
+
+    def get_state(self):
+        S = np.zeros(self.shape(4), np.complex128)
+        RK = np.zeros(self.shape(3), np.complex128)
+        _check(load().pb_get_state(self.h, _ptr(S), _ptr(RK)))
+        return S, RK
+
+    def init_stores(self):
+        _check(load().pb_init_stores(self.h))
+
+    def substep(self, s: int, dt: float):
+        d = ctypes.c_double(dt)
+        o = PbSubstepOut()
+        _check(load().pb_substep(self.h, s, ctypes.byref(d), ctypes.byref(o)))
+        return o.as_dict()
+
+    def step(self, dt: float):
+        d = ctypes.c_double(dt)
+        o = (PbSubstepOut * self.cfg.rk_step)()
+        _check(load().pb_step(self.h, ctypes.byref(d), o))
+        return [x.as_dict() for x in o]
+
+    def sync(self):
+        _check(load().pb_sync(self.h))
+
+    def nonlinear_host(self, S: np.ndarray):
+        P = np.zeros(self.shape(10), np.complex128)
+        vm = ctypes.c_double(0)
+        _check(load().pb_nonlinear_host(self.h, _ptr(np.ascontiguousarray(S)), _ptr(P), 0, ctypes.byref(vm)))
+        return P, vm.value
+
+    def wavespace_host(self, s: int, dt: float, P: np.ndarray, RK: np.ndarray, v_old: np.ndarray | None = None):
+        o = PbSubstepOut()
+        _check(load().pb_wavespace_host(self.h, s, dt, _ptr(P), _ptr(RK),
+                                        None if v_old is None else _ptr(np.ascontiguousarray(v_old)),
+                                        ctypes.byref(o)))
+        return o.as_dict()

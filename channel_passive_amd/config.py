@@ -1,0 +1,132 @@
+"""Run configurations for the channel time step and the ctypes mirror of
+``pb_config`` (include/pb_capi.h).
+
+The physical set-up follows the reference's ``input`` records
+(SOURCE_PASSOPB/read_input_file.f90:39-116): box sizes are given as
+``LMFx, LMFy, LMFz`` with Lx = 2*pi*LMFx, Ly = 2*LMFy, Lz = 2*pi*LMFz, the
+flow rate and the bulk scalar are held at ``bulk_u`` / ``bulk_t``
+(wave00.f90:202, 612).  Friction-velocity units are used (u_tau = 1,
+nu = 1/Re_tau), so dt = 0.005 at Re_tau = 180 is dt+ = 0.9.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+
+PB_KNOTS_REFERENCE = 0
+PB_KNOTS_TANH = 1
+
+
+class PbConfig(ctypes.Structure):
+    """Byte-for-byte mirror of ``struct pb_config``."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64),
+        ("labs_k", ctypes.c_int32), ("knot_map", ctypes.c_int32),
+        ("stretch", ctypes.c_double),
+        ("lmfx", ctypes.c_double), ("lmfy", ctypes.c_double), ("lmfz", ctypes.c_double),
+        ("nu", ctypes.c_double), ("pr", ctypes.c_double),
+        ("bulk_u", ctypes.c_double), ("bulk_w", ctypes.c_double), ("bulk_t", ctypes.c_double),
+        ("u_gal", ctypes.c_double),
+        ("cfl", ctypes.c_double), ("dt_factor", ctypes.c_double), ("dy_factor", ctypes.c_double),
+        ("opp_alpha", ctypes.c_double), ("opp_ydist", ctypes.c_double),
+        ("rk_step", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ]
+
+
+def dean_bulk_velocity(re_tau: float) -> float:
+    """U_b+ from Dean's correlation Re_tau = 0.09 Re_b^0.88, Re_b = 2 h U_b / nu."""
+    re_b = (re_tau / 0.09) ** (1.0 / 0.88)
+    return re_b / (2.0 * re_tau)
+
+
+@dataclasses.dataclass
+class ChannelConfig:
+    nx: int
+    ny: int
+    nz: int
+    re_tau: float = 180.0
+    lmfx: float = 1.0
+    lmfy: float = 1.0
+    lmfz: float = 0.5
+    labs_k: int = 7               # LABS_K; GSL order labs_k + 1
+    knot_map: int = PB_KNOTS_TANH
+    stretch: float = 2.0
+    pr: float = 0.71
+    bulk_u: float | None = None   # default: Dean's correlation
+    bulk_w: float = 0.0
+    bulk_t: float = 1.0
+    u_gal: float = 0.0
+    cfl: float = 1.0
+    dt_factor: float = 1.0        # fixed dt
+    dy_factor: float = 1.0
+    opp_alpha: float = 0.0
+    opp_yplus: float = 10.0
+    dt: float = 0.005
+    rk_step: int = 3
+    name: str = ""
+
+    @property
+    def nu(self) -> float:
+        return 1.0 / self.re_tau
+
+    @property
+    def nxh(self) -> int:
+        return self.nx // 2
+
+    @property
+    def n_spec(self) -> int:
+        return self.ny * self.nxh * self.nz
+
+    def to_c(self) -> PbConfig:
+        c = PbConfig()
+        c.nx, c.ny, c.nz = self.nx, self.ny, self.nz
+        c.labs_k, c.knot_map, c.stretch = self.labs_k, self.knot_map, self.stretch
+        c.lmfx, c.lmfy, c.lmfz = self.lmfx, self.lmfy, self.lmfz
+        c.nu, c.pr = self.nu, self.pr
+        c.bulk_u = self.bulk_u if self.bulk_u is not None else dean_bulk_velocity(self.re_tau)
+        c.bulk_w, c.bulk_t, c.u_gal = self.bulk_w, self.bulk_t, self.u_gal
+        c.cfl, c.dt_factor, c.dy_factor = self.cfl, self.dt_factor, self.dy_factor
+        c.opp_alpha = self.opp_alpha
+        c.opp_ydist = self.opp_yplus / self.re_tau
+        c.rk_step = self.rk_step
+        return c
+
+    def bulk_u_value(self) -> float:
+        return self.bulk_u if self.bulk_u is not None else dean_bulk_velocity(self.re_tau)
+
+
+def preset(name: str) -> ChannelConfig:
+    """The workloads of the benchmark description, by name."""
+    if name == "re180_64":      # small, CPU-runnable; opposition control off
+        return ChannelConfig(64, 65, 64, re_tau=180.0, lmfx=1.0, lmfz=0.5, dt=0.005, name=name)
+    if name == "re180_192":     # production resolution with opposition control at y+ = 10
+        return ChannelConfig(192, 129, 160, re_tau=180.0, lmfx=2.0, lmfz=2.0 / 3.0, dt=0.005,
+                             opp_alpha=1.0, opp_yplus=10.0, name=name)
+    if name == "re1000":        # large single-GPU share; opposition control at y+ = 15
+        return ChannelConfig(1024, 385, 768, re_tau=1000.0, lmfx=1.0, lmfz=0.5, dt=2.0e-4,
+                             opp_alpha=1.0, opp_yplus=15.0, name=name)
+    if name == "tiny":          # unit-test size
+        return ChannelConfig(16, 33, 12, re_tau=180.0, lmfx=1.0, lmfz=0.5, dt=0.005, name=name)
+    raise KeyError(name)
+
+
+def physical_wavenumbers(cfg: ChannelConfig):
+    """kx, kz index tables (module_ppdfftw.f90:347-384) times 1/LMF."""
+    import numpy as np
+    nh = cfg.nz // 2
+    kx = np.arange(cfg.nxh, dtype=np.int64)
+    kz = np.empty(cfg.nz, dtype=np.int64)
+    kz[: nh + 1] = np.arange(nh + 1)
+    kz[nh + 1:] = np.arange(nh + 2, cfg.nz + 1) - 2 * nh - 1
+    return kx, kz, kx / cfg.lmfx, kz / cfg.lmfz
+
+
+def roofline_bytes_per_step(cfg: ChannelConfig) -> int:
+    """Algorithmic HBM bytes of one RK3 step: 3 substeps x 30 spectral
+    field passes x 16 B x ny*(nx/2)*nz."""
+    return 3 * 30 * 16 * cfg.n_spec
+
+
+PI = math.pi

@@ -1,0 +1,569 @@
+// capi.hip -- implementation of include/pb_capi.h on one gfx950 GPU.
+//
+// Replaces the body of time_march (SOURCE_PASSOPB/time_march.f90:36-128):
+// the MPI pencil transposes collapse into device-layout choices, every
+// stage runs as kernels on one HIP stream and the state never leaves HBM
+// between pb_set_state and pb_get_state.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pb_capi.h"
+#include "bspline.h"
+#include "kernels.hip"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return fail(PB_E_HIP, "%s: %s", #call, hipGetErrorString(e_));      \
+  } while (0)
+
+pb::FftPlan make_fft_plan(int N) {
+  pb::FftPlan p{};
+  p.N = N;
+  p.np = 0;
+  int m = N;
+  for (int r : {8, 4, 2, 3, 5})
+    while (m % r == 0 && p.np < 16) { p.r[p.np++] = r; m /= r; }
+  if (m != 1) p.np = -1;
+  return p;
+}
+
+std::vector<double2> twiddles(int N) {
+  std::vector<double2> t(N);
+  const long double pi = 3.14159265358979323846264338327950288L;
+  for (int j = 0; j < N; ++j) {
+    const long double a = 2.0L * pi * (long double)j / (long double)N;
+    t[j] = make_double2((double)cosl(a), -(double)sinl(a));
+  }
+  return t;
+}
+
+}  // namespace
+
+struct pb_ctx {
+  pb_config cfg;
+  int nx, ny, nz, nxh, nx32, nz32, nh, bw, W, order;
+  size_t plane, field, pplane, pfield, nsys;
+  double Lx, Lz, dxr, dzr, cflx, rescale, lmfxr, lmfzr;
+  pb::Collocation col;
+  std::vector<double> d0, d1, d2, d0lu, dyr;
+  int opp_first[2] = {0, 0};
+  std::vector<double> opp_rows;
+  pb::FftPlan pz, px;
+  int Bz;
+  hipStream_t stream = nullptr;
+  // device memory
+  double2 *U = nullptr, *RK = nullptr, *pad = nullptr, *buf = nullptr, *vd = nullptr;
+  double *dbuf = nullptr, *vpart = nullptr, *scal = nullptr;
+  double *dD0 = nullptr, *dD1 = nullptr, *dD2 = nullptr, *dD0LU = nullptr, *dinteg = nullptr, *ddyr = nullptr,
+         *drows = nullptr;
+  double2 *twz = nullptr, *twx = nullptr;
+  int *slotmap = nullptr;
+  double *hscal = nullptr;  // pinned host mirror of scal
+  size_t bytes = 0;
+  bool have_state = false;
+  pb::RK3 rk;
+  // per-stage event timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, size_t>> ev_marks;  // (stage, index of start event)
+
+  pb::LineParams line_params(int s, double dt) const {
+    pb::LineParams P;
+    P.g = pb::BandGeom{ny, bw};
+    P.D0 = pb::SharedBand{dD0, W};
+    P.D1 = pb::SharedBand{dD1, W};
+    P.D2 = pb::SharedBand{dD2, W};
+    P.D0LU = pb::SharedBand{dD0LU, W};
+    P.integ = dinteg;
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 3; ++b) P.rk[a][b] = rk.c[a][b];
+    P.dt = dt;
+    P.nu = cfg.nu;
+    P.pr = cfg.pr;
+    P.bulk_u = cfg.bulk_u;
+    P.bulk_w = cfg.bulk_w;
+    P.bulk_t = cfg.bulk_t;
+    P.u_gal = cfg.u_gal;
+    P.s = s;
+    P.S = cfg.rk_step;
+    return P;
+  }
+};
+
+// scal layout: [0] dt, [1] velmax, [2 + 4*(s-1) + q] mean outputs of substep s
+static constexpr int SCAL_N = 2 + 4 * 3;
+
+extern "C" {
+
+const char *pb_last_error(void) { return g_err.c_str(); }
+
+static int ctx_free(pb_ctx *c) {
+  if (!c) return 0;
+  void *ptrs[] = {c->U, c->RK, c->pad, c->buf, c->vd, c->dbuf, c->vpart, c->scal, c->dD0, c->dD1, c->dD2,
+                  c->dD0LU, c->dinteg, c->ddyr, c->drows, c->twz, c->twx, c->slotmap};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int pb_destroy(pb_ctx *ctx) { return ctx_free(ctx); }
+
+int pb_create(const pb_config *cfg, pb_ctx **out) {
+  if (!cfg || !out) return fail(PB_E_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->nx < 4 || cfg->nz < 4 || cfg->nx % 2 || cfg->nz % 2 || cfg->ny < 8)
+    return fail(PB_E_ARG, "bad grid %lld x %lld x %lld", (long long)cfg->nx, (long long)cfg->ny, (long long)cfg->nz);
+  if ((cfg->nx * 3) % 2 || (cfg->nz * 3) % 2) return fail(PB_E_ARG, "nx, nz must give integer 3/2 sizes");
+  if (cfg->rk_step != 3) return fail(PB_E_ARG, "only the 3-substep scheme is implemented (rk_step=%d)", cfg->rk_step);
+  pb_ctx *c = new pb_ctx();
+  c->cfg = *cfg;
+  c->nx = (int)cfg->nx; c->ny = (int)cfg->ny; c->nz = (int)cfg->nz;
+  c->nxh = c->nx / 2; c->nx32 = c->nx * 3 / 2; c->nz32 = c->nz * 3 / 2; c->nh = c->nz / 2;
+  c->order = cfg->labs_k + 1;
+  const double pi = 3.14159265358979323;  // module_global_variable.f90:18
+  c->Lx = cfg->lmfx * 2 * pi;
+  c->Lz = cfg->lmfz * 2 * pi;
+  c->rescale = 1.0 / (double)(c->nx32 * c->nz32);
+  c->dxr = (double)c->nx / c->Lx;
+  c->dzr = (double)c->nz / c->Lz;
+  c->cflx = cfg->cfl / pi;
+  c->lmfxr = 1.0 / cfg->lmfx;
+  c->lmfzr = 1.0 / cfg->lmfz;
+  c->rk = pb::rk3_coefficients();
+  int r = pb::build_collocation(c->order, c->ny, cfg->knot_map, cfg->stretch, 2.0 * cfg->lmfy, c->col);
+  if (r) { ctx_free(c); return fail(PB_E_ARG, "B-spline set-up failed (%d): check ny, labs_k, knots", r); }
+  c->bw = pb::eval_bandwidth(c->col);
+  if (c->bw < 1) { ctx_free(c); return fail(PB_E_ARG, "no band structure found"); }
+  c->W = 2 * c->bw + 1;
+  pb::to_band(c->col.D[0], c->ny, c->bw, c->d0);
+  pb::to_band(c->col.D[1], c->ny, c->bw, c->d1);
+  pb::to_band(c->col.D[2], c->ny, c->bw, c->d2);
+  c->d0lu = c->d0;
+  {
+    pb::OwnBand ob{c->d0lu.data(), 1, c->W};
+    pb::band_lu(pb::BandGeom{c->ny, c->bw}, ob);
+  }
+  c->dyr.resize(c->ny);
+  const std::vector<double> &y = c->col.y;
+  for (int i = 0; i < c->ny; ++i) {
+    double d;
+    if (i == 0) d = std::fabs(y[1] - y[0]);
+    else if (i == c->ny - 1) d = std::fabs(y[c->ny - 1] - y[c->ny - 2]);
+    else d = std::min(std::fabs(y[i] - y[i - 1]), std::fabs(y[i] - y[i + 1]));
+    c->dyr[i] = 1.0 / (cfg->dy_factor * d);
+  }
+  if (cfg->opp_alpha != 0.0) {
+    std::vector<double> lo, hi;
+    if (cfg->opp_ydist <= 0.0 || cfg->opp_ydist >= 1.0 ||
+        pb::basis_row(c->col, -1.0 + cfg->opp_ydist, c->opp_first[0], lo) ||
+        pb::basis_row(c->col, 1.0 - cfg->opp_ydist, c->opp_first[1], hi)) {
+      ctx_free(c);
+      return fail(PB_E_ARG, "opposition-control plane %g outside the channel", cfg->opp_ydist);
+    }
+    c->opp_rows = lo;
+    c->opp_rows.insert(c->opp_rows.end(), hi.begin(), hi.end());
+  }
+  c->pz = make_fft_plan(c->nz32);
+  c->px = make_fft_plan(c->nx32);
+  if (c->pz.np < 0 || c->px.np < 0) {
+    ctx_free(c);
+    return fail(PB_E_ARG, "3/2 sizes %d, %d must factor into 2,3,5", c->nx32, c->nz32);
+  }
+  if (2 * c->nx32 > pb::FFT_MAX_ELEMS || c->nz32 * 1 > pb::FFT_MAX_ELEMS) {
+    ctx_free(c);
+    return fail(PB_E_ARG, "transform lengths exceed the LDS tile limit");
+  }
+  int dev = 0, lds_max = 65536;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+    ctx_free(c);
+    return fail(PB_E_HIP, "no HIP device");
+  }
+  const size_t lds_cap = std::min<size_t>((size_t)lds_max, 96 * 1024);
+  if ((size_t)c->nz32 * sizeof(double2) > lds_cap || (size_t)2 * c->nx32 * sizeof(double2) > lds_cap) {
+    ctx_free(c);
+    return fail(PB_E_ARG, "transform lengths exceed the LDS limit (%d bytes)", lds_max);
+  }
+  // z tiles: B consecutive kx columns per block (B*16 bytes contiguous per z row)
+  c->Bz = 1;
+  while (c->Bz * 2 <= 16 && c->nxh % (c->Bz * 2) == 0 && (size_t)c->nz32 * c->Bz * 2 * sizeof(double2) <= lds_cap &&
+         c->nz32 * c->Bz * 2 <= 4608)
+    c->Bz *= 2;
+  c->plane = (size_t)c->nz * c->nxh;
+  c->field = c->plane * c->ny;
+  c->pplane = (size_t)c->nz32 * c->nxh;
+  c->pfield = c->pplane * c->ny;
+  c->nsys = c->plane;
+
+  auto dalloc = [&](void **p, size_t bytes) -> int {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(PB_E_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    c->bytes += bytes;
+    return 0;
+  };
+  const size_t bandb = (size_t)c->ny * c->W * sizeof(double);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    ctx_free(c);
+    return fail(PB_E_HIP, "no HIP device / stream");
+  }
+  int rc = 0;
+  rc = rc ? rc : dalloc((void **)&c->U, 9 * c->field * sizeof(double2));
+  rc = rc ? rc : dalloc((void **)&c->RK, 3 * c->field * sizeof(double2));
+  rc = rc ? rc : dalloc((void **)&c->pad, 8 * c->pfield * sizeof(double2));
+  rc = rc ? rc : dalloc((void **)&c->buf, 2 * c->field * sizeof(double2));
+  rc = rc ? rc : dalloc((void **)&c->vd, 2 * c->plane * sizeof(double2));
+  rc = rc ? rc : dalloc((void **)&c->dbuf, (size_t)c->W * c->field * sizeof(double));
+  rc = rc ? rc : dalloc((void **)&c->vpart, (size_t)c->ny * c->nz32 * sizeof(double));
+  rc = rc ? rc : dalloc((void **)&c->scal, SCAL_N * sizeof(double));
+  rc = rc ? rc : dalloc((void **)&c->dD0, bandb);
+  rc = rc ? rc : dalloc((void **)&c->dD1, bandb);
+  rc = rc ? rc : dalloc((void **)&c->dD2, bandb);
+  rc = rc ? rc : dalloc((void **)&c->dD0LU, bandb);
+  rc = rc ? rc : dalloc((void **)&c->dinteg, c->ny * sizeof(double));
+  rc = rc ? rc : dalloc((void **)&c->ddyr, c->ny * sizeof(double));
+  rc = rc ? rc : dalloc((void **)&c->drows, std::max<size_t>(1, c->opp_rows.size()) * sizeof(double));
+  rc = rc ? rc : dalloc((void **)&c->twz, c->nz32 * sizeof(double2));
+  rc = rc ? rc : dalloc((void **)&c->twx, c->nx32 * sizeof(double2));
+  rc = rc ? rc : dalloc((void **)&c->slotmap, 16 * sizeof(int));
+  if (rc) { ctx_free(c); return rc; }
+  if (hipHostMalloc((void **)&c->hscal, SCAL_N * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    ctx_free(c);
+    return fail(PB_E_HIP, "hipHostMalloc failed");
+  }
+  std::vector<double2> tz = twiddles(c->nz32), tx = twiddles(c->nx32);
+#define UP(dst, src, n) \
+  if (hipMemcpy(dst, src, n, hipMemcpyHostToDevice) != hipSuccess) { ctx_free(c); return fail(PB_E_HIP, "upload " #dst); }
+  UP(c->dD0, c->d0.data(), bandb);
+  UP(c->dD1, c->d1.data(), bandb);
+  UP(c->dD2, c->d2.data(), bandb);
+  UP(c->dD0LU, c->d0lu.data(), bandb);
+  UP(c->dinteg, c->col.integ.data(), c->ny * sizeof(double));
+  UP(c->ddyr, c->dyr.data(), c->ny * sizeof(double));
+  if (!c->opp_rows.empty()) UP(c->drows, c->opp_rows.data(), c->opp_rows.size() * sizeof(double));
+  UP(c->twz, tz.data(), c->nz32 * sizeof(double2));
+  UP(c->twx, tx.data(), c->nx32 * sizeof(double2));
+#undef UP
+  if (hipMemset(c->scal, 0, SCAL_N * sizeof(double)) != hipSuccess) { ctx_free(c); return fail(PB_E_HIP, "memset"); }
+  *out = c;
+  return PB_OK;
+}
+
+int pb_get_info(const pb_ctx *c, int32_t *bw, int64_t *bytes) {
+  if (!c) return fail(PB_E_ARG, "null context");
+  if (bw) *bw = c->bw;
+  if (bytes) *bytes = (int64_t)c->bytes;
+  return PB_OK;
+}
+
+int pb_get_labs(const pb_ctx *c, double *y, double *d0, double *d1, double *d2, double *d0lu, double *integ) {
+  if (!c) return fail(PB_E_ARG, "null context");
+  const size_t nb = (size_t)c->ny * c->W;
+  if (y) std::copy(c->col.y.begin(), c->col.y.end(), y);
+  if (d0) std::copy(c->d0.begin(), c->d0.begin() + nb, d0);
+  if (d1) std::copy(c->d1.begin(), c->d1.begin() + nb, d1);
+  if (d2) std::copy(c->d2.begin(), c->d2.begin() + nb, d2);
+  if (d0lu) std::copy(c->d0lu.begin(), c->d0lu.begin() + nb, d0lu);
+  if (integ) std::copy(c->col.integ.begin(), c->col.integ.end(), integ);
+  return PB_OK;
+}
+
+// copy nslot reference-layout slots from host into device slots map[]
+static int upload_ref(pb_ctx *c, const double *host, int nslot, const int *map, double2 *dev_base) {
+  const size_t bytes = (size_t)nslot * c->field * sizeof(double2);
+  if (bytes > 8 * c->pfield * sizeof(double2)) return fail(PB_E_ARG, "staging too small");
+  HIPCHK(hipMemcpyAsync(c->slotmap, map, nslot * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->pad, host, bytes, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(pb::ref_to_dev_kernel, dim3(c->nz, nslot), dim3(256), 0, c->stream, c->pad, dev_base, c->ny,
+                     c->nxh, c->nz, c->slotmap);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PB_OK;
+}
+
+static int download_ref(pb_ctx *c, double *host, int nslot, const int *map, const double2 *dev_base) {
+  const size_t bytes = (size_t)nslot * c->field * sizeof(double2);
+  HIPCHK(hipMemcpyAsync(c->slotmap, map, nslot * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(pb::dev_to_ref_kernel, dim3(c->nz, nslot), dim3(256), 0, c->stream, dev_base, c->pad, c->ny,
+                     c->nxh, c->nz, c->slotmap);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(host, c->pad, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PB_OK;
+}
+
+static const int kStateMap[4] = {0, 1, 2, 6};
+static const int kRKMap[3] = {0, 1, 2};
+
+int pb_set_state(pb_ctx *c, const double *U, const double *rk) {
+  if (!c || !U) return fail(PB_E_ARG, "null argument");
+  int r = upload_ref(c, U, 4, kStateMap, c->U);
+  if (r) return r;
+  if (rk) {
+    r = upload_ref(c, rk, 3, kRKMap, c->RK);
+    if (r) return r;
+  }
+  c->have_state = true;
+  return PB_OK;
+}
+
+int pb_get_state(pb_ctx *c, double *U, double *rk) {
+  if (!c) return fail(PB_E_ARG, "null context");
+  if (!c->have_state) return fail(PB_E_STATE, "no state set");
+  if (U) { int r = download_ref(c, U, 4, kStateMap, c->U); if (r) return r; }
+  if (rk) { int r = download_ref(c, rk, 3, kRKMap, c->RK); if (r) return r; }
+  return PB_OK;
+}
+
+static unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+// Record an event pair around a launch when timing is on.
+static hipEvent_t next_event(pb_ctx *c) {
+  if (c->ev_used == c->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->ev_pool.push_back(e);
+  }
+  return c->ev_pool[c->ev_used++];
+}
+struct StageTimer {
+  pb_ctx *c;
+  int stage;
+  size_t idx = 0;
+  bool on = false;
+  StageTimer(pb_ctx *cc, int st) : c(cc), stage(st) {
+    if (!c->timing) return;
+    idx = c->ev_used;
+    hipEvent_t a = next_event(c), b = next_event(c);
+    if (!a || !b) return;
+    on = (hipEventRecord(a, c->stream) == hipSuccess);
+  }
+  ~StageTimer() {
+    if (!on) return;
+    if (hipEventRecord(c->ev_pool[idx + 1], c->stream) == hipSuccess) c->ev_marks.push_back({stage, idx});
+  }
+};
+
+int pb_init_stores(pb_ctx *c) {
+  if (!c || !c->have_state) return fail(PB_E_STATE, "pb_init_stores needs a state");
+  pb::LineParams P = c->line_params(1, 0.0);
+  hipLaunchKernelGGL(pb::init_stores_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P,
+                     (pb::cplx *)c->U, (pb::cplx *)c->RK, (pb::cplx *)c->buf, c->plane, c->nsys, c->nxh, c->nh,
+                     c->lmfxr, c->lmfzr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PB_OK;
+}
+
+// ---- stage launchers ------------------------------------------------------
+static int launch_oppvd(pb_ctx *c, const double2 *vfield_base_minus_one) {
+  pb::LineParams P = c->line_params(1, 0.0);
+  StageTimer tm(c, PB_STAGE_OPPVD);
+  hipLaunchKernelGGL(pb::oppvd_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P,
+                     (const pb::cplx *)vfield_base_minus_one, (pb::cplx *)c->buf, (pb::cplx *)c->vd, c->plane,
+                     c->nsys, c->opp_first[0], c->opp_first[1], c->drows, c->order);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+
+static int launch_nonlinear(pb_ctx *c, int update_dt) {
+  const size_t ldsz = (size_t)c->nz32 * c->Bz * sizeof(double2);
+  {
+  StageTimer tm(c, PB_STAGE_ZINV);
+  hipLaunchKernelGGL(pb::zinv_kernel, dim3(c->nxh / c->Bz, c->ny, 4), dim3(pb::FFT_THREADS), ldsz, c->stream, c->U,
+                     c->pad, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield);
+  HIPCHK(hipGetLastError());
+  }
+  const size_t ldsx = (size_t)c->nx32 * 2 * sizeof(double2);
+  {
+  StageTimer tm(c, PB_STAGE_XPHYS);
+  hipLaunchKernelGGL(pb::xphys_kernel, dim3(c->nz32, c->ny), dim3(pb::FFT_THREADS), ldsx, c->stream, c->pad, c->px,
+                     c->twx, c->nxh, c->nz32, c->pfield, c->ddyr, c->dxr, c->dzr, c->vpart);
+  HIPCHK(hipGetLastError());
+  }
+  {
+  StageTimer tm(c, PB_STAGE_VELMAX);
+  hipLaunchKernelGGL(pb::velmax_kernel, dim3(1), dim3(256), 0, c->stream, c->vpart, (size_t)c->ny * c->nz32, c->scal,
+                     update_dt, c->cfg.dt_factor, c->cflx);
+  HIPCHK(hipGetLastError());
+  }
+  StageTimer tm(c, PB_STAGE_ZFWD);
+  hipLaunchKernelGGL(pb::zfwd_kernel, dim3(c->nxh / c->Bz, c->ny, 8), dim3(pb::FFT_THREADS), ldsz, c->stream, c->pad,
+                     c->U, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield, c->rescale);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+
+static int launch_wavespace(pb_ctx *c, int s) {
+  pb::LineParams P = c->line_params(s, 0.0);
+  pb::WsArgs A;
+  A.U = (pb::cplx *)c->U;
+  A.RK = (pb::cplx *)c->RK;
+  A.buf = (pb::cplx *)c->buf;
+  A.dbuf = c->dbuf;
+  A.vd = c->cfg.opp_alpha != 0.0 ? (const pb::cplx *)c->vd : nullptr;
+  A.plane = c->plane;
+  A.field = c->field;
+  A.nxh = c->nxh;
+  A.nz = c->nz;
+  A.nh = c->nh;
+  A.lmfxr = c->lmfxr;
+  A.lmfzr = c->lmfzr;
+  A.opp_alpha = c->cfg.opp_alpha;
+  A.mean_out = c->scal + 2 + 4 * (s - 1);
+  StageTimer tm(c, PB_STAGE_WAVESPACE);
+  hipLaunchKernelGGL(pb::wavespace_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P, A, c->nsys,
+                     c->scal);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+
+static int check_dt(double dt) {
+  if (!(dt > 1e-14) || dt > 1e5) return fail(PB_E_NUMERIC, "dt out of range: %g", dt);
+  return PB_OK;
+}
+
+static int substep_async(pb_ctx *c, int s) {
+  int r;
+  if (c->cfg.opp_alpha != 0.0 && (r = launch_oppvd(c, c->U))) return r;
+  if ((r = launch_nonlinear(c, s == 1))) return r;
+  return launch_wavespace(c, s);
+}
+
+static int read_outputs(pb_ctx *c, int s0, int s1, pb_substep_out *out, double *dt_inout) {
+  HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SCAL_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *dt_inout = c->hscal[0];
+  if (out)
+    for (int s = s0; s <= s1; ++s) {
+      pb_substep_out &o = out[s - s0];
+      o.dt = c->hscal[0];
+      o.velmax = c->hscal[1];
+      o.phi_u = c->hscal[2 + 4 * (s - 1) + 0];
+      o.phi_t = c->hscal[2 + 4 * (s - 1) + 1];
+      o.tau_w = c->hscal[2 + 4 * (s - 1) + 2];
+      o.q_w = c->hscal[2 + 4 * (s - 1) + 3];
+    }
+  return check_dt(*dt_inout);
+}
+
+int pb_substep(pb_ctx *c, int32_t s, double *dt_inout, pb_substep_out *out) {
+  if (!c || !dt_inout) return fail(PB_E_ARG, "null argument");
+  if (!c->have_state) return fail(PB_E_STATE, "pb_substep before pb_set_state");
+  if (s < 1 || s > c->cfg.rk_step) return fail(PB_E_ARG, "substep %d out of range", s);
+  int r = check_dt(*dt_inout);
+  if (r && !(s == 1 && c->cfg.dt_factor != 1.0)) return r;
+  HIPCHK(hipMemcpyAsync(c->scal, dt_inout, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if ((r = substep_async(c, s))) return r;
+  return read_outputs(c, s, s, out, dt_inout);
+}
+
+int pb_step(pb_ctx *c, double *dt_inout, pb_substep_out *out) {
+  if (!c || !dt_inout) return fail(PB_E_ARG, "null argument");
+  if (!c->have_state) return fail(PB_E_STATE, "pb_step before pb_set_state");
+  int r = check_dt(*dt_inout);
+  if (r && c->cfg.dt_factor == 1.0) return r;
+  HIPCHK(hipMemcpyAsync(c->scal, dt_inout, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  for (int s = 1; s <= c->cfg.rk_step; ++s)
+    if ((r = substep_async(c, s))) return r;
+  return read_outputs(c, 1, c->cfg.rk_step, out, dt_inout);
+}
+
+int pb_sync(pb_ctx *c) {
+  if (!c) return fail(PB_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PB_OK;
+}
+
+int pb_timing(pb_ctx *c, int32_t enable) {
+  if (!c) return fail(PB_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->timing = enable != 0;
+  c->ev_used = 0;
+  c->ev_marks.clear();
+  return PB_OK;
+}
+
+int pb_timing_read(pb_ctx *c, double *ms, int64_t *launches) {
+  if (!c || !ms || !launches) return fail(PB_E_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < PB_NSTAGE; ++i) { ms[i] = 0.0; launches[i] = 0; }
+  for (const auto &m : c->ev_marks) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, c->ev_pool[m.second], c->ev_pool[m.second + 1]));
+    ms[m.first] += t;
+    launches[m.first] += 1;
+  }
+  return PB_OK;
+}
+
+// ---- host-array drop-ins ---------------------------------------------------
+int pb_nonlinear_host(pb_ctx *c, const double *U, double *P, int32_t want_vv, double *velmax) {
+  if (!c || !U || !P) return fail(PB_E_ARG, "null argument");
+  if (want_vv) return fail(PB_E_ARG, "the v*v statistics field is not produced on the device");
+  int r = upload_ref(c, U, 4, kStateMap, c->U);
+  if (r) return r;
+  c->have_state = true;
+  if ((r = launch_nonlinear(c, 0))) return r;
+  static const int map8[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+  // reference slots 0-8 of P (slot 5 is left as the device holds it: unused)
+  r = download_ref(c, P, 9, map8, c->U);
+  if (r) return r;
+  if (velmax) {
+    HIPCHK(hipMemcpy(c->hscal, c->scal, SCAL_N * sizeof(double), hipMemcpyDeviceToHost));
+    *velmax = c->hscal[1];
+  }
+  return PB_OK;
+}
+
+int pb_wavespace_host(pb_ctx *c, int32_t s, double dt, double *U, double *rk, const double *v_old,
+                      pb_substep_out *out) {
+  if (!c || !U || !rk) return fail(PB_E_ARG, "null argument");
+  if (s < 1 || s > c->cfg.rk_step) return fail(PB_E_ARG, "substep %d out of range", s);
+  static const int map9[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+  int r;
+  if (c->cfg.opp_alpha != 0.0) {
+    if (!v_old) return fail(PB_E_ARG, "opposition control needs v_old");
+    // stage v_old in U slot 1 and evaluate the detection planes
+    static const int map1[1] = {1};
+    if ((r = upload_ref(c, v_old, 1, map1, c->U))) return r;
+    if ((r = launch_oppvd(c, c->U))) return r;
+  }
+  if ((r = upload_ref(c, U, 9, map9, c->U))) return r;
+  if ((r = upload_ref(c, rk, 3, kRKMap, c->RK))) return r;
+  c->have_state = true;
+  HIPCHK(hipMemcpyAsync(c->scal, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if ((r = launch_wavespace(c, s))) return r;
+  if ((r = download_ref(c, U, 9, map9, c->U))) return r;
+  if ((r = download_ref(c, rk, 3, kRKMap, c->RK))) return r;
+  double d = dt;
+  return read_outputs(c, s, s, out, &d);
+}
+
+}  // extern "C"

@@ -1,0 +1,418 @@
+// kernels.hip -- gfx950 kernels of one RK3-CN substep.
+//
+//   zinv_kernel     y_to_x: 3/2 zero-padded inverse c2c DFT along z
+//                   (module_ppdfftw.f90:830-861)
+//   xphys_kernel    dft_x_realspace: c2r along x, the nine products, CFL
+//                   maximum, r2c along x (dft_x_realspace.f90:68-213)
+//   zfwd_kernel     x_to_y + rescale_data: forward c2c along z, truncation,
+//                   Nyquist zeroing, 1/(nx32*nz32) (module_ppdfftw.f90:794-823,
+//                   rescale_data.f90)
+//   wavespace_kernel  one thread per (kx,kz) system (wavespace.f90:46-107)
+//
+// Device layout of every spectral field: [y][kz][kx] (kx fastest); padded
+// z-pencil fields: [y][z][kx].  All FFTs are unnormalised like FFTW's.
+#include <hip/hip_runtime.h>
+
+#include "pb_common.h"
+#include "solver.h"
+
+namespace pb {
+
+struct FftPlan {
+  int N;
+  int np;
+  int r[16];
+};
+
+__device__ __forceinline__ double2 c2(double r, double i) { return make_double2(r, i); }
+__device__ __forceinline__ double2 cadd2(double2 a, double2 b) { return c2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub2(double2 a, double2 b) { return c2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
+  return c2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// multiply by sign*i  (sign = -1 forward, +1 backward)
+__device__ __forceinline__ double2 mul_si(double2 a, int sign) { return sign < 0 ? c2(a.y, -a.x) : c2(-a.y, a.x); }
+
+// in-register DFT of size R with exponent sign (W = exp(sign*2*pi*i/R))
+template <int R>
+__device__ __forceinline__ void dft(double2 *v, int sign);
+
+template <>
+__device__ __forceinline__ void dft<2>(double2 *v, int) {
+  const double2 a = v[0], b = v[1];
+  v[0] = cadd2(a, b);
+  v[1] = csub2(a, b);
+}
+template <>
+__device__ __forceinline__ void dft<4>(double2 *v, int sign) {
+  const double2 a0 = cadd2(v[0], v[2]), a1 = csub2(v[0], v[2]);
+  const double2 b0 = cadd2(v[1], v[3]), b1 = mul_si(csub2(v[1], v[3]), sign);
+  v[0] = cadd2(a0, b0);
+  v[2] = csub2(a0, b0);
+  v[1] = cadd2(a1, b1);
+  v[3] = csub2(a1, b1);
+}
+template <>
+__device__ __forceinline__ void dft<3>(double2 *v, int sign) {
+  const double c = -0.5, s = sign * 0.86602540378443864676;
+  const double2 t = cadd2(v[1], v[2]), d = csub2(v[1], v[2]);
+  const double2 m = c2(v[0].x + c * t.x, v[0].y + c * t.y);
+  const double2 rot = c2(-s * d.y, s * d.x);
+  v[0] = cadd2(v[0], t);
+  v[1] = cadd2(m, rot);
+  v[2] = csub2(m, rot);
+}
+template <>
+__device__ __forceinline__ void dft<5>(double2 *v, int sign) {
+  const double c1 = 0.30901699437494742410, c2v = -0.80901699437494742410;
+  const double s1 = sign * 0.95105651629515357212, s2 = sign * 0.58778525229247312917;
+  const double2 t1 = cadd2(v[1], v[4]), t2 = cadd2(v[2], v[3]);
+  const double2 d1 = csub2(v[1], v[4]), d2 = csub2(v[2], v[3]);
+  const double2 m1 = c2(v[0].x + c1 * t1.x + c2v * t2.x, v[0].y + c1 * t1.y + c2v * t2.y);
+  const double2 m2 = c2(v[0].x + c2v * t1.x + c1 * t2.x, v[0].y + c2v * t1.y + c1 * t2.y);
+  const double2 r1 = c2(-(s1 * d1.y + s2 * d2.y), s1 * d1.x + s2 * d2.x);
+  const double2 r2 = c2(-(s2 * d1.y - s1 * d2.y), s2 * d1.x - s1 * d2.x);
+  v[0] = cadd2(v[0], cadd2(t1, t2));
+  v[1] = cadd2(m1, r1);
+  v[4] = csub2(m1, r1);
+  v[2] = cadd2(m2, r2);
+  v[3] = csub2(m2, r2);
+}
+template <>
+__device__ __forceinline__ void dft<8>(double2 *v, int sign) {
+  const double h = 0.70710678118654752440;
+  double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+  dft<4>(e, sign);
+  dft<4>(o, sign);
+  // twiddle o[q] by W8^q
+  o[1] = c2(h * (o[1].x - sign * o[1].y), h * (o[1].y + sign * o[1].x));
+  o[2] = mul_si(o[2], sign);
+  o[3] = c2(h * (-o[3].x - sign * o[3].y), h * (-o[3].y + sign * o[3].x));
+  for (int q = 0; q < 4; ++q) {
+    v[q] = cadd2(e[q], o[q]);
+    v[q + 4] = csub2(e[q], o[q]);
+  }
+}
+
+// One Stockham pass of radix R on x[n*B + b] (n < N, b < B), in place via
+// registers.  Ns = product of the radices already applied.
+template <int R, int MAXB>
+__device__ __forceinline__ void stockham_pass(double2 *x, int N, int B, int Ns, const double2 *__restrict__ tw,
+                                              int sign) {
+  const int NR = N / R;
+  const int nbf = NR * B;
+  const int T = blockDim.x;
+  double2 v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int t = threadIdx.x + q * T;
+    if (t < nbf) {
+      const int b = t % B, j = t / B;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = x[(j + r * NR) * B + b];
+    }
+  }
+  __syncthreads();
+  const int tstep = N / (Ns * R);
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int t = threadIdx.x + q * T;
+    if (t < nbf) {
+      const int b = t % B, j = t / B;
+      const int k = j % Ns;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        double2 w = tw[r * k * tstep];
+        if (sign > 0) w.y = -w.y;
+        v[q][r] = cmul2(v[q][r], w);
+      }
+      dft<R>(v[q], sign);
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[(base + r * Ns) * B + b] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Maximum elements (N*B) per block handled by lds_fft with 256 threads.
+constexpr int FFT_MAX_ELEMS = 8192;
+constexpr int FFT_THREADS = 256;
+
+__device__ void lds_fft(double2 *x, int B, const FftPlan &P, const double2 *__restrict__ tw, int sign) {
+  int Ns = 1;
+  for (int p = 0; p < P.np; ++p) {
+    switch (P.r[p]) {
+      case 2: stockham_pass<2, FFT_MAX_ELEMS / 2 / FFT_THREADS>(x, P.N, B, Ns, tw, sign); break;
+      case 3: stockham_pass<3, (FFT_MAX_ELEMS / 3 + FFT_THREADS - 1) / FFT_THREADS>(x, P.N, B, Ns, tw, sign); break;
+      case 4: stockham_pass<4, FFT_MAX_ELEMS / 4 / FFT_THREADS>(x, P.N, B, Ns, tw, sign); break;
+      case 5: stockham_pass<5, (FFT_MAX_ELEMS / 5 + FFT_THREADS - 1) / FFT_THREADS>(x, P.N, B, Ns, tw, sign); break;
+      case 8: stockham_pass<8, FFT_MAX_ELEMS / 8 / FFT_THREADS>(x, P.N, B, Ns, tw, sign); break;
+    }
+    Ns *= P.r[p];
+  }
+}
+
+// ------------------------------------------------------------------ z inverse
+// grid: (nxh/B, ny, nfield); the 4 state fields u, v, w, theta live in U
+// slots 0,1,2,6.  Output pad[f][y][z][kx].
+__global__ void __launch_bounds__(FFT_THREADS) zinv_kernel(const double2 *__restrict__ U, double2 *__restrict__ pad,
+                                                           FftPlan P, const double2 *__restrict__ tw, int nxh, int nz,
+                                                           int ny, int B, size_t field, size_t pfield) {
+  extern __shared__ double2 lds[];
+  const int kx0 = blockIdx.x * B, y = blockIdx.y, f = blockIdx.z;
+  const int slot = (f == 3) ? 6 : f;
+  const int N = P.N, nh = nz / 2;
+  const double2 *src = U + (size_t)slot * field + (size_t)y * nz * nxh + kx0;
+  for (int e = threadIdx.x; e < N * B; e += blockDim.x) {
+    const int z = e / B, b = e % B;
+    double2 val = c2(0.0, 0.0);
+    if (z < nh) val = src[(size_t)z * nxh + b];
+    else if (z > 2 * nh) val = src[(size_t)(z - nh) * nxh + b];
+    lds[e] = val;
+  }
+  __syncthreads();
+  lds_fft(lds, B, P, tw, +1);
+  double2 *dst = pad + (size_t)f * pfield + (size_t)y * N * nxh + kx0;
+  for (int e = threadIdx.x; e < N * B; e += blockDim.x) {
+    const int z = e / B, b = e % B;
+    dst[(size_t)z * nxh + b] = lds[e];
+  }
+}
+
+// ---------------------------------------------------------------- x physical
+// grid: (nz32, ny): one x line per block.  pad fields 0..3 = u,v,w,theta in,
+// 0..7 = spectra of u^2-v^2, uv, w^2-v^2, vw, uw, u theta, v theta, w theta out.
+__global__ void __launch_bounds__(FFT_THREADS) xphys_kernel(double2 *__restrict__ pad, FftPlan P,
+                                                            const double2 *__restrict__ tw, int nxh, int nz32,
+                                                            size_t pfield, const double *__restrict__ dyr,
+                                                            double dxr, double dzr, double *__restrict__ vpart) {
+  extern __shared__ double2 lds[];  // [n][2]
+  __shared__ double red[FFT_THREADS];
+  const int z = blockIdx.x, y = blockIdx.y, N = P.N;
+  double2 *line = pad + (size_t)y * nz32 * nxh + (size_t)z * nxh;
+  // Hermitian completion of (u + i v) and (w + i theta); FFTW c2r ignores Im of the DC bin
+  for (int m = threadIdx.x; m < N; m += blockDim.x) {
+    double2 z1 = c2(0.0, 0.0), z2 = c2(0.0, 0.0);
+    if (m == 0) {
+      z1 = c2(line[0].x, line[pfield].x);
+      z2 = c2(line[2 * pfield].x, line[3 * pfield].x);
+    } else if (m < nxh) {
+      const double2 a = line[m], b = line[pfield + m], c = line[2 * pfield + m], d = line[3 * pfield + m];
+      z1 = c2(a.x - b.y, a.y + b.x);
+      z2 = c2(c.x - d.y, c.y + d.x);
+    } else if (m > N - nxh) {
+      const int q = N - m;
+      const double2 a = line[q], b = line[pfield + q], c = line[2 * pfield + q], d = line[3 * pfield + q];
+      z1 = c2(a.x + b.y, -a.y + b.x);
+      z2 = c2(c.x + d.y, -c.y + d.x);
+    }
+    lds[2 * m] = z1;
+    lds[2 * m + 1] = z2;
+  }
+  __syncthreads();
+  lds_fft(lds, 2, P, tw, +1);
+  // products at this thread's points, kept in registers
+  constexpr int MAXP = FFT_MAX_ELEMS / 2 / FFT_THREADS;
+  double pu[MAXP], pv[MAXP], pw[MAXP], pt[MAXP];
+  const double dy = dyr[y];
+  double vmax = 0.0;
+#pragma unroll
+  for (int q = 0; q < MAXP; ++q) {
+    const int m = threadIdx.x + q * blockDim.x;
+    if (m < N) {
+      pu[q] = lds[2 * m].x; pv[q] = lds[2 * m].y; pw[q] = lds[2 * m + 1].x; pt[q] = lds[2 * m + 1].y;
+      const double vel = (fabs(pu[q]) * dxr) + (fabs(pv[q]) * dy) + (fabs(pw[q]) * dzr);
+      vmax = fmax(vel, vmax);
+    }
+  }
+  red[threadIdx.x] = vmax;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) vpart[(size_t)y * nz32 + z] = red[0];
+  // two pairs of products per round: (P1 + i P2, P3 + i P4), (P5 + i P7, P8 + i P9)
+  for (int round = 0; round < 2; ++round) {
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q) {
+      const int m = threadIdx.x + q * blockDim.x;
+      if (m < N) {
+        const double t1 = pu[q], t2 = pv[q], t3 = pw[q], t4 = pt[q];
+        const double bv2 = t2 * t2;
+        if (round == 0) {
+          lds[2 * m] = c2(t1 * t1 - bv2, t1 * t2);
+          lds[2 * m + 1] = c2(t3 * t3 - bv2, t2 * t3);
+        } else {
+          lds[2 * m] = c2(t1 * t3, t1 * t4);
+          lds[2 * m + 1] = c2(t2 * t4, t3 * t4);
+        }
+      }
+    }
+    __syncthreads();
+    lds_fft(lds, 2, P, tw, -1);
+    // unpack the two real spectra of each pair, keep modes 0..nxh-1
+    for (int m = threadIdx.x; m < nxh; m += blockDim.x) {
+      const int q = (m == 0) ? 0 : N - m;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const double2 Z = lds[2 * m + b], Zc = lds[2 * q + b];
+        const double2 A = c2((Z.x + Zc.x) * 0.5, (Z.y - Zc.y) * 0.5);
+        const double2 Bv = c2((Z.y + Zc.y) * 0.5, (Zc.x - Z.x) * 0.5);
+        const int fa = round * 4 + 2 * b;
+        line[(size_t)fa * pfield + m] = A;
+        line[(size_t)(fa + 1) * pfield + m] = Bv;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ z forward
+// grid: (nxh/B, ny, 8).  pad field g -> U slot (0,1,2,3,4,6,7,8)[g], with
+// truncation and rescale.
+__global__ void __launch_bounds__(FFT_THREADS) zfwd_kernel(const double2 *__restrict__ pad, double2 *__restrict__ U,
+                                                           FftPlan P, const double2 *__restrict__ tw, int nxh, int nz,
+                                                           int ny, int B, size_t field, size_t pfield,
+                                                           double rescale) {
+  extern __shared__ double2 lds[];
+  const int kx0 = blockIdx.x * B, y = blockIdx.y, g = blockIdx.z;
+  const int slot = g < 5 ? g : g + 1;
+  const int N = P.N, nh = nz / 2;
+  const double2 *src = pad + (size_t)g * pfield + (size_t)y * N * nxh + kx0;
+  for (int e = threadIdx.x; e < N * B; e += blockDim.x) {
+    const int z = e / B, b = e % B;
+    lds[e] = src[(size_t)z * nxh + b];
+  }
+  __syncthreads();
+  lds_fft(lds, B, P, tw, -1);
+  double2 *dst = U + (size_t)slot * field + (size_t)y * nz * nxh + kx0;
+  for (int e = threadIdx.x; e < nz * B; e += blockDim.x) {
+    const int j = e / B, b = e % B;
+    double2 val;
+    if (j < nh) val = lds[j * B + b];
+    else if (j == nh) val = c2(0.0, 0.0);
+    else val = lds[(j + nh) * B + b];
+    dst[(size_t)j * nxh + b] = c2(val.x * rescale, val.y * rescale);
+  }
+}
+
+// ------------------------------------------------- CFL maximum and dt update
+// single block: velmax = max(vpart); on substep 1 (update != 0)
+// dt = dt_factor*dt + (1-dt_factor)*cflx/velmax  (dft_x_realspace.f90:489-498)
+__global__ void velmax_kernel(const double *__restrict__ vpart, size_t n, double *__restrict__ scal, int update,
+                              double dt_factor, double cflx) {
+  __shared__ double red[256];
+  double v = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, vpart[i]);
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    scal[1] = red[0];
+    if (update) scal[0] = dt_factor * scal[0] + (1.0 - dt_factor) * cflx / red[0];
+  }
+}
+
+// ------------------------------------------------------------------ wavespace
+__global__ void __launch_bounds__(64) wavespace_kernel(LineParams P, WsArgs A, size_t nsys, const double *scal) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  LineParams Q = P;
+  Q.dt = scal[0];
+  wavespace_line(Q, A, sys);
+}
+
+// opposition-control detection planes: vd[0][sys], vd[1][sys]
+__global__ void __launch_bounds__(64) oppvd_kernel(LineParams P, const cplx *U, cplx *buf, cplx *vd, size_t plane,
+                                                   size_t nsys, int first_lo, int first_hi, const double *rows, int k) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const SV v{const_cast<cplx *>(U) + (size_t)1 * plane * P.g.n + sys, plane};
+  const SV b{buf + sys, plane};
+  vd[sys] = interp_plane(P, v, b, first_lo, rows, k);
+  vd[plane + sys] = interp_plane(P, v, b, first_hi, rows + k, k);
+}
+
+// ------------------------------------------------------- layout conversions
+// reference (slot, kz, kx, y) [y fastest]  <->  device (slot, y, kz, kx)
+// one block per (slot, kz); 32x32 tiles through LDS.
+__global__ void ref_to_dev_kernel(const double2 *__restrict__ in, double2 *__restrict__ out, int ny, int nxh, int nz,
+                                  const int *slot_map) {
+  __shared__ double2 tile[32][33];
+  const int j = blockIdx.x, sl = blockIdx.y;
+  const int dslot = slot_map[sl];
+  const double2 *src = in + ((size_t)sl * nz + j) * nxh * ny;  // [kx][y]
+  double2 *dst = out + (size_t)dslot * ny * nz * nxh + (size_t)j * nxh;  // [y][(plane)] + kx
+  const size_t plane = (size_t)nz * nxh;
+  for (int k0 = 0; k0 < nxh; k0 += 32)
+    for (int y0 = 0; y0 < ny; y0 += 32) {
+      for (int t = threadIdx.x; t < 32 * 32; t += blockDim.x) {
+        const int kk = t / 32, yy = t % 32;
+        if (k0 + kk < nxh && y0 + yy < ny) tile[kk][yy] = src[(size_t)(k0 + kk) * ny + y0 + yy];
+      }
+      __syncthreads();
+      for (int t = threadIdx.x; t < 32 * 32; t += blockDim.x) {
+        const int yy = t / 32, kk = t % 32;
+        if (k0 + kk < nxh && y0 + yy < ny) dst[(size_t)(y0 + yy) * plane + k0 + kk] = tile[kk][yy];
+      }
+      __syncthreads();
+    }
+}
+
+__global__ void dev_to_ref_kernel(const double2 *__restrict__ in, double2 *__restrict__ out, int ny, int nxh, int nz,
+                                  const int *slot_map) {
+  __shared__ double2 tile[32][33];
+  const int j = blockIdx.x, sl = blockIdx.y;
+  const int dslot = slot_map[sl];
+  double2 *dst = out + ((size_t)sl * nz + j) * nxh * ny;
+  const double2 *src = in + (size_t)dslot * ny * nz * nxh + (size_t)j * nxh;
+  const size_t plane = (size_t)nz * nxh;
+  for (int k0 = 0; k0 < nxh; k0 += 32)
+    for (int y0 = 0; y0 < ny; y0 += 32) {
+      for (int t = threadIdx.x; t < 32 * 32; t += blockDim.x) {
+        const int yy = t / 32, kk = t % 32;
+        if (k0 + kk < nxh && y0 + yy < ny) tile[kk][yy] = src[(size_t)(y0 + yy) * plane + k0 + kk];
+      }
+      __syncthreads();
+      for (int t = threadIdx.x; t < 32 * 32; t += blockDim.x) {
+        const int kk = t / 32, yy = t % 32;
+        if (k0 + kk < nxh && y0 + yy < ny) dst[(size_t)(k0 + kk) * ny + y0 + yy] = tile[kk][yy];
+      }
+      __syncthreads();
+    }
+}
+
+// stores from the state (set_initial_data_uvw, initial_step.f90:139-179)
+__global__ void __launch_bounds__(64) init_stores_kernel(LineParams P, cplx *U, cplx *RK, cplx *buf, size_t plane,
+                                                         size_t nsys, int nxh, int nh, double lmfxr, double lmfzr) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const int n = P.g.n;
+  const size_t field = plane * n;
+  const int k = (int)(sys % nxh), j = (int)(sys / nxh);
+  const int kzi = (j <= nh) ? j : j + 1 - 2 * nh - 1;
+  SV u{U + sys, plane}, v{U + field + sys, plane}, w{U + 2 * field + sys, plane}, th{U + 6 * field + sys, plane};
+  SV r1{RK + sys, plane}, r2{RK + field + sys, plane}, r3{RK + 2 * field + sys, plane}, d{buf + sys, plane};
+  if (k == 0 && kzi == 0) {
+    for (int i = 0; i < n; ++i) { r1[i] = u[i]; r2[i] = w[i]; r3[i] = th[i]; }
+    return;
+  }
+  for (int i = 0; i < n; ++i) d[i] = v[i];
+  band_bs(P.g, P.D0LU, d);
+  band_mv(P.g, P.D2, d, r2);
+  const double kxp = k * lmfxr, kzp = kzi * lmfzr;
+  const double k2 = kxp * kxp + kzp * kzp;
+  const cplx ikz = mk(0.0, kzp), ikx = mk(0.0, kxp);
+  for (int i = 0; i < n; ++i) {
+    r2[i] = csub(r2[i], cscale(v[i], k2));
+    r1[i] = csub(cmul(ikz, u[i]), cmul(ikx, w[i]));
+  }
+  for (int i = 0; i < n; ++i) r3[i] = th[i];
+}
+
+}  // namespace pb

@@ -1,0 +1,451 @@
+// solver.h -- per-(kx,kz) wall-normal solvers of one RK3-CN substep.
+//
+// One call handles one (kx,kz) system (one GPU thread).  Every vector is a
+// column of ny complex values in the device layout [y][kz][kx] (kx fastest),
+// so consecutive threads touch consecutive addresses at every y.  The code
+// is host+device so that a CPU build of the very same functions can be
+// compared bit-for-bit with the oracle.  Arithmetic follows the reference
+// statement by statement (SOURCE_PASSOPB/hvhg.f90, solve_g.f90, solve_t.f90,
+// solve_v_f.f90, solve_u_w.f90, wave00.f90) and the packed-band kernels of
+// module_labs.f90; compile with FP contraction off.
+#pragma once
+
+#include "pb_common.h"
+
+namespace pb {
+
+// strided complex column
+struct SV {
+  cplx *p;
+  size_t s;
+  PB_HD cplx &operator[](int i) const { return p[(size_t)i * s]; }
+};
+
+// band read from a shared (k-independent) matrix: A[r*W + slot]
+struct SharedBand {
+  const double *a;
+  int W;
+  PB_HD double operator()(int r, int slot) const { return a[r * W + slot]; }
+};
+
+// per-system band: element (r, slot) at p[(r*W + slot) * s]
+struct OwnBand {
+  double *p;
+  size_t s;
+  int W;
+  PB_HD double &at(int r, int slot) const { return p[(size_t)(r * W + slot) * s]; }
+  PB_HD double operator()(int r, int slot) const { return p[(size_t)(r * W + slot) * s]; }
+};
+
+// ---------------------------------------------------------------- LABS ops
+// No-pivot Crout (fixed_bandec*, module_labs.f90:943-1198, 3487-3636):
+// L(r,c) = A(r,c) - sum_{p=c-1 down} L(r,p) U(p,c);
+// U(r,c) = (1/L(r,r)) * (A(r,c) - sum_{p=r-1 down} L(r,p) U(p,c)).
+PB_HD void band_lu(const BandGeom &g, const OwnBand &A) {
+  for (int r = 0; r < g.n; ++r) {
+    const int f = g.first(r);
+    for (int c = f; c <= r; ++c) {
+      double v = A(r, g.slot(r, c));
+      for (int p = c - 1; p >= f; --p) {
+        if (c > g.last(p)) break;
+        v = v - A(r, g.slot(r, p)) * A(p, g.slot(p, c));
+      }
+      A.at(r, g.slot(r, c)) = v;
+    }
+    const double inv = 1.0 / A(r, g.bw);
+    for (int c = r + 1; c <= g.last(r); ++c) {
+      double v = A(r, g.slot(r, c));
+      for (int p = r - 1; p >= f; --p) {
+        if (c > g.last(p)) break;
+        v = v - A(r, g.slot(r, p)) * A(p, g.slot(p, c));
+      }
+      A.at(r, g.slot(r, c)) = inv * v;
+    }
+  }
+}
+
+// forward (farthest first, divide by the pivot) and backward (nearest first)
+// substitution (fixed_banbks*, module_labs.f90:3769-3823, 4252-4318)
+template <class M>
+PB_HD void band_bs(const BandGeom &g, const M &A, const SV &b) {
+  for (int r = 0; r < g.n; ++r) {
+    cplx v = b[r];
+    for (int p = g.first(r); p < r; ++p) v = csub(v, cscale(b[p], A(r, g.slot(r, p))));
+    b[r] = cdivr(v, A(r, g.bw));
+  }
+  for (int r = g.n - 1; r >= 0; --r) {
+    cplx v = b[r];
+    for (int c = r + 1; c <= g.last(r); ++c) v = csub(v, cscale(b[c], A(r, g.slot(r, c))));
+    b[r] = v;
+  }
+}
+
+// b = A x, ascending column order (LABS_mat_mul_4, module_labs.f90:4395-4430)
+PB_HD void band_mv(const BandGeom &g, const SharedBand &A, const SV &x, const SV &b) {
+  for (int r = 0; r < g.n; ++r) {
+    const int f = g.first(r), l = g.last(r);
+    cplx v = cscale(x[f], A(r, g.slot(r, f)));
+    for (int c = f + 1; c <= l; ++c) v = cadd(v, cscale(x[c], A(r, g.slot(r, c))));
+    b[r] = v;
+  }
+}
+
+// -------------------------------------------------------------- line state
+struct LineParams {
+  BandGeom g;
+  SharedBand D0, D1, D2, D0LU;
+  const double *integ;  // LABS_INTEGRATE
+  double rk[4][3];
+  double dt, nu, pr, bulk_u, bulk_w, bulk_t, u_gal;
+  int s, S;  // substep (1-based), steps per RK step
+  PB_HD double coef(int row, int sub) const { return rk[row][sub - 1]; }
+};
+
+// Helmholtz LHS with diagonal-only wall rows (solve_g.f90:42-71, wave00.f90:58-82)
+PB_HD void helmholtz_lu(const LineParams &P, const OwnBand &Db, double k2, double beta, bool mean) {
+  const int n = P.g.n, W = P.g.W(), m = P.g.bw;
+  const double dummy = 1.0 + beta * k2;
+  for (int i = 0; i < W; ++i) Db.at(0, i) = 0.0;
+  Db.at(0, m) = mean ? (P.D0(0, m) - beta * P.D2(0, m)) : (dummy * P.D0(0, m) - beta * P.D2(0, m));
+  for (int j = 1; j < n - 1; ++j)
+    for (int i = 0; i < W; ++i)
+      Db.at(j, i) = mean ? (P.D0(j, i) - beta * P.D2(j, i)) : (dummy * P.D0(j, i) - beta * P.D2(j, i));
+  for (int i = 0; i < W; ++i) Db.at(n - 1, i) = 0.0;
+  Db.at(n - 1, m) = mean ? (P.D0(n - 1, m) - beta * P.D2(n - 1, m)) : (dummy * P.D0(n - 1, m) - beta * P.D2(n - 1, m));
+  band_lu(P.g, Db);
+}
+
+// hvhg.f90:141-276
+PB_HD void hvhg(const LineParams &P, SV u1, SV u2, SV u3, SV u4, SV u5, SV u7, SV u8, SV u9, double kx, double kz) {
+  const int n = P.g.n;
+  const double kz2 = kz * kz, kx2 = kx * kx, kxkz = kx * kz;
+  const cplx ikx = iunit(kx), ikz = iunit(kz);
+  const cplx ikx3 = iunit(kx * kx * kx), ikxkz2 = iunit(kx * (kz * kz));
+  const cplx ikx2kz = iunit((kx * kx) * kz), ikz3 = iunit(kz * kz * kz);
+  for (int i = 0; i < n; ++i) {
+    const cplx d = u1[i];
+    u1[i] = cadd(cscale(csub(u1[i], u3[i]), kxkz), cscale(u5[i], kz2 - kx2));
+    u3[i] = csub(csub(cscale(d, -kx2), cscale(u5[i], 2.0 * kxkz)), cscale(u3[i], kz2));
+  }
+  band_bs(P.g, P.D0LU, u3);
+  band_mv(P.g, P.D1, u3, u5);
+  const cplx a = cadd(ikx3, ikxkz2), b = cadd(ikx2kz, ikz3);
+  for (int i = 0; i < n; ++i) u5[i] = cadd(cadd(u5[i], cmul(a, u2[i])), cmul(b, u4[i]));
+  for (int i = 0; i < n; ++i) {
+    const cplx t3 = csub(cmul(ikx, u4[i]), cmul(ikz, u2[i]));
+    const cplx t4 = cadd(cmul(ikz, u4[i]), cmul(ikx, u2[i]));
+    u3[i] = t3;
+    u4[i] = t4;
+  }
+  band_bs(P.g, P.D0LU, u3);
+  band_mv(P.g, P.D1, u3, u2);
+  band_bs(P.g, P.D0LU, u4);
+  band_mv(P.g, P.D2, u4, u3);
+  for (int i = 0; i < n; ++i) {
+    u5[i] = cadd(u5[i], u3[i]);
+    u1[i] = cadd(u1[i], u2[i]);
+  }
+  for (int i = 0; i < n; ++i) u7[i] = cadd(cmul(ikx, u7[i]), cmul(ikz, u9[i]));
+  band_bs(P.g, P.D0LU, u8);
+  band_mv(P.g, P.D1, u8, u9);
+  for (int i = 0; i < n; ++i) u7[i] = cneg(cadd(u7[i], u9[i]));
+}
+
+// solve_g.f90:28-201 (scalar == false) and solve_t.f90:28-201 (scalar == true)
+PB_HD void solve_gt(const LineParams &P, const OwnBand &Db, SV u1, SV store, SV buf1, SV buf2, double kx, double kz,
+                    bool scalar) {
+  const int n = P.g.n, s = P.s, S = P.S;
+  const double k2 = kx * kx + kz * kz, dt = P.dt;
+  auto visc = [&](double cf) { return scalar ? dt * cf * P.nu / P.pr : dt * cf * P.nu; };
+  helmholtz_lu(P, Db, k2, visc(P.coef(1, s)), false);
+  if (s == 1) {
+    const double alpha = visc(P.coef(0, s)), gamma = dt * P.coef(2, s);
+    const double dummy = 1.0 - alpha * k2;
+    for (int i = 0; i < n; ++i) buf1[i] = cadd(cscale(store[i], dummy), cscale(u1[i], gamma));
+    band_bs(P.g, P.D0LU, store);
+    band_mv(P.g, P.D2, store, buf2);
+    for (int i = 0; i < n; ++i) buf1[i] = cadd(buf1[i], cscale(buf2[i], alpha));
+  } else {
+    const double gamma = dt * P.coef(2, s);
+    for (int i = 0; i < n; ++i) buf1[i] = cadd(store[i], cscale(u1[i], gamma));
+  }
+  buf1[0] = mk(0.0, 0.0);
+  buf1[n - 1] = mk(0.0, 0.0);
+  if (s != S) {
+    const double zeta = dt * P.coef(3, s + 1);
+    for (int i = 0; i < n; ++i) store[i] = cscale(u1[i], zeta);
+  }
+  band_bs(P.g, Db, buf1);
+  band_mv(P.g, P.D0, buf1, u1);
+  band_mv(P.g, P.D2, buf1, buf2);
+  if (s != S) {
+    const double alpha = visc(P.coef(0, s + 1));
+    const double dummy = 1.0 - alpha * k2;
+    for (int i = 0; i < n; ++i) store[i] = cadd(cadd(store[i], cscale(u1[i], dummy)), cscale(buf2[i], alpha));
+  } else {
+    for (int i = 0; i < n; ++i) store[i] = u1[i];
+  }
+}
+
+// solve_v_f.f90:38-375; vlo/vhi = wall values of v (0 without opposition control)
+PB_HD void solve_v_f(const LineParams &P, const OwnBand &Db, SV v, SV dv, SV hv, SV sphi, SV buf1, SV buf2, SV buf3,
+                     double kx, double kz, cplx vlo, cplx vhi) {
+  const int n = P.g.n, s = P.s, S = P.S, W = P.g.W(), m = P.g.bw;
+  const double k2 = kx * kx + kz * kz, dt = P.dt, nu = P.nu;
+  if (s == 1) {
+    const double alpha = dt * P.coef(0, s) * nu, gamma = dt * P.coef(2, s);
+    const double dummy = 1.0 - alpha * k2;
+    for (int i = 0; i < n; ++i) buf1[i] = cadd(cscale(sphi[i], dummy), cscale(hv[i], gamma));
+    band_bs(P.g, P.D0LU, sphi);
+    band_mv(P.g, P.D2, sphi, buf2);
+    for (int i = 0; i < n; ++i) buf1[i] = cadd(buf1[i], cscale(buf2[i], alpha));
+  } else {
+    const double gamma = dt * P.coef(2, s);
+    for (int i = 0; i < n; ++i) buf1[i] = cadd(sphi[i], cscale(hv[i], gamma));
+  }
+  buf1[0] = mk(0.0, 0.0);
+  buf1[n - 1] = mk(0.0, 0.0);
+  if (s != S) {
+    const double zeta = dt * P.coef(3, s + 1);
+    for (int i = 0; i < n; ++i) sphi[i] = cscale(hv[i], zeta);
+  }
+  const double d00 = Db(0, m);
+  buf2[0] = mk(d00, d00);
+  for (int i = 1; i < n; ++i) buf2[i] = mk(0.0, 0.0);
+  band_bs(P.g, Db, buf1);
+  band_bs(P.g, Db, buf2);
+  band_mv(P.g, P.D0, buf1, hv);
+  band_mv(P.g, P.D0, buf2, buf3);
+  band_mv(P.g, P.D2, buf1, dv);
+  if (s != S) {
+    const double alpha = dt * nu * P.coef(0, s + 1);
+    const double dummy = 1.0 - alpha * k2;
+    for (int i = 0; i < n; ++i) sphi[i] = cadd(cadd(sphi[i], cscale(hv[i], dummy)), cscale(dv[i], alpha));
+  } else {
+    for (int i = 0; i < n; ++i) sphi[i] = hv[i];
+  }
+  // Poisson LHS [D2 - k2 D0] with D0-diagonal wall rows (solve_v_f.f90:205-230)
+  for (int i = 0; i < W; ++i) Db.at(0, i) = 0.0;
+  Db.at(0, m) = P.D0(0, m);
+  for (int j = 1; j < n - 1; ++j)
+    for (int i = 0; i < W; ++i) Db.at(j, i) = -k2 * P.D0(j, i) + P.D2(j, i);
+  for (int i = 0; i < W; ++i) Db.at(n - 1, i) = 0.0;
+  Db.at(n - 1, m) = P.D0(n - 1, m);
+  band_lu(P.g, Db);
+  hv[0] = vlo;
+  hv[n - 1] = vhi;
+  band_bs(P.g, Db, hv);
+  for (int i = 0; i < n; ++i) buf1[i] = buf3[i];
+  buf1[0] = mk(0.0, 0.0);
+  buf1[n - 1] = mk(0.0, 0.0);
+  band_bs(P.g, Db, buf1);
+  band_mv(P.g, P.D0, hv, v);
+  band_mv(P.g, P.D1, hv, dv);
+  band_mv(P.g, P.D1, buf1, hv);
+  const cplx h0 = hv[0], hn = hv[n - 1], d0 = dv[0], dn = dv[n - 1];
+  const cplx den = csub(cmul(h0, h0), cmul(hn, hn));
+  const cplx c1 = cdiv(csub(cmul(hn, dn), cmul(h0, d0)), den);
+  const cplx c2 = cdiv(csub(cmul(h0, dn), cmul(hn, d0)), den);
+  for (int i = 0; i < n; ++i) dv[i] = csub(cadd(dv[i], cmul(c1, hv[i])), cmul(c2, hv[n - 1 - i]));
+  band_mv(P.g, P.D2, buf2, hv);
+  if (s != S) {
+    const double alpha = dt * nu * P.coef(0, s + 1);
+    const double dummy = 1.0 - alpha * k2;
+    const cplx c1d = cscale(c1, dummy), c2d = cscale(c2, dummy), c1a = cscale(c1, alpha), c2a = cscale(c2, alpha);
+    for (int i = 0; i < n; ++i) {
+      cplx acc = cadd(sphi[i], cmul(c1d, buf3[i]));
+      acc = cadd(acc, cmul(c2d, buf3[n - 1 - i]));
+      acc = cadd(acc, cmul(c1a, hv[i]));
+      acc = cadd(acc, cmul(c2a, hv[n - 1 - i]));
+      sphi[i] = acc;
+    }
+  } else {
+    for (int i = 0; i < n; ++i) sphi[i] = cadd(cadd(sphi[i], cmul(c1, buf3[i])), cmul(c2, buf3[n - 1 - i]));
+  }
+  band_mv(P.g, P.D0, buf1, hv);
+  for (int i = 0; i < n; ++i) v[i] = cadd(cadd(v[i], cmul(c1, hv[i])), cmul(c2, hv[n - 1 - i]));
+}
+
+// solve_u_w.f90:18-42 (g -> u, f -> w)
+PB_HD void solve_u_w(int n, SV g, SV f, double kx, double kz) {
+  const cplx ci = mk(0.0, 1.0);
+  const double den = kx * kx + kz * kz;
+  const cplx kxx = cdivr(cmul(ci, mk(kx, 0.0)), den), kzz = cdivr(cmul(ci, mk(kz, 0.0)), den);
+  for (int i = 0; i < n; ++i) {
+    const cplx dxg = cmul(kxx, g[i]), dzg = cmul(kzz, g[i]);
+    const cplx dxf = cmul(kxx, f[i]), dzf = cmul(kzz, f[i]);
+    g[i] = csub(dxf, dzg);
+    f[i] = cadd(dxg, dzf);
+  }
+}
+
+// LABS_integration (module_labs.f90:401-413)
+PB_HD cplx integrate(const LineParams &P, const SV &src) {
+  cplx t = mk(0.0, 0.0);
+  for (int i = 0; i < P.g.n; ++i) t = cadd(t, cscale(src[i], P.integ[i]));
+  return t;
+}
+
+// wave00.f90:41-685 with calculate_phi (704-714); out4 = phi_u, phi_t, tau_w, q_w
+PB_HD void wave00(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, SV u4, SV u5, SV u7, SV u8, SV u9,
+                  SV st1, SV st2, SV st3, SV buf1, double *out4) {
+  const int n = P.g.n, s = P.s, S = P.S, W = P.g.W(), m = P.g.bw;
+  const double dt = P.dt, nu = P.nu, pr = P.pr;
+  auto calc_phi = [](double bulk, cplx I) { return (bulk - 0.5 * I.re) / (0.5 * I.im); };
+  auto wall_d1 = [&](const SV &c) {
+    return (((P.D1(0, m) * c[0].re + P.D1(0, m + 1) * c[1].re) - P.D1(n - 1, m - 1) * c[n - 2].re) -
+            P.D1(n - 1, m) * c[n - 1].re);
+  };
+  (void)W;
+  // ---- U
+  helmholtz_lu(P, Db, 0.0, dt * P.coef(1, s) * nu, true);
+  if (s == 1) {
+    for (int i = 0; i < n; ++i) u1[i] = st1[i];
+    band_bs(P.g, P.D0LU, u1);
+    band_bs(P.g, P.D0LU, u2);
+    band_mv(P.g, P.D2, u1, buf1);
+    band_mv(P.g, P.D1, u2, u3);
+    const double alpha = dt * P.coef(0, s) * nu, gamma = dt * P.coef(2, s);
+    for (int i = 1; i < n; ++i) buf1[i] = csub(cadd(st1[i], cscale(buf1[i], alpha)), cscale(u3[i], gamma));
+  } else {
+    band_bs(P.g, P.D0LU, u2);
+    band_mv(P.g, P.D1, u2, u3);
+    const double gamma = dt * P.coef(2, s);
+    for (int i = 1; i < n; ++i) buf1[i] = csub(st1[i], cscale(u3[i], gamma));
+  }
+  buf1[0] = mk(0.0, 0.0);
+  for (int i = 1; i < n - 1; ++i) buf1[i] = mk(buf1[i].re + P.u_gal, 1.0);
+  buf1[n - 1] = mk(0.0, 0.0);
+  band_bs(P.g, Db, buf1);
+  double phi = calc_phi(P.bulk_u, integrate(P, buf1));
+  for (int i = 0; i < n; ++i) buf1[i] = mk(buf1[i].re + phi * buf1[i].im, 0.0);
+  out4[0] = phi;
+  if (s == S) out4[2] = 0.5 * nu * wall_d1(buf1);
+  band_mv(P.g, P.D0, buf1, u1);
+  for (int i = 0; i < n; ++i) u1[i] = mk(u1[i].re - P.u_gal, u1[i].im);
+  if (s != S) {
+    band_mv(P.g, P.D2, buf1, st1);
+    const double alpha = dt * P.coef(0, s + 1) * nu, zeta = dt * P.coef(3, s + 1);
+    for (int i = 0; i < n; ++i) st1[i] = csub(cadd(cscale(st1[i], alpha), u1[i]), cscale(u3[i], zeta));
+  } else {
+    for (int i = 0; i < n; ++i) st1[i] = u1[i];
+  }
+  // ---- W
+  if (s == 1) {
+    for (int i = 0; i < n; ++i) u2[i] = st2[i];
+    band_bs(P.g, P.D0LU, u2);
+    band_mv(P.g, P.D2, u2, buf1);
+    band_bs(P.g, P.D0LU, u4);
+    band_mv(P.g, P.D1, u4, u5);
+    const double alpha = dt * P.coef(0, s) * nu, gamma = dt * P.coef(2, s);
+    for (int i = 1; i < n; ++i) buf1[i] = csub(cadd(st2[i], cscale(buf1[i], alpha)), cscale(u5[i], gamma));
+  } else {
+    band_bs(P.g, P.D0LU, u4);
+    band_mv(P.g, P.D1, u4, u5);
+    const double gamma = dt * P.coef(2, s);
+    for (int i = 1; i < n; ++i) buf1[i] = csub(st2[i], cscale(u5[i], gamma));
+  }
+  buf1[0] = mk(0.0, 0.0);
+  for (int i = 1; i < n - 1; ++i) buf1[i] = mk(buf1[i].re, 1.0);
+  buf1[n - 1] = mk(0.0, 0.0);
+  band_bs(P.g, Db, buf1);
+  phi = calc_phi(P.bulk_w, integrate(P, buf1));
+  phi = 0.0;  // wave00.f90:396
+  for (int i = 0; i < n; ++i) buf1[i] = mk(buf1[i].re + phi * buf1[i].im, 0.0);
+  band_mv(P.g, P.D0, buf1, u3);
+  if (s != S) {
+    band_mv(P.g, P.D2, buf1, st2);
+    const double alpha = dt * P.coef(0, s + 1) * nu, zeta = dt * P.coef(3, s + 1);
+    for (int i = 0; i < n; ++i) st2[i] = csub(cadd(cscale(st2[i], alpha), u3[i]), cscale(u5[i], zeta));
+  } else {
+    for (int i = 0; i < n; ++i) st2[i] = u3[i];
+  }
+  for (int i = 0; i < n; ++i) u2[i] = mk(0.0, 0.0);
+  // ---- Theta
+  helmholtz_lu(P, Db, 0.0, dt * P.coef(1, s) * nu / pr, true);
+  if (s == 1) {
+    for (int i = 0; i < n; ++i) u7[i] = st3[i];
+    band_bs(P.g, P.D0LU, u7);
+    band_mv(P.g, P.D2, u7, buf1);
+    band_bs(P.g, P.D0LU, u8);
+    band_mv(P.g, P.D1, u8, u9);
+    const double alpha = dt * P.coef(0, s) * nu / pr, gamma = dt * P.coef(2, s);
+    for (int i = 1; i < n; ++i) buf1[i] = csub(cadd(st3[i], cscale(buf1[i], alpha)), cscale(u9[i], gamma));
+  } else {
+    band_bs(P.g, P.D0LU, u8);
+    band_mv(P.g, P.D1, u8, u9);
+    const double gamma = dt * P.coef(2, s);
+    for (int i = 1; i < n; ++i) buf1[i] = csub(st3[i], cscale(u9[i], gamma));
+  }
+  buf1[0] = mk(0.0, 0.0);
+  for (int i = 1; i < n - 1; ++i) buf1[i] = mk(buf1[i].re, 1.0);
+  buf1[n - 1] = mk(0.0, 0.0);
+  band_bs(P.g, Db, buf1);
+  phi = calc_phi(P.bulk_t, integrate(P, buf1));
+  for (int i = 0; i < n; ++i) buf1[i] = mk(buf1[i].re + phi * buf1[i].im, 0.0);
+  out4[1] = phi;
+  if (s == S) out4[3] = 0.5 * nu / pr * wall_d1(buf1);
+  band_mv(P.g, P.D0, buf1, u7);
+  if (s != S) {
+    band_mv(P.g, P.D2, buf1, st3);
+    const double alpha = dt * P.coef(0, s + 1) * nu / pr, zeta = dt * P.coef(3, s + 1);
+    for (int i = 0; i < n; ++i) st3[i] = csub(cadd(cscale(st3[i], alpha), u7[i]), cscale(u9[i], zeta));
+  } else {
+    for (int i = 0; i < n; ++i) st3[i] = u7[i];
+  }
+}
+
+// Detection-plane value of a collocation column: c = D0^{-1} v (in buf),
+// v(y_d) = sum_j B_j(y_d) c_j, ascending j.
+PB_HD cplx interp_plane(const LineParams &P, const SV &v, const SV &buf, int first, const double *vals, int k) {
+  for (int i = 0; i < P.g.n; ++i) buf[i] = v[i];
+  band_bs(P.g, P.D0LU, buf);
+  cplx acc = cscale(buf[first], vals[0]);
+  for (int j = 1; j < k; ++j) acc = cadd(acc, cscale(buf[first + j], vals[j]));
+  return acc;
+}
+
+// Everything one system needs: field base pointers in the [slot][y][kz][kx]
+// layout and per-system scratch.
+struct WsArgs {
+  cplx *U;        // 9 slots: products in, u,v,w,theta (slots 0,1,2,6) out
+  cplx *RK;       // 3 slots
+  cplx *buf;      // 2 scratch columns per system
+  double *dbuf;   // W*ny doubles per system
+  const cplx *vd; // detection-plane v (2 planes) or nullptr
+  size_t plane;   // nz*nxh
+  size_t field;   // ny*plane
+  int nxh, nz, nh;
+  double lmfxr, lmfzr, opp_alpha;
+  double *mean_out;  // 4 doubles written by the (0,0) system
+};
+
+PB_HD void wavespace_line(const LineParams &P, const WsArgs &A, size_t sys) {
+  const int k = (int)(sys % A.nxh), j = (int)(sys / A.nxh);
+  const int kxi = k, kzi = (j <= A.nh) ? j : j + 1 - 2 * A.nh - 1;  // module_ppdfftw.f90:360-374
+  auto col = [&](cplx *base, int slot) { return SV{base + (size_t)slot * A.field + sys, A.plane}; };
+  OwnBand Db{A.dbuf + sys, A.plane, P.g.W()};
+  SV b1 = col(A.buf, 0), b2 = col(A.buf, 1);
+  if (kxi == 0 && kzi == 0) {
+    double o4[4] = {0, 0, 0, 0};
+    wave00(P, Db, col(A.U, 0), col(A.U, 1), col(A.U, 2), col(A.U, 3), col(A.U, 4), col(A.U, 6), col(A.U, 7),
+           col(A.U, 8), col(A.RK, 0), col(A.RK, 1), col(A.RK, 2), b1, o4);
+    if (A.mean_out)
+      for (int q = 0; q < 4; ++q) A.mean_out[q] = o4[q];
+    return;
+  }
+  const double kx = kxi * A.lmfxr, kz = kzi * A.lmfzr;
+  hvhg(P, col(A.U, 0), col(A.U, 1), col(A.U, 2), col(A.U, 3), col(A.U, 4), col(A.U, 6), col(A.U, 7), col(A.U, 8), kx,
+       kz);
+  solve_gt(P, Db, col(A.U, 0), col(A.RK, 0), b1, b2, kx, kz, false);
+  cplx vlo = mk(0.0, 0.0), vhi = mk(0.0, 0.0);
+  if (A.opp_alpha != 0.0 && A.vd) {
+    vlo = cscale(A.vd[sys], -A.opp_alpha);
+    vhi = cscale(A.vd[A.plane + sys], -A.opp_alpha);
+  }
+  solve_v_f(P, Db, col(A.U, 1), col(A.U, 2), col(A.U, 4), col(A.RK, 1), col(A.U, 3), b1, b2, kx, kz, vlo, vhi);
+  solve_u_w(P.g.n, col(A.U, 0), col(A.U, 2), kx, kz);
+  solve_gt(P, Db, col(A.U, 6), col(A.RK, 2), b1, b2, kx, kz, true);
+}
+
+}  // namespace pb

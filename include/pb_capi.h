@@ -1,0 +1,158 @@
+/*
+ * pb_capi.h -- C ABI of the MI355X (gfx950) RK3-CN channel time step.
+ *
+ * The library replaces the per-substep call chain of the PoongBack channel
+ * DNS (reference: SOURCE_PASSOPB/time_march.f90:36-128) on one GPU.  Every
+ * entry point takes plain pointers and sizes, returns 0 on success and a
+ * negative PB_E* code on failure (pb_last_error() then describes it); no
+ * entry point aborts the process.
+ *
+ * Spectral layout at the boundary (identical to the reference y-pencil):
+ *   U(ny, nx/2, nz, nslot)   Fortran order, complex(c_double_complex)
+ *   i.e. element (y, kx, kz, slot) at ((slot*nz + kz)*(nx/2) + kx)*ny + y,
+ *   interleaved (re, im) doubles.  kx = 0 .. nx/2-1; kz index j maps to
+ *   0 .. nz/2, -(nz/2-1) .. -1 (module_ppdfftw.f90:360-374).  Values are
+ *   collocation-point values, not B-spline coefficients (hvhg.f90:68-69).
+ *   RK_STORE(ny, nx/2, nz, 3): slot 0 = g (mean U at kx=kz=0), slot 1 = phi
+ *   (mean W), slot 2 = theta store (mean Theta)  (module_ppdfftw.f90:304).
+ *
+ * Inside the library the state is device resident and laid out
+ * (slot, y, kz, kx) with kx fastest; conversions happen only in
+ * pb_set_state / pb_get_state and the host-array drop-ins below.
+ *
+ * One context must not be used from two threads at once; all device work
+ * of a context runs on one HIP stream.
+ */
+#ifndef PB_CAPI_H
+#define PB_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PB_OK            0
+#define PB_E_ARG        -1   /* invalid argument / size */
+#define PB_E_HIP        -2   /* HIP runtime error (no device, OOM, launch) */
+#define PB_E_NUMERIC    -3   /* dt out of range (dft_x_realspace.f90:494) */
+#define PB_E_STATE      -4   /* call out of order (e.g. substep before set_state) */
+
+/* knot maps for the B-spline breakpoints (labs_fill_full.cpp:80-101) */
+#define PB_KNOTS_REFERENCE 0 /* uniform if |stretch| <= 1e-16, else sin map (line 92) */
+#define PB_KNOTS_TANH      1 /* 0.5*Ly*tanh(s*xi)/tanh(s)  (line 89, commented out in the reference) */
+
+/* Physical and numerical parameters.  Mirrors the `input` file records that
+ * reach the time step (read_input_file.f90:39-116) plus the opposition
+ * control the reference leaves commented out (solve_v_f.f90:253-257). */
+typedef struct pb_config {
+  int64_t nx, ny, nz;        /* grid; nx, nz even; ny = number of B-splines */
+  int32_t labs_k;            /* LABS_K; GSL order passed to the generator is labs_k+1 (module_labs.f90:103) */
+  int32_t knot_map;          /* PB_KNOTS_* */
+  double  stretch;           /* knot stretching factor */
+  double  lmfx, lmfy, lmfz;  /* Lx = 2*pi*lmfx, Ly = 2*lmfy, Lz = 2*pi*lmfz (read_input_file.f90:100-102) */
+  double  nu, pr;            /* viscosity, Prandtl number */
+  double  bulk_u, bulk_w, bulk_t; /* held bulk values (wave00.f90:202, 395, 612) */
+  double  u_gal;             /* Galilean shift (wave00.f90:171, 227) */
+  double  cfl, dt_factor, dy_factor; /* dt = dt_factor*dt + (1-dt_factor)*CFL/pi/velmax (dft_x_realspace.f90:492) */
+  double  opp_alpha;         /* opposition-control gain alpha; 0 = reference behaviour (wall v = 0) */
+  double  opp_ydist;         /* detection plane distance from each wall in outer units (y+_d / Re_tau) */
+  int32_t rk_step;           /* number of RK substeps per step; 3 (read_input_file.f90:121) */
+  int32_t reserved;
+} pb_config;
+
+typedef struct pb_ctx pb_ctx;
+
+/* Per-substep scalar outputs that the reference returns through module
+ * globals (wave00.f90:208-211, 619-622; dft_x_realspace.f90:491-492). */
+typedef struct pb_substep_out {
+  double dt;        /* time step after the CFL update (substep 1 only changes it) */
+  double velmax;    /* max(|u|dxr + |v|dyr + |w|dzr); computed on substep 1 */
+  double phi_u;     /* mean pressure-gradient forcing coefficient, phi_dpdx(s) */
+  double phi_t;     /* scalar source coefficient, phi_dtdx(s) */
+  double tau_w;     /* wall_shear_stress (set when s == rk_step) */
+  double q_w;       /* wall_theta_stress (set when s == rk_step) */
+} pb_substep_out;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* Build the B-spline operators (LABS_initialize, module_labs.f90:77-238),
+ * the RK3 coefficients (RK3_initialize, module_runge_kutta.f90:30-51), the
+ * wavenumber tables and dyr (module_ppdfftw.f90:347-384, 572-583) and
+ * allocate the device state on the current HIP device. */
+int pb_create(const pb_config *cfg, pb_ctx **out);
+int pb_destroy(pb_ctx *ctx);
+const char *pb_last_error(void);
+
+/* Grid facts computed at create time: bw = LABS_BW (module_labs.f90:245-293). */
+int pb_get_info(const pb_ctx *ctx, int32_t *bw, int64_t *device_bytes);
+
+/* Host copies of the collocation operators: y (ny), D0/D1/D2 bands
+ * (2bw+1, ny) in the reference's near-boundary packed format
+ * (LABS_full_to_band, module_labs.f90:301-328), D0_LU and the integration
+ * weights.  Any pointer may be NULL. */
+int pb_get_labs(const pb_ctx *ctx, double *y, double *d0, double *d1, double *d2,
+                double *d0_lu, double *integ);
+
+/* ---- state exchange (off the timed path) ------------------------------- */
+
+/* U: (ny, nx/2, nz, 4) slots u, v, w, theta; rk: (ny, nx/2, nz, 3). */
+int pb_set_state(pb_ctx *ctx, const double *U, const double *rk);
+int pb_get_state(pb_ctx *ctx, double *U, double *rk);
+/* Fill the RK stores from u, v, w, theta already set by pb_set_state with
+ * rk == NULL (set_initial_data_uvw, initial_step.f90:139-179). */
+int pb_init_stores(pb_ctx *ctx);
+
+/* ---- the time step (device resident; no host<->device copies) ---------- */
+
+/* One RK3-CN substep s (1..rk_step) with time step *dt_inout:
+ * y_to_x x4, dft_x_realspace, x_to_y x8, rescale_data and wavespace
+ * (time_march.f90:63-120) in one call.  On substep 1 dt is updated from the
+ * CFL condition when dt_factor != 1. */
+int pb_substep(pb_ctx *ctx, int32_t s, double *dt_inout, pb_substep_out *out);
+/* rk_step substeps; out may be NULL or point to rk_step records. */
+int pb_step(pb_ctx *ctx, double *dt_inout, pb_substep_out *out);
+/* Wait for the context's stream. */
+int pb_sync(pb_ctx *ctx);
+
+/* Per-stage device timing with HIP events recorded on the context's stream
+ * around every kernel launch (no counterpart in the reference, whose phase
+ * timers are commented out: time_march.f90:62-124).  enable != 0 starts
+ * recording and clears the accumulators. */
+#define PB_STAGE_OPPVD     0  /* detection-plane v for opposition control */
+#define PB_STAGE_ZINV      1  /* y_to_x: padded inverse z DFT */
+#define PB_STAGE_XPHYS     2  /* dft_x_realspace: x DFTs + products */
+#define PB_STAGE_VELMAX    3  /* CFL maximum / dt */
+#define PB_STAGE_ZFWD      4  /* x_to_y + rescale_data */
+#define PB_STAGE_WAVESPACE 5  /* wavespace: all (kx,kz) systems */
+#define PB_NSTAGE          6
+int pb_timing(pb_ctx *ctx, int32_t enable);
+/* Sum of event times (ms) and launch counts per stage; synchronises. */
+int pb_timing_read(pb_ctx *ctx, double *ms, int64_t *launches);
+
+/* ---- host-array drop-ins for the reference routines -------------------- */
+/* These copy host arrays in the reference layout to the device, run the
+ * corresponding kernels and copy back.  They exist for incremental adoption
+ * and parity testing; the timed path is pb_substep. */
+
+/* wavespace(U, RK_STORE, ...)  (wavespace.f90:22): U(ny,nx/2,nz,10) holds the
+ * rescaled product spectra in slots 1-5, 7-9 (0-based 0-4, 6-8) and slot 10
+ * (0-based 9) is vold; on return slots 0,1,2,6 hold u, v, w, theta.
+ * rk(ny,nx/2,nz,3) is updated in place.  v_old (ny,nx/2,nz) is the v of
+ * the substep's input state (used by opposition control only; may be NULL
+ * when opp_alpha == 0). */
+int pb_wavespace_host(pb_ctx *ctx, int32_t s, double dt, double *U, double *rk,
+                      const double *v_old, pb_substep_out *out);
+
+/* y_to_x + dft_x_realspace + x_to_y + rescale_data for the four state
+ * fields u, v, w, theta in U(ny,nx/2,nz,4): writes the product spectra
+ * P(ny,nx/2,nz,10) in the reference slot order (slot 5 = v*v only when
+ * want_vv != 0, slot 9 = v).  velmax receives the CFL maximum. */
+int pb_nonlinear_host(pb_ctx *ctx, const double *U, double *P, int32_t want_vv,
+                      double *velmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PB_CAPI_H */

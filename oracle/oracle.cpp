@@ -1,0 +1,1302 @@
+// oracle/oracle.cpp -- CPU restatement of the reference RK3-CN substep.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity oracle and the CPU
+// baseline: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load it.  The product library (channel_passive_amd/csrc) never
+// links it.
+//
+// It restates, in plain C++ with strict IEEE FP64 (built with
+// -ffp-contract=off, no FMA), the algorithm of yaojieuh/CHANNEL_PASSIVE
+// (files cited relative to SOURCE_PASSOPB/):
+//   * B-spline operators          labs_fill_full.cpp:35-256  (GSL 1.16 / 2.x
+//     bspline + glfixed are not available; de Boor's BSPLVB/BSPLVD, the
+//     published algorithms GSL ports, are restated instead)
+//   * LABS packed band algebra    module_labs.f90:245-328, 401-413, 3487-3636,
+//                                 4252-4318, 4395-4430
+//   * 3/2-dealiased transforms    module_ppdfftw.f90:794-861, dft_x_realspace.f90:68-213
+//     (FFTW 3.3.x is not available; an own mixed-radix FFT computes the same
+//     unnormalised DFTs)
+//   * rescale, wavespace          rescale_data.f90, wavespace.f90:46-107
+//   * per-line solvers            hvhg.f90, solve_g.f90, solve_t.f90,
+//                                 solve_v_f.f90, solve_u_w.f90, wave00.f90
+//   * RK3 coefficients            module_runge_kutta.f90:34-48
+//   * stores from u,v,w           initial_step.f90:139-179
+//
+// Complex arithmetic is written out component-wise so that the order of
+// every rounding is explicit: complex*real and complex/real act on each
+// component (what gfortran emits for a real operand), complex*complex is
+// (ar*br - ai*bi, ar*bi + ai*br), complex/complex is Smith's algorithm
+// (gfortran's -fcx-fortran-rules division).
+//
+// Parity pins: see tests/test_oracle_*.py (scipy BSpline, numpy.fft, dense
+// LU, closed-form laminar states, divergence/wall invariants).
+
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+#include "../include/pb_capi.h"
+
+namespace {
+
+// ----------------------------------------------------------------- complex
+struct cd {
+  double re, im;
+};
+static inline cd C(double r, double i) { cd z; z.re = r; z.im = i; return z; }
+static inline cd add(cd a, cd b) { return C(a.re + b.re, a.im + b.im); }
+static inline cd sub(cd a, cd b) { return C(a.re - b.re, a.im - b.im); }
+static inline cd mulr(cd a, double r) { return C(a.re * r, a.im * r); }
+static inline cd divr(cd a, double r) { return C(a.re / r, a.im / r); }
+static inline cd mul(cd a, cd b) {
+  return C(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+// Smith's complex division (GCC expand_complex_div_wide, Fortran rules).
+static inline cd cdiv(cd a, cd b) {
+  double tr, ti;
+  if (std::fabs(b.re) < std::fabs(b.im)) {
+    double ratio = b.re / b.im;
+    double div = (b.re * ratio) + b.im;
+    tr = ((a.re * ratio) + a.im) / div;
+    ti = ((a.im * ratio) - a.re) / div;
+  } else {
+    double ratio = b.im / b.re;
+    double div = (b.im * ratio) + b.re;
+    tr = ((a.im * ratio) + a.re) / div;
+    ti = (a.im - (a.re * ratio)) / div;
+  }
+  return C(tr, ti);
+}
+// i*k as the reference forms it: ci*k with ci = (0,1) -> (0*k - 1*0, 0*0 + 1*k)
+static inline cd imag_unit_times(double k) { return C(0.0 * k - 1.0 * 0.0, 0.0 * 0.0 + 1.0 * k); }
+
+const double PI_REF = 3.141592653589793238462643;  // labs_fill_full.cpp:24
+
+// ---------------------------------------------------------------- B-splines
+// GSL workspace equivalent: order k, ncoeffs n = nbreak + k - 2, knots n + k.
+struct Bspl {
+  int k = 0, n = 0, nbreak = 0;
+  std::vector<double> t;  // knots
+};
+
+// gsl_bspline_knots: k-fold end knots, interior breakpoints.
+static void bspl_knots(Bspl &w, const std::vector<double> &brk) {
+  w.nbreak = (int)brk.size();
+  const int l = w.nbreak - 1;
+  w.n = l + w.k - 1;
+  w.t.assign(w.n + w.k, 0.0);
+  for (int i = 0; i < w.k; ++i) w.t[i] = brk[0];
+  for (int i = 1; i < l; ++i) w.t[w.k - 1 + i] = brk[i];
+  for (int i = w.n; i < w.n + w.k; ++i) w.t[i] = brk[l];
+}
+
+// Interval search: t[i] <= x < t[i+1], right end point belongs to the last
+// interval (GSL bspline_find_interval).  Returns -1 when outside.
+static int bspl_interval(const Bspl &w, double x) {
+  if (x < w.t[0]) return -1;
+  const int l = w.nbreak - 1;
+  int i;
+  for (i = w.k - 1; i < w.k + l - 1; ++i) {
+    const double ti = w.t[i], tip1 = w.t[i + 1];
+    if (ti <= x && x < tip1) break;
+    if (ti < x && x == tip1 && tip1 == w.t[w.k + l - 1]) break;
+  }
+  if (i == w.k + l - 1) return -1;
+  return i;
+}
+
+// de Boor BSPLVB (A Practical Guide to Splines, 1978, ch. X): values of the
+// jhigh nonzero B-splines of order jhigh at x, left = interval index.
+static void bsplvb(const std::vector<double> &t, int jhigh, int index, double x, int left,
+                   int &j, double *deltal, double *deltar, double *biatx) {
+  if (index == 1) {
+    j = 0;
+    biatx[0] = 1.0;
+  }
+  for (; j < jhigh - 1; ++j) {
+    deltar[j] = t[left + j + 1] - x;
+    deltal[j] = x - t[left - j];
+    double saved = 0.0;
+    for (int i = 0; i <= j; ++i) {
+      const double term = biatx[i] / (deltar[i] + deltal[j - i]);
+      biatx[i] = saved + deltar[i] * term;
+      saved = deltal[j - i] * term;
+    }
+    biatx[j + 1] = saved;
+  }
+}
+
+// de Boor BSPLVD: dbiatx[i*(nd+1) + m] = m-th derivative of the i-th nonzero
+// B-spline (from the left) at x, m = 0..nderiv.
+static void bsplvd(const std::vector<double> &t, int k, double x, int left, int nderiv,
+                   std::vector<double> &dbiatx) {
+  const int nd1 = nderiv + 1;
+  dbiatx.assign((size_t)k * nd1, 0.0);
+  std::vector<double> a((size_t)k * k, 0.0), deltal(k + 1), deltar(k + 1), col(k, 0.0);
+  const int mhigh = std::min(nderiv, k - 1);
+  int jj = 0;
+  auto DB = [&](int i, int m) -> double & { return dbiatx[(size_t)i * nd1 + m]; };
+  auto A = [&](int i, int j) -> double & { return a[(size_t)i * k + j]; };
+  bsplvb(t, k - mhigh, 1, x, left, jj, deltal.data(), deltar.data(), col.data());
+  for (int i = 0; i < k; ++i) DB(i, 0) = col[i];
+  if (nderiv > 0) {
+    int ideriv = mhigh;
+    for (int m = 1; m <= mhigh; ++m) {
+      for (int j = ideriv, jp1mid = 0; j < k; ++j, ++jp1mid) DB(j, ideriv) = DB(jp1mid, 0);
+      --ideriv;
+      for (int i = 0; i < k; ++i) col[i] = DB(i, 0);
+      bsplvb(t, k - ideriv, 2, x, left, jj, deltal.data(), deltar.data(), col.data());
+      for (int i = 0; i < k; ++i) DB(i, 0) = col[i];
+    }
+    int jlow = 0;
+    for (int i = 0; i < k; ++i) {
+      for (int j = jlow; j < k; ++j) A(j, i) = 0.0;
+      jlow = i;
+      A(i, i) = 1.0;
+    }
+    for (int m = 1; m <= mhigh; ++m) {
+      const int kmm = k - m;
+      const double fkmm = (double)(float)kmm;
+      int il = left;
+      int i = k - 1;
+      for (int ld = 0; ld < kmm; ++ld) {
+        const double factor = fkmm / (t[il + kmm] - t[il]);
+        for (int j = 0; j <= i; ++j) A(i, j) = factor * (A(i, j) - A(i - 1, j));
+        --il;
+        --i;
+      }
+      for (i = 0; i < k; ++i) {
+        double sum = 0;
+        const int jl = std::max(i, m);
+        for (int j = jl; j < k; ++j) sum += A(j, i) * DB(j, m);
+        DB(i, m) = sum;
+      }
+    }
+  }
+}
+
+// Greville abscissa: mean of t[i+1 .. i+k-1] with the long-double running
+// mean of gsl_stats_mean.
+static double bspl_greville(const Bspl &w, int i) {
+  int km1 = w.k - 1;
+  const double *data = w.t.data() + (i + 1);
+  if (km1 == 0) { km1 = 2; data -= 1; }
+  long double mean = 0;
+  for (int m = 0; m < km1; ++m) mean += (data[m] - mean) / (m + 1);
+  return (double)mean;
+}
+
+// Gauss-Legendre nodes (positive half, ascending) and weights for n points
+// by Newton iteration on P_n in long double.
+static void gauss_legendre_half(int n, std::vector<double> &x, std::vector<double> &wt) {
+  const int h = (n + 1) / 2;
+  std::vector<long double> xs, ws;
+  for (int i = 0; i < h; ++i) {
+    long double z = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (n + 0.5L));
+    long double pp = 0;
+    for (int it = 0; it < 100; ++it) {
+      long double p1 = 1, p2 = 0;
+      for (int j = 1; j <= n; ++j) {
+        long double p3 = p2;
+        p2 = p1;
+        p1 = ((2 * j - 1) * z * p2 - (j - 1) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1);
+      long double z1 = z;
+      z = z1 - p1 / pp;
+      if (fabsl(z - z1) < 1e-19L) {
+        // one more evaluation at the converged node for the weight
+        p1 = 1; p2 = 0;
+        for (int j = 1; j <= n; ++j) {
+          long double p3 = p2;
+          p2 = p1;
+          p1 = ((2 * j - 1) * z * p2 - (j - 1) * p3) / j;
+        }
+        pp = n * (z * p1 - p2) / (z * z - 1);
+        break;
+      }
+    }
+    xs.push_back(z);
+    ws.push_back(2 / ((1 - z * z) * pp * pp));
+  }
+  // ascending positive nodes; for odd n the first is 0
+  x.clear(); wt.clear();
+  for (int i = h - 1; i >= 0; --i) {
+    long double xv = xs[i];
+    if (n % 2 == 1 && i == h - 1) xv = 0;
+    x.push_back((double)xv);
+    wt.push_back((double)ws[i]);
+  }
+}
+
+// gsl_integration_glfixed_point: i-th point (ascending) of the n-point rule on [a,b].
+static void gl_point(double a, double b, int i, int n, const std::vector<double> &x,
+                     const std::vector<double> &wt, double &xi, double &wi) {
+  const double A = (b - a) / 2;
+  const double B = (a + b) / 2;
+  if (n % 2 == 1) {
+    const int k = i - n / 2;
+    const int s = k < 0 ? -1 : 1;
+    xi = B + s * A * x[s * k];
+    wi = A * wt[s * k];
+  } else if (i < n / 2) {
+    const int idx = (n / 2) - 1 - i;
+    xi = B - A * x[idx];
+    wi = A * wt[idx];
+  } else {
+    const int idx = i - n / 2;
+    xi = B + A * x[idx];
+    wi = A * wt[idx];
+  }
+}
+
+// LABS_fill_full (labs_fill_full.cpp:35-195): k = GSL order, n = ny.
+// M0..M4 row-major: M[r*n + c] = d^m B_c / dy^m at collocation point r.
+static int labs_fill_full(int k, int n, int knot_map, double stretch, double ly,
+                          std::vector<double> M[5], std::vector<double> &yc,
+                          std::vector<double> &brk, std::vector<double> &integ,
+                          std::vector<double> &center) {
+  const int nbreak = n - k + 2;
+  if (nbreak < 2) return -1;
+  brk.assign(nbreak, 0.0);
+  if (knot_map == PB_KNOTS_TANH) {
+    for (int i = 0; i < nbreak; ++i)
+      brk[i] = 0.5 * ly * std::tanh(stretch * (-1.0 + (2.0 * i) / (nbreak - 1))) / std::tanh(stretch);
+  } else if (std::fabs(stretch) <= 0.0000000000000001) {
+    for (int i = 0; i < nbreak; ++i) brk[i] = -1.0 + 2.0 / (nbreak - 1) * i;
+  } else {
+    for (int i = 0; i < nbreak; ++i)
+      brk[i] = std::sin(stretch * PI_REF / 2.0 * (-1.0 + (2.0 * i) / (nbreak - 1))) /
+               std::sin(stretch * PI_REF / 2.0);
+  }
+  Bspl w;
+  w.k = k;
+  bspl_knots(w, brk);
+  if (w.n != n) return -2;
+  for (int m = 0; m < 5; ++m) M[m].assign((size_t)n * n, 0.0);
+  yc.assign(n, 0.0);
+  std::vector<double> db;
+  for (int i = 0; i < n; ++i) {
+    const double y = bspl_greville(w, i);
+    yc[i] = y;
+    const int left = bspl_interval(w, y);
+    if (left < 0) return -3;
+    bsplvd(w.t, k, y, left, 4, db);
+    const int mmax = std::min(4, k - 1);
+    for (int j = 0; j < k; ++j) {
+      const int c = left - k + 1 + j;
+      for (int m = 1; m <= 4; ++m) M[m][(size_t)i * n + c] = (m <= mmax) ? db[(size_t)j * 5 + m] : 0.0;
+    }
+  }
+  std::vector<double> dl(k + 1), dr(k + 1), bia(k);
+  for (int i = 0; i < n; ++i) {
+    const double y = yc[i];
+    const int left = bspl_interval(w, y);
+    int jj = 0;
+    bsplvb(w.t, k, 1, y, left, jj, dl.data(), dr.data(), bia.data());
+    for (int j = 0; j < k; ++j) M[0][(size_t)i * n + (left - k + 1 + j)] = bia[j];
+  }
+  // integration weights (labs_fill_full.cpp:206-256), nderiv = 0
+  const int ngl = (k - 0 + 1) / 2;
+  std::vector<double> glx, glw;
+  gauss_legendre_half(ngl, glx, glw);
+  integ.assign(n, 0.0);
+  for (int i = 0; i < nbreak - 1; ++i) {
+    const double a = w.t[k - 1 + i], b = w.t[k + i];
+    for (int j = 0; j < ngl; ++j) {
+      double xj, wj;
+      gl_point(a, b, j, ngl, glx, glw, xj, wj);
+      const int left = bspl_interval(w, xj);
+      int jj = 0;
+      bsplvb(w.t, k, 1, xj, left, jj, dl.data(), dr.data(), bia.data());
+      const int ks = left - k + 1;
+      for (int m = 0; m < k; ++m) integ[ks + m] += wj * bia[m];
+    }
+  }
+  // centre-line values (labs_fill_full.cpp:146-149)
+  center.assign(n, 0.0);
+  {
+    const int left = bspl_interval(w, 0.0);
+    int jj = 0;
+    bsplvb(w.t, k, 1, 0.0, left, jj, dl.data(), dr.data(), bia.data());
+    for (int j = 0; j < k; ++j) center[left - k + 1 + j] = bia[j];
+  }
+  return 0;
+}
+
+// B-spline values at an arbitrary y (used for the opposition-control
+// detection plane).  Returns the first nonzero index; vals gets k values.
+static int bspl_row_at(int k, int n, const std::vector<double> &brk, double y, std::vector<double> &vals) {
+  Bspl w;
+  w.k = k;
+  bspl_knots(w, brk);
+  (void)n;
+  const int left = bspl_interval(w, y);
+  if (left < 0) return -1;
+  std::vector<double> dl(k + 1), dr(k + 1);
+  vals.assign(k, 0.0);
+  int jj = 0;
+  bsplvb(w.t, k, 1, y, left, jj, dl.data(), dr.data(), vals.data());
+  return left - k + 1;
+}
+
+// ------------------------------------------------------------------- LABS
+// LABS_eval_bw (module_labs.f90:245-293) on row-major matrices (row = collocation point).
+static int labs_eval_bw(const std::vector<double> M[5], int n) {
+  for (int k = 1;; ++k) {
+    double err = 0.0;
+    auto chk = [&](int r, int c) {  // 1-based matrix row r, column c
+      for (int m = 0; m < 3; ++m) err = std::max(err, std::fabs(M[m][(size_t)(r - 1) * n + (c - 1)]));
+    };
+    for (int r = 1; r <= k + 1; ++r)
+      for (int c = 2 * k + 2; c <= n; ++c) chk(r, c);
+    for (int r = k + 2; r <= n - k - 1; ++r) {
+      for (int c = 1; c <= r - k - 1; ++c) chk(r, c);
+      for (int c = r + k + 1; c <= n; ++c) chk(r, c);
+    }
+    for (int r = n - k; r <= n; ++r)
+      for (int c = 1; c <= n - 2 * k - 1; ++c) chk(r, c);
+    if (err <= 0.0) return k;
+    if (2 * k + 1 >= n) return -1;
+  }
+}
+
+// Packed band geometry (LABS_full_to_band, module_labs.f90:301-328), 0-based.
+struct Band {
+  int n, bw, w;  // w = 2bw+1
+  int first(int r) const {
+    if (r < bw) return 0;
+    if (r >= n - bw) return n - 1 - 2 * bw;
+    return r - bw;
+  }
+  int last(int r) const {
+    if (r < bw) return 2 * bw;
+    if (r >= n - bw) return n - 1;
+    return r + bw;
+  }
+  int slot(int r, int c) const {
+    if (r < bw && c > r + bw) return 2 * bw - c;                 // extra right entries, top rows
+    if (r >= n - bw && c < r - bw) return n - 1 - c;             // extra left entries, bottom rows
+    return bw + (c - r);
+  }
+};
+
+static void labs_full_to_band(const double *full, double *band, int bw, int n) {
+  Band B{n, bw, 2 * bw + 1};
+  std::fill(band, band + (size_t)n * B.w, 0.0);
+  for (int r = 0; r < n; ++r)
+    for (int c = B.first(r); c <= B.last(r); ++c) band[(size_t)r * B.w + B.slot(r, c)] = full[(size_t)r * n + c];
+}
+
+// No-pivot Crout LU in place (fixed_bandec*, module_labs.f90:943-1198, 3487-3636):
+// L (with diagonal) below, unit U above; every entry's update subtracts the
+// products of the nearest previous row first; U rows are scaled by the
+// reciprocal of the pivot.
+static void labs_lu(double *A, int n, int bw) {
+  Band B{n, bw, 2 * bw + 1};
+  auto at = [&](int r, int c) -> double & { return A[(size_t)r * B.w + B.slot(r, c)]; };
+  for (int r = 0; r < n; ++r) {
+    const int f = B.first(r);
+    for (int c = f; c <= r; ++c) {
+      double v = at(r, c);
+      for (int p = c - 1; p >= f; --p) {
+        if (c > B.last(p)) break;
+        v = v - at(r, p) * at(p, c);
+      }
+      at(r, c) = v;
+    }
+    const double inv = 1.0 / at(r, r);
+    for (int c = r + 1; c <= B.last(r); ++c) {
+      double v = at(r, c);
+      for (int p = r - 1; p >= f; --p) {
+        if (c > B.last(p)) break;
+        v = v - at(r, p) * at(p, c);
+      }
+      at(r, c) = inv * v;
+    }
+  }
+}
+
+// Forward/back substitution with complex RHS (fixed_banbks*, module_labs.f90:3769-3823, 4252-4318).
+static void labs_bs(const double *A, int n, int bw, cd *b) {
+  Band B{n, bw, 2 * bw + 1};
+  auto at = [&](int r, int c) -> double { return A[(size_t)r * B.w + B.slot(r, c)]; };
+  for (int r = 0; r < n; ++r) {
+    cd v = b[r];
+    for (int p = B.first(r); p < r; ++p) v = sub(v, mulr(b[p], at(r, p)));
+    b[r] = divr(v, at(r, r));
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    cd v = b[r];
+    for (int c = r + 1; c <= B.last(r); ++c) v = sub(v, mulr(b[c], at(r, c)));
+    b[r] = v;
+  }
+}
+
+// Band matvec b = A x, ascending column order (LABS_mat_mul_4, module_labs.f90:4395-4430).
+static void labs_mv(const double *A, int n, int bw, const cd *x, cd *b) {
+  Band B{n, bw, 2 * bw + 1};
+  for (int r = 0; r < n; ++r) {
+    const int f = B.first(r), l = B.last(r);
+    cd v = mulr(x[f], A[(size_t)r * B.w + B.slot(r, f)]);
+    for (int c = f + 1; c <= l; ++c) v = add(v, mulr(x[c], A[(size_t)r * B.w + B.slot(r, c)]));
+    b[r] = v;
+  }
+}
+
+// LABS_integration (module_labs.f90:401-413)
+static cd labs_integrate(const std::vector<double> &w, const cd *src, int n) {
+  cd t = C(0.0, 0.0);
+  for (int i = 0; i < n; ++i) t = add(t, mulr(src[i], w[i]));
+  return t;
+}
+
+// -------------------------------------------------------------------- FFT
+// Unnormalised DFT  X[k] = sum_j x[j] exp(sign*2*pi*i*j*k/N)  by recursive
+// mixed-radix decimation in time.
+struct FFTPlan {
+  int n = 0;
+  std::vector<int> radix;       // n = prod radix
+  std::vector<cd> tw_f, tw_b;   // exp(-2 pi i m/n), exp(+2 pi i m/n)
+};
+
+static FFTPlan make_plan(int n) {
+  FFTPlan p;
+  p.n = n;
+  int m = n;
+  for (int r : {4, 2, 3, 5, 7}) {
+    while (m % r == 0) { p.radix.push_back(r); m /= r; }
+  }
+  for (int r = 11; m > 1; r += 2)
+    while (m % r == 0) { p.radix.push_back(r); m /= r; }
+  p.tw_f.resize(n);
+  p.tw_b.resize(n);
+  for (int j = 0; j < n; ++j) {
+    const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)n;
+    const double c = (double)cosl(a), s = (double)sinl(a);
+    p.tw_f[j] = C(c, -s);
+    p.tw_b[j] = C(c, s);
+  }
+  return p;
+}
+
+// out[0..n) = DFT of in[0, stride, 2*stride, ...]; level selects the radix.
+static void fft_rec(const FFTPlan &p, const cd *in, cd *out, int n, int stride, int level, int sign, cd *scratch) {
+  if (n == 1) { out[0] = in[0]; return; }
+  const int r = p.radix[level];
+  const int m = n / r;
+  const std::vector<cd> &tw = sign < 0 ? p.tw_f : p.tw_b;
+  const int tstep = p.n / n;  // twiddle index scale for this level
+  // sub-transforms of the decimated sequences into out[q*m .. q*m+m)
+  for (int q = 0; q < r; ++q) fft_rec(p, in + (size_t)q * stride, out + (size_t)q * m, m, stride * r, level + 1, sign, scratch);
+  // combine: X[k + s*m] = sum_q W_n^{q(k+s m)} Y_q[k]
+  for (int k = 0; k < m; ++k) {
+    for (int q = 0; q < r; ++q) {
+      const cd y = out[(size_t)q * m + k];
+      scratch[q] = (q == 0) ? y : mul(y, tw[((size_t)q * k * tstep) % p.n]);
+    }
+    for (int s = 0; s < r; ++s) {
+      cd acc = scratch[0];
+      for (int q = 1; q < r; ++q) {
+        const int e = (int)(((long)q * s * m * tstep) % p.n);
+        acc = add(acc, mul(scratch[q], tw[e]));
+      }
+      scratch[r + s] = acc;
+    }
+    for (int s = 0; s < r; ++s) out[(size_t)s * m + k] = scratch[r + s];
+  }
+}
+
+static void fft(const FFTPlan &p, const cd *in, cd *out, int sign) {
+  std::vector<cd> scratch(64);
+  fft_rec(p, in, out, p.n, 1, 0, sign, scratch.data());
+}
+
+// ------------------------------------------------------------ the context
+struct Ctx {
+  pb_config cfg;
+  int nx, ny, nz, nxh, nx32, nz32, nh, bw, W, kgsl;
+  double Lx, Ly, Lz, lmfxr, lmfzr, dxr, dzr, cflx, rescale;
+  std::vector<double> y, brk, integ, center, dyr;
+  std::vector<double> full[5];
+  std::vector<double> d0, d1, d2, d0lu;   // packed bands (n x W)
+  std::vector<int> kx, kz;
+  double rk[4][3];
+  // opposition control: B-spline rows at the two detection planes
+  int opp_lo_first = 0, opp_hi_first = 0;
+  std::vector<double> opp_lo_vals, opp_hi_vals;
+  FFTPlan px, pz;
+};
+
+static void rk3_coef(double c[4][3]) {  // module_runge_kutta.f90:34-48, c[row][substep]
+  c[0][0] = 29.0 / 96.0;  c[0][1] = -3.0 / 40.0;  c[0][2] = 1.0 / 6.0;
+  c[1][0] = 37.0 / 160.0; c[1][1] = 5.0 / 24.0;   c[1][2] = 1.0 / 6.0;
+  c[2][0] = 8.0 / 15.0;   c[2][1] = 5.0 / 12.0;   c[2][2] = 3.0 / 4.0;
+  c[3][0] = 0.0;          c[3][1] = -17.0 / 60.0; c[3][2] = -5.0 / 12.0;
+}
+
+// Index into a reference-layout array A(ny, nxh, nz, slot).
+static inline size_t ix(const Ctx &c, int slot, int j, int k, int y) {
+  return (((size_t)slot * c.nz + j) * c.nxh + k) * c.ny + y;
+}
+
+// --------------------------------------------------------- per-line solvers
+struct Line {
+  const Ctx *c;
+  int n, bw;
+  const double *D0lu, *D0, *D1, *D2;
+  std::vector<double> Dbuf;
+  double dt;
+  int s;  // 1-based substep
+  double coef(int row, int sub) const { return c->rk[row][sub - 1]; }
+};
+
+// hvhg.f90:141-276
+static void hvhg(Line &L, cd *u1, cd *u2, cd *u3, cd *u4, cd *u5, cd *u7, cd *u8, cd *u9, double kx, double kz) {
+  const int n = L.n, bw = L.bw;
+  const double kz2 = kz * kz, kx2 = kx * kx, kxkz = kx * kz;
+  const cd ikx = imag_unit_times(kx), ikz = imag_unit_times(kz);
+  const cd ikx3 = imag_unit_times(kx * kx * kx);
+  const cd ikxkz2 = imag_unit_times(kx * (kz * kz));
+  const cd ikx2kz = imag_unit_times((kx * kx) * kz);
+  const cd ikz3 = imag_unit_times(kz * kz * kz);
+  for (int i = 0; i < n; ++i) {
+    const cd dummy = u1[i];
+    u1[i] = add(mulr(sub(u1[i], u3[i]), kxkz), mulr(u5[i], kz2 - kx2));
+    u3[i] = sub(sub(mulr(dummy, -kx2), mulr(u5[i], 2.0 * kxkz)), mulr(u3[i], kz2));
+  }
+  labs_bs(L.D0lu, n, bw, u3);
+  labs_mv(L.D1, n, bw, u3, u5);
+  const cd a = add(ikx3, ikxkz2), b = add(ikx2kz, ikz3);
+  for (int i = 0; i < n; ++i) u5[i] = add(add(u5[i], mul(a, u2[i])), mul(b, u4[i]));
+  for (int i = 0; i < n; ++i) {
+    const cd t3 = sub(mul(ikx, u4[i]), mul(ikz, u2[i]));
+    const cd t4 = add(mul(ikz, u4[i]), mul(ikx, u2[i]));
+    u3[i] = t3;
+    u4[i] = t4;
+  }
+  labs_bs(L.D0lu, n, bw, u3);
+  labs_mv(L.D1, n, bw, u3, u2);
+  labs_bs(L.D0lu, n, bw, u4);
+  labs_mv(L.D2, n, bw, u4, u3);
+  for (int i = 0; i < n; ++i) {
+    u5[i] = add(u5[i], u3[i]);
+    u1[i] = add(u1[i], u2[i]);
+  }
+  for (int i = 0; i < n; ++i) u7[i] = add(mul(ikx, u7[i]), mul(ikz, u9[i]));
+  labs_bs(L.D0lu, n, bw, u8);
+  labs_mv(L.D1, n, bw, u8, u9);
+  for (int i = 0; i < n; ++i) {
+    const cd t = add(u7[i], u9[i]);
+    u7[i] = C(-t.re, -t.im);
+  }
+}
+
+// LHS of the Helmholtz problem with diagonal-only wall rows (solve_g.f90:42-71).
+static void helmholtz_lhs(Line &L, double k2, double beta, bool mean) {
+  const int n = L.n, W = 2 * L.bw + 1, m = L.bw;
+  double *Db = L.Dbuf.data();
+  const double dummy = 1.0 + beta * k2;
+  for (int i = 0; i < W; ++i) Db[i] = 0.0;
+  Db[m] = mean ? (L.D0[m] - beta * L.D2[m]) : (dummy * L.D0[m] - beta * L.D2[m]);
+  for (int j = 1; j < n - 1; ++j)
+    for (int i = 0; i < W; ++i) {
+      const size_t q = (size_t)j * W + i;
+      Db[q] = mean ? (L.D0[q] - beta * L.D2[q]) : (dummy * L.D0[q] - beta * L.D2[q]);
+    }
+  const size_t e = (size_t)(n - 1) * W;
+  for (int i = 0; i < W; ++i) Db[e + i] = 0.0;
+  Db[e + m] = mean ? (L.D0[e + m] - beta * L.D2[e + m]) : (dummy * L.D0[e + m] - beta * L.D2[e + m]);
+  labs_lu(Db, n, L.bw);
+}
+
+// solve_g.f90:28-201 (nuv = nu) and solve_t.f90 (nuv = nu/pr); buf1, buf2 scratch.
+static void solve_gt(Line &L, cd *u1, cd *store, cd *buf1, cd *buf2, double kx, double kz, double nu, double pr, bool scalar) {
+  const int n = L.n, bw = L.bw, s = L.s, S = L.c->cfg.rk_step;
+  const double k2 = kx * kx + kz * kz;
+  const double dt = L.dt;
+  auto visc = [&](double coefv) { return scalar ? dt * coefv * nu / pr : dt * coefv * nu; };
+  const double beta = visc(L.coef(1, s));
+  helmholtz_lhs(L, k2, beta, false);
+  if (s == 1) {
+    const double alpha = visc(L.coef(0, s));
+    const double gamma = dt * L.coef(2, s);
+    const double dummy = 1.0 - alpha * k2;
+    for (int i = 0; i < n; ++i) buf1[i] = add(mulr(store[i], dummy), mulr(u1[i], gamma));
+    labs_bs(L.D0lu, n, bw, store);
+    labs_mv(L.D2, n, bw, store, buf2);
+    for (int i = 0; i < n; ++i) buf1[i] = add(buf1[i], mulr(buf2[i], alpha));
+  } else {
+    const double gamma = dt * L.coef(2, s);
+    for (int i = 0; i < n; ++i) buf1[i] = add(store[i], mulr(u1[i], gamma));
+  }
+  buf1[0] = C(0.0, 0.0);
+  buf1[n - 1] = C(0.0, 0.0);
+  if (s != S) {
+    const double zeta = dt * L.coef(3, s + 1);
+    for (int i = 0; i < n; ++i) store[i] = mulr(u1[i], zeta);
+  }
+  labs_bs(L.Dbuf.data(), n, bw, buf1);
+  labs_mv(L.D0, n, bw, buf1, u1);
+  labs_mv(L.D2, n, bw, buf1, buf2);
+  if (s != S) {
+    const double alpha = visc(L.coef(0, s + 1));
+    const double dummy = 1.0 - alpha * k2;
+    for (int i = 0; i < n; ++i) store[i] = add(add(store[i], mulr(u1[i], dummy)), mulr(buf2[i], alpha));
+  } else {
+    for (int i = 0; i < n; ++i) store[i] = u1[i];
+  }
+}
+
+// solve_v_f.f90:38-375.  vwall_lo / vwall_hi are the wall values of v
+// (0 in the reference, -alpha*v(y_d) with opposition control).
+static void solve_v_f(Line &L, cd *v, cd *dv, cd *hv, cd *store_phi, cd *buf1, cd *buf2, cd *buf3,
+                      double kx, double kz, double nu, cd vwall_lo, cd vwall_hi) {
+  const int n = L.n, bw = L.bw, s = L.s, S = L.c->cfg.rk_step, W = 2 * bw + 1, m = bw;
+  const double k2 = kx * kx + kz * kz;
+  const double dt = L.dt;
+  double *Db = L.Dbuf.data();
+  if (s == 1) {
+    const double alpha = dt * L.coef(0, s) * nu;
+    const double gamma = dt * L.coef(2, s);
+    const double dummy = 1.0 - alpha * k2;
+    for (int i = 0; i < n; ++i) buf1[i] = add(mulr(store_phi[i], dummy), mulr(hv[i], gamma));
+    labs_bs(L.D0lu, n, bw, store_phi);
+    labs_mv(L.D2, n, bw, store_phi, buf2);
+    for (int i = 0; i < n; ++i) buf1[i] = add(buf1[i], mulr(buf2[i], alpha));
+  } else {
+    const double gamma = dt * L.coef(2, s);
+    for (int i = 0; i < n; ++i) buf1[i] = add(store_phi[i], mulr(hv[i], gamma));
+  }
+  buf1[0] = C(0.0, 0.0);
+  buf1[n - 1] = C(0.0, 0.0);
+  if (s != S) {
+    const double zeta = dt * L.coef(3, s + 1);
+    for (int i = 0; i < n; ++i) store_phi[i] = mulr(hv[i], zeta);
+  }
+  buf2[0] = C(Db[m], Db[m]);
+  for (int i = 1; i < n; ++i) buf2[i] = C(0.0, 0.0);
+  labs_bs(Db, n, bw, buf1);
+  labs_bs(Db, n, bw, buf2);
+  labs_mv(L.D0, n, bw, buf1, hv);
+  labs_mv(L.D0, n, bw, buf2, buf3);
+  labs_mv(L.D2, n, bw, buf1, dv);
+  if (s != S) {
+    const double alpha = dt * nu * L.coef(0, s + 1);
+    const double dummy = 1.0 - alpha * k2;
+    for (int i = 0; i < n; ++i) store_phi[i] = add(add(store_phi[i], mulr(hv[i], dummy)), mulr(dv[i], alpha));
+  } else {
+    for (int i = 0; i < n; ++i) store_phi[i] = hv[i];
+  }
+  // Poisson LHS [D2 - k2 D0] with D0-diagonal wall rows (solve_v_f.f90:205-230)
+  for (int i = 0; i < W; ++i) Db[i] = 0.0;
+  Db[m] = L.D0[m];
+  for (int j = 1; j < n - 1; ++j)
+    for (int i = 0; i < W; ++i) {
+      const size_t q = (size_t)j * W + i;
+      Db[q] = -k2 * L.D0[q] + L.D2[q];
+    }
+  const size_t e = (size_t)(n - 1) * W;
+  for (int i = 0; i < W; ++i) Db[e + i] = 0.0;
+  Db[e + m] = L.D0[e + m];
+  labs_lu(Db, n, bw);
+  hv[0] = vwall_lo;
+  hv[n - 1] = vwall_hi;
+  labs_bs(Db, n, bw, hv);
+  for (int i = 0; i < n; ++i) buf1[i] = buf3[i];
+  buf1[0] = C(0.0, 0.0);
+  buf1[n - 1] = C(0.0, 0.0);
+  labs_bs(Db, n, bw, buf1);
+  labs_mv(L.D0, n, bw, hv, v);
+  labs_mv(L.D1, n, bw, hv, dv);
+  labs_mv(L.D1, n, bw, buf1, hv);
+  const cd den = sub(mul(hv[0], hv[0]), mul(hv[n - 1], hv[n - 1]));
+  const cd c1 = cdiv(sub(mul(hv[n - 1], dv[n - 1]), mul(hv[0], dv[0])), den);
+  const cd c2 = cdiv(sub(mul(hv[0], dv[n - 1]), mul(hv[n - 1], dv[0])), den);
+  for (int i = 0; i < n; ++i) dv[i] = sub(add(dv[i], mul(c1, hv[i])), mul(c2, hv[n - 1 - i]));
+  labs_mv(L.D2, n, bw, buf2, hv);
+  if (s != S) {
+    const double alpha = dt * nu * L.coef(0, s + 1);
+    const double dummy = 1.0 - alpha * k2;
+    for (int i = 0; i < n; ++i) {
+      // store_phi + C1*dummy*buf3(i) + C2*dummy*buf3(ny+1-i) + C1*alpha*hv(i) + C2*alpha*hv(ny+1-i)
+      const cd c1d = mulr(c1, dummy), c2d = mulr(c2, dummy), c1a = mulr(c1, alpha), c2a = mulr(c2, alpha);
+      cd acc = add(store_phi[i], mul(c1d, buf3[i]));
+      acc = add(acc, mul(c2d, buf3[n - 1 - i]));
+      acc = add(acc, mul(c1a, hv[i]));
+      acc = add(acc, mul(c2a, hv[n - 1 - i]));
+      store_phi[i] = acc;
+    }
+  } else {
+    for (int i = 0; i < n; ++i) store_phi[i] = add(add(store_phi[i], mul(c1, buf3[i])), mul(c2, buf3[n - 1 - i]));
+  }
+  labs_mv(L.D0, n, bw, buf1, hv);
+  for (int i = 0; i < n; ++i) v[i] = add(add(v[i], mul(c1, hv[i])), mul(c2, hv[n - 1 - i]));
+}
+
+// solve_u_w.f90:18-42 (g -> u, f -> w in place)
+static void solve_u_w(int n, cd *g, cd *f, double kx, double kz) {
+  const cd ci = C(0.0, 1.0);
+  const double den = kx * kx + kz * kz;
+  // kxx = ci*kx/(kx*kx+kz*kz): complex*real then complex/real
+  const cd kxx = divr(mul(ci, C(kx, 0.0)), den);
+  const cd kzz = divr(mul(ci, C(kz, 0.0)), den);
+  for (int i = 0; i < n; ++i) {
+    const cd dxg = mul(kxx, g[i]), dzg = mul(kzz, g[i]);
+    const cd dxf = mul(kxx, f[i]), dzf = mul(kzz, f[i]);
+    g[i] = sub(dxf, dzg);
+    f[i] = add(dxg, dzf);
+  }
+}
+
+struct MeanOut {
+  double phi_u = 0, phi_t = 0, tau_w = 0, q_w = 0;
+  bool wall = false;
+};
+
+// wave00.f90:41-685 and calculate_phi (704-714)
+static void wave00(Line &L, cd *u1, cd *u2, cd *u3, cd *u4, cd *u5, cd *u7, cd *u8, cd *u9,
+                   cd *st1, cd *st2, cd *st3, cd *buf1, double nu, double pr, double bulk_u,
+                   double bulk_w, double bulk_t, double u_gal, MeanOut &mo) {
+  const int n = L.n, bw = L.bw, s = L.s, S = L.c->cfg.rk_step, W = 2 * bw + 1, m = bw;
+  const double dt = L.dt;
+  const std::vector<double> &integ = L.c->integ;
+  const double *D1 = L.D1;
+  auto calc_phi = [&](double bulk, cd I) { return (bulk - 0.5 * I.re) / (0.5 * I.im); };
+  auto wall_d1 = [&](const cd *c) {
+    return (((D1[m] * c[0].re + D1[m + 1] * c[1].re) - D1[(size_t)(n - 1) * W + m - 1] * c[n - 2].re) -
+            D1[(size_t)(n - 1) * W + m] * c[n - 1].re);
+  };
+  // ---- U
+  double beta = dt * L.coef(1, s) * nu;
+  helmholtz_lhs(L, 0.0, beta, true);
+  if (s == 1) {
+    for (int i = 0; i < n; ++i) u1[i] = st1[i];
+    labs_bs(L.D0lu, n, bw, u1);
+    labs_bs(L.D0lu, n, bw, u2);
+    labs_mv(L.D2, n, bw, u1, buf1);
+    labs_mv(L.D1, n, bw, u2, u3);
+    const double alpha = dt * L.coef(0, s) * nu;
+    const double gamma = dt * L.coef(2, s);
+    for (int i = 1; i < n; ++i) buf1[i] = sub(add(st1[i], mulr(buf1[i], alpha)), mulr(u3[i], gamma));
+  } else {
+    labs_bs(L.D0lu, n, bw, u2);
+    labs_mv(L.D1, n, bw, u2, u3);
+    const double gamma = dt * L.coef(2, s);
+    for (int i = 1; i < n; ++i) buf1[i] = sub(st1[i], mulr(u3[i], gamma));
+  }
+  buf1[0] = C(0.0, 0.0);
+  for (int i = 1; i < n - 1; ++i) buf1[i] = C(buf1[i].re + u_gal, 1.0);
+  buf1[n - 1] = C(0.0, 0.0);
+  labs_bs(L.Dbuf.data(), n, bw, buf1);
+  double phi = calc_phi(bulk_u, labs_integrate(integ, buf1, n));
+  for (int i = 0; i < n; ++i) buf1[i] = C(buf1[i].re + phi * buf1[i].im, 0.0);
+  mo.phi_u = phi;
+  if (s == S) {
+    mo.tau_w = 0.5 * nu * wall_d1(buf1);
+    mo.wall = true;
+  }
+  labs_mv(L.D0, n, bw, buf1, u1);
+  for (int i = 0; i < n; ++i) u1[i] = C(u1[i].re - u_gal, u1[i].im);
+  if (s != S) {
+    labs_mv(L.D2, n, bw, buf1, st1);
+    const double alpha = dt * L.coef(0, s + 1) * nu;
+    const double zeta = dt * L.coef(3, s + 1);
+    for (int i = 0; i < n; ++i) st1[i] = sub(add(mulr(st1[i], alpha), u1[i]), mulr(u3[i], zeta));
+  } else {
+    for (int i = 0; i < n; ++i) st1[i] = u1[i];
+  }
+  // ---- W
+  if (s == 1) {
+    for (int i = 0; i < n; ++i) u2[i] = st2[i];
+    labs_bs(L.D0lu, n, bw, u2);
+    labs_mv(L.D2, n, bw, u2, buf1);
+    labs_bs(L.D0lu, n, bw, u4);
+    labs_mv(L.D1, n, bw, u4, u5);
+    const double alpha = dt * L.coef(0, s) * nu;
+    const double gamma = dt * L.coef(2, s);
+    for (int i = 1; i < n; ++i) buf1[i] = sub(add(st2[i], mulr(buf1[i], alpha)), mulr(u5[i], gamma));
+  } else {
+    labs_bs(L.D0lu, n, bw, u4);
+    labs_mv(L.D1, n, bw, u4, u5);
+    const double gamma = dt * L.coef(2, s);
+    for (int i = 1; i < n; ++i) buf1[i] = sub(st2[i], mulr(u5[i], gamma));
+  }
+  buf1[0] = C(0.0, 0.0);
+  for (int i = 1; i < n - 1; ++i) buf1[i] = C(buf1[i].re, 1.0);
+  buf1[n - 1] = C(0.0, 0.0);
+  labs_bs(L.Dbuf.data(), n, bw, buf1);
+  phi = calc_phi(bulk_w, labs_integrate(integ, buf1, n));
+  phi = 0.0;
+  for (int i = 0; i < n; ++i) buf1[i] = C(buf1[i].re + phi * buf1[i].im, 0.0);
+  labs_mv(L.D0, n, bw, buf1, u3);
+  if (s != S) {
+    labs_mv(L.D2, n, bw, buf1, st2);
+    const double alpha = dt * L.coef(0, s + 1) * nu;
+    const double zeta = dt * L.coef(3, s + 1);
+    for (int i = 0; i < n; ++i) st2[i] = sub(add(mulr(st2[i], alpha), u3[i]), mulr(u5[i], zeta));
+  } else {
+    for (int i = 0; i < n; ++i) st2[i] = u3[i];
+  }
+  for (int i = 0; i < n; ++i) u2[i] = C(0.0, 0.0);
+  // ---- Theta
+  beta = dt * L.coef(1, s) * nu / pr;
+  helmholtz_lhs(L, 0.0, beta, true);
+  if (s == 1) {
+    for (int i = 0; i < n; ++i) u7[i] = st3[i];
+    labs_bs(L.D0lu, n, bw, u7);
+    labs_mv(L.D2, n, bw, u7, buf1);
+    labs_bs(L.D0lu, n, bw, u8);
+    labs_mv(L.D1, n, bw, u8, u9);
+    const double alpha = dt * L.coef(0, s) * nu / pr;
+    const double gamma = dt * L.coef(2, s);
+    for (int i = 1; i < n; ++i) buf1[i] = sub(add(st3[i], mulr(buf1[i], alpha)), mulr(u9[i], gamma));
+  } else {
+    labs_bs(L.D0lu, n, bw, u8);
+    labs_mv(L.D1, n, bw, u8, u9);
+    const double gamma = dt * L.coef(2, s);
+    for (int i = 1; i < n; ++i) buf1[i] = sub(st3[i], mulr(u9[i], gamma));
+  }
+  buf1[0] = C(0.0, 0.0);
+  for (int i = 1; i < n - 1; ++i) buf1[i] = C(buf1[i].re, 1.0);
+  buf1[n - 1] = C(0.0, 0.0);
+  labs_bs(L.Dbuf.data(), n, bw, buf1);
+  phi = calc_phi(bulk_t, labs_integrate(integ, buf1, n));
+  for (int i = 0; i < n; ++i) buf1[i] = C(buf1[i].re + phi * buf1[i].im, 0.0);
+  mo.phi_t = phi;
+  if (s == S) mo.q_w = 0.5 * nu / pr * wall_d1(buf1);
+  labs_mv(L.D0, n, bw, buf1, u7);
+  if (s != S) {
+    labs_mv(L.D2, n, bw, buf1, st3);
+    const double alpha = dt * L.coef(0, s + 1) * nu / pr;
+    const double zeta = dt * L.coef(3, s + 1);
+    for (int i = 0; i < n; ++i) st3[i] = sub(add(mulr(st3[i], alpha), u7[i]), mulr(u9[i], zeta));
+  } else {
+    for (int i = 0; i < n; ++i) st3[i] = u7[i];
+  }
+}
+
+// B-spline interpolation of a collocation-value column at a detection plane:
+// c = LABS_bs(D0_LU, v); v(y_d) = sum_j B_j(y_d) c_j (ascending j).
+static cd interp_at(const Ctx &c, const cd *v, int first, const std::vector<double> &vals) {
+  std::vector<cd> tmp(v, v + c.ny);
+  labs_bs(c.d0lu.data(), c.ny, c.bw, tmp.data());
+  cd acc = mulr(tmp[first], vals[0]);
+  for (size_t j = 1; j < vals.size(); ++j) acc = add(acc, mulr(tmp[first + j], vals[j]));
+  return acc;
+}
+
+}  // namespace
+
+// ======================================================================= ABI
+extern "C" {
+
+// LABS setup as a standalone call (for tests).  full: 5 row-major ny x ny.
+int orc_labs_setup(int ny, int labs_k, int knot_map, double stretch, double ly, double *y,
+                   double *brk, double *integ, double *center, double *full5, int *bw_out) {
+  std::vector<double> M[5], yc, b, in, ce;
+  const int r = labs_fill_full(labs_k + 1, ny, knot_map, stretch, ly, M, yc, b, in, ce);
+  if (r) return r;
+  if (y) std::copy(yc.begin(), yc.end(), y);
+  if (brk) std::copy(b.begin(), b.end(), brk);
+  if (integ) std::copy(in.begin(), in.end(), integ);
+  if (center) std::copy(ce.begin(), ce.end(), center);
+  if (full5)
+    for (int m = 0; m < 5; ++m) std::copy(M[m].begin(), M[m].end(), full5 + (size_t)m * ny * ny);
+  if (bw_out) *bw_out = labs_eval_bw(M, ny);
+  return 0;
+}
+
+int orc_bspline_row(int ny, int labs_k, int knot_map, double stretch, double ly, double yv, double *vals,
+                    int *first) {
+  std::vector<double> M[5], yc, b, in, ce;
+  const int r = labs_fill_full(labs_k + 1, ny, knot_map, stretch, ly, M, yc, b, in, ce);
+  if (r) return r;
+  std::vector<double> v;
+  const int f = bspl_row_at(labs_k + 1, ny, b, yv, v);
+  if (f < 0) return -4;
+  std::copy(v.begin(), v.end(), vals);
+  *first = f;
+  return 0;
+}
+
+void orc_full_to_band(const double *full, double *band, int bw, int n) { labs_full_to_band(full, band, bw, n); }
+void orc_band_lu(double *A, int n, int bw) { labs_lu(A, n, bw); }
+void orc_band_solve(const double *A, int n, int bw, double *b) { labs_bs(A, n, bw, reinterpret_cast<cd *>(b)); }
+void orc_band_matvec(const double *A, int n, int bw, const double *x, double *b) {
+  labs_mv(A, n, bw, reinterpret_cast<const cd *>(x), reinterpret_cast<cd *>(b));
+}
+
+// Unnormalised complex DFT; sign -1 forward (FFTW_FORWARD), +1 backward.
+void orc_fft(const double *in, double *out, int n, int sign) {
+  FFTPlan p = make_plan(n);
+  fft(p, reinterpret_cast<const cd *>(in), reinterpret_cast<cd *>(out), sign);
+}
+
+void *orc_create(const pb_config *cfg) {
+  Ctx *c = new Ctx();
+  c->cfg = *cfg;
+  c->nx = (int)cfg->nx; c->ny = (int)cfg->ny; c->nz = (int)cfg->nz;
+  c->nxh = c->nx / 2; c->nx32 = c->nx * 3 / 2; c->nz32 = c->nz * 3 / 2; c->nh = c->nz / 2;
+  c->kgsl = cfg->labs_k + 1;
+  const double pi = 3.14159265358979323;  // module_global_variable.f90:18
+  c->Lx = cfg->lmfx * 2 * pi;
+  c->Ly = cfg->lmfy * 2;
+  c->Lz = cfg->lmfz * 2 * pi;
+  c->rescale = 1.0 / (double)(c->nx32 * c->nz32);
+  c->dxr = (double)c->nx / c->Lx;
+  c->dzr = (double)c->nz / c->Lz;
+  c->cflx = cfg->cfl / pi;
+  c->lmfxr = 1.0 / cfg->lmfx;
+  c->lmfzr = 1.0 / cfg->lmfz;
+  if (labs_fill_full(c->kgsl, c->ny, cfg->knot_map, cfg->stretch, c->Ly, c->full, c->y, c->brk, c->integ,
+                     c->center)) {
+    delete c;
+    return nullptr;
+  }
+  c->bw = labs_eval_bw(c->full, c->ny);
+  if (c->bw < 1) { delete c; return nullptr; }
+  c->W = 2 * c->bw + 1;
+  const size_t bs = (size_t)c->ny * c->W;
+  c->d0.resize(bs); c->d1.resize(bs); c->d2.resize(bs);
+  labs_full_to_band(c->full[0].data(), c->d0.data(), c->bw, c->ny);
+  labs_full_to_band(c->full[1].data(), c->d1.data(), c->bw, c->ny);
+  labs_full_to_band(c->full[2].data(), c->d2.data(), c->bw, c->ny);
+  c->d0lu = c->d0;
+  labs_lu(c->d0lu.data(), c->ny, c->bw);
+  rk3_coef(c->rk);
+  // wavenumber tables (module_ppdfftw.f90:347-384)
+  c->kx.resize(c->nxh);
+  for (int i = 0; i < c->nxh; ++i) c->kx[i] = i;
+  c->kz.resize(c->nz);
+  for (int i = 0; i <= c->nh; ++i) c->kz[i] = i;
+  for (int i = c->nh + 1; i < c->nz; ++i) c->kz[i] = i + 1 - 2 * c->nh - 1;
+  // dyr (module_ppdfftw.f90:572-583)
+  c->dyr.resize(c->ny);
+  for (int i = 0; i < c->ny; ++i) {
+    double d;
+    if (i == 0) d = std::fabs(c->y[1] - c->y[0]);
+    else if (i == c->ny - 1) d = std::fabs(c->y[c->ny - 1] - c->y[c->ny - 2]);
+    else d = std::min(std::fabs(c->y[i] - c->y[i - 1]), std::fabs(c->y[i] - c->y[i + 1]));
+    c->dyr[i] = 1.0 / (cfg->dy_factor * d);
+  }
+  if (cfg->opp_alpha != 0.0) {
+    c->opp_lo_first = bspl_row_at(c->kgsl, c->ny, c->brk, -1.0 + cfg->opp_ydist, c->opp_lo_vals);
+    c->opp_hi_first = bspl_row_at(c->kgsl, c->ny, c->brk, 1.0 - cfg->opp_ydist, c->opp_hi_vals);
+    if (c->opp_lo_first < 0 || c->opp_hi_first < 0) { delete c; return nullptr; }
+  }
+  c->px = make_plan(c->nx32);
+  c->pz = make_plan(c->nz32);
+  return c;
+}
+
+void orc_destroy(void *h) { delete static_cast<Ctx *>(h); }
+
+int orc_info(void *h, int *bw, double *y, double *d0, double *d1, double *d2, double *d0lu, double *integ,
+             double *dyr) {
+  Ctx *c = static_cast<Ctx *>(h);
+  if (bw) *bw = c->bw;
+  const size_t bs = (size_t)c->ny * c->W;
+  if (y) std::copy(c->y.begin(), c->y.end(), y);
+  if (d0) std::copy(c->d0.begin(), c->d0.begin() + bs, d0);
+  if (d1) std::copy(c->d1.begin(), c->d1.begin() + bs, d1);
+  if (d2) std::copy(c->d2.begin(), c->d2.begin() + bs, d2);
+  if (d0lu) std::copy(c->d0lu.begin(), c->d0lu.begin() + bs, d0lu);
+  if (integ) std::copy(c->integ.begin(), c->integ.end(), integ);
+  if (dyr) std::copy(c->dyr.begin(), c->dyr.end(), dyr);
+  return 0;
+}
+
+// set_initial_data_uvw (initial_step.f90:139-179): state U(ny,nxh,nz,4)
+// slots u,v,w,theta -> RK(ny,nxh,nz,3).
+int orc_init_stores(void *h, const double *Ud, double *RKd) {
+  Ctx *c = static_cast<Ctx *>(h);
+  const cd *U = reinterpret_cast<const cd *>(Ud);
+  cd *RK = reinterpret_cast<cd *>(RKd);
+  const int n = c->ny;
+#pragma omp parallel for schedule(static)
+  for (int l = 0; l < c->nxh * c->nz; ++l) {
+    const int k = l % c->nxh, j = l / c->nxh;
+    std::vector<cd> d(n);
+    const cd *u = U + ix(*c, 0, j, k, 0), *v = U + ix(*c, 1, j, k, 0), *w = U + ix(*c, 2, j, k, 0),
+             *th = U + ix(*c, 3, j, k, 0);
+    cd *r1 = RK + ix(*c, 0, j, k, 0), *r2 = RK + ix(*c, 1, j, k, 0), *r3 = RK + ix(*c, 2, j, k, 0);
+    if (c->kx[k] == 0 && c->kz[j] == 0) {
+      for (int i = 0; i < n; ++i) { r1[i] = u[i]; r2[i] = w[i]; r3[i] = th[i]; }
+    } else {
+      for (int i = 0; i < n; ++i) d[i] = v[i];
+      labs_bs(c->d0lu.data(), n, c->bw, d.data());
+      labs_mv(c->d2.data(), n, c->bw, d.data(), r2);
+      const double kxp = c->kx[k] * c->lmfxr, kzp = c->kz[j] * c->lmfzr;
+      const double k2 = kxp * kxp + kzp * kzp;
+      const cd ikz = C(0.0, kzp), ikx = C(0.0, kxp);
+      for (int i = 0; i < n; ++i) {
+        r2[i] = sub(r2[i], mulr(v[i], k2));
+        r1[i] = sub(mul(ikz, u[i]), mul(ikx, w[i]));
+      }
+      for (int i = 0; i < n; ++i) r3[i] = th[i];
+    }
+  }
+  return 0;
+}
+
+// Strided view of a field set: element (slot, kz index j, kx index k, y).
+struct View {
+  cd *p;
+  size_t sl, j, k, y;
+  cd &operator()(int f, int jj, int kk, int yy) const { return p[f * sl + jj * j + kk * k + yy * y]; }
+};
+
+// Nonlinear part of the substep for one y plane: y_to_x (x4),
+// dft_x_realspace and x_to_y (9 or 10 fields) and rescale_data.
+static double nonlinear_plane(const Ctx *c, const View &S, const View &P, int want_vv, int y,
+                              std::vector<cd> &G, std::vector<cd> &H) {
+  const int nxh = c->nxh, nz32 = c->nz32, nx32 = c->nx32, nh = c->nh;
+  std::vector<cd> col(nz32), colo(nz32), xs(nx32), xo(nx32);
+  std::vector<double> phys((size_t)4 * nx32), out((size_t)10 * nx32);
+  double vmax = 0.0;
+  // z inverse with 3/2 padding (module_ppdfftw.f90:830-861)
+  for (int f = 0; f < 4; ++f)
+    for (int k = 0; k < nxh; ++k) {
+      for (int jj = 0; jj < nh; ++jj) col[jj] = S(f, jj, k, y);
+      for (int jj = nh; jj <= 2 * nh; ++jj) col[jj] = C(0.0, 0.0);
+      for (int jj = 2 * nh + 1; jj < nz32; ++jj) col[jj] = S(f, jj - nh, k, y);
+      fft(c->pz, col.data(), colo.data(), +1);
+      for (int z = 0; z < nz32; ++z) G[((size_t)f * nz32 + z) * nxh + k] = colo[z];
+    }
+  for (int z = 0; z < nz32; ++z) {
+    // c2r with zero padding nxh -> nx32/2+1 (dft_x_realspace.f90:72-105)
+    for (int f = 0; f < 4; ++f) {
+      for (int m = 0; m < nx32; ++m) xs[m] = C(0.0, 0.0);
+      const cd *g = &G[((size_t)f * nz32 + z) * nxh];
+      xs[0] = C(g[0].re, 0.0);  // FFTW c2r ignores Im of the DC bin
+      for (int m = 1; m < nxh; ++m) {
+        xs[m] = g[m];
+        xs[nx32 - m] = C(g[m].re, -g[m].im);
+      }
+      fft(c->px, xs.data(), xo.data(), +1);
+      for (int m = 0; m < nx32; ++m) phys[(size_t)f * nx32 + m] = xo[m].re;
+    }
+    // products (dft_x_realspace.f90:109-127), 0-based slots
+    const double *p = phys.data();
+    for (int m = 0; m < nx32; ++m) {
+      const double t1 = p[m], t2 = p[nx32 + m], t3 = p[2 * nx32 + m], t4 = p[3 * nx32 + m];
+      const double bv2 = t2 * t2;
+      out[(size_t)8 * nx32 + m] = t3 * t4;
+      out[(size_t)7 * nx32 + m] = t2 * t4;
+      out[(size_t)6 * nx32 + m] = t1 * t4;
+      out[(size_t)4 * nx32 + m] = t1 * t3;
+      out[(size_t)3 * nx32 + m] = t2 * t3;
+      out[(size_t)2 * nx32 + m] = t3 * t3 - bv2;
+      out[(size_t)1 * nx32 + m] = t1 * t2;
+      out[(size_t)0 * nx32 + m] = t1 * t1 - bv2;
+      out[(size_t)9 * nx32 + m] = t2;
+      out[(size_t)5 * nx32 + m] = bv2;
+      const double vel = (std::fabs(t1) * c->dxr) + (std::fabs(t2) * c->dyr[y]) + (std::fabs(t3) * c->dzr);
+      vmax = std::max(vel, vmax);
+    }
+    // r2c, keep 0..nxh-1
+    for (int f = 0; f < 10; ++f) {
+      if (f == 5 && !want_vv) continue;
+      for (int m = 0; m < nx32; ++m) xs[m] = C(out[(size_t)f * nx32 + m], 0.0);
+      fft(c->px, xs.data(), xo.data(), -1);
+      for (int m = 0; m < nxh; ++m) H[((size_t)f * nz32 + z) * nxh + m] = xo[m];
+    }
+  }
+  // z forward with truncation, then rescale (module_ppdfftw.f90:794-823; rescale_data.f90)
+  for (int f = 0; f < 10; ++f) {
+    if (f == 5 && !want_vv) continue;
+    for (int k = 0; k < nxh; ++k) {
+      for (int z = 0; z < nz32; ++z) col[z] = H[((size_t)f * nz32 + z) * nxh + k];
+      fft(c->pz, col.data(), colo.data(), -1);
+      for (int jj = 0; jj < nh; ++jj) P(f, jj, k, y) = mulr(colo[jj], c->rescale);
+      P(f, nh, k, y) = mulr(C(0.0, 0.0), c->rescale);
+      for (int jj = 2 * nh + 1; jj < nz32; ++jj) P(f, jj - nh, k, y) = mulr(colo[jj], c->rescale);
+    }
+  }
+  return vmax;
+}
+
+// Nonlinear part for the y planes [y0, y1) of reference-layout arrays
+// S(ny,nxh,nz,4) -> P(ny,nxh,nz,10); velmax = max over the range.
+int orc_nonlinear(void *h, const double *Sd, double *Pd, int want_vv, int y0, int y1, double *velmax) {
+  Ctx *c = static_cast<Ctx *>(h);
+  const size_t ny = c->ny, nxh = c->nxh, nz = c->nz;
+  View S{reinterpret_cast<cd *>(const_cast<double *>(Sd)), nz * nxh * ny, nxh * ny, ny, 1};
+  View P{reinterpret_cast<cd *>(Pd), nz * nxh * ny, nxh * ny, ny, 1};
+  double vmax = 0.0;
+#pragma omp parallel reduction(max : vmax)
+  {
+    std::vector<cd> G((size_t)4 * c->nz32 * nxh), H((size_t)10 * c->nz32 * nxh);
+#pragma omp for schedule(dynamic, 1)
+    for (int y = y0; y < y1; ++y) vmax = std::max(vmax, nonlinear_plane(c, S, P, want_vv, y, G, H));
+  }
+  if (velmax) *velmax = vmax;
+  return 0;
+}
+
+// Detection-plane values of v for opposition control: out(nxh, nz, 2)
+// complex, [0] at y = -1 + ydist, [1] at y = 1 - ydist.
+int orc_opp_planes(void *h, const double *Sd, double *out) {
+  Ctx *c = static_cast<Ctx *>(h);
+  const cd *S = reinterpret_cast<const cd *>(Sd);
+  cd *o = reinterpret_cast<cd *>(out);
+  const size_t nl = (size_t)c->nxh * c->nz;
+#pragma omp parallel for schedule(static)
+  for (long l = 0; l < (long)nl; ++l) {
+    const int k = (int)(l % c->nxh), j = (int)(l / c->nxh);
+    const cd *v = S + ix(*c, 1, j, k, 0);
+    o[l] = interp_at(*c, v, c->opp_lo_first, c->opp_lo_vals);
+    o[nl + l] = interp_at(*c, v, c->opp_hi_first, c->opp_hi_vals);
+  }
+  return 0;
+}
+
+// wavespace (wavespace.f90:46-107) over lines [l0, l1) of the (kx,kz) grid
+// (l = kz_index*nxh + kx).  U(ny,nxh,nz,10) holds the products on entry and
+// u,v,w,theta in slots 0,1,2,6 on exit; RK(ny,nxh,nz,3) is updated.
+// vd (nxh,nz,2) = detection-plane v for opposition control (may be NULL).
+// mean[4] = phi_u, phi_t, tau_w, q_w (written by the line that holds (0,0)).
+int orc_wavespace(void *h, int s, double dt, double *Ud, double *RKd, const double *vdd, int l0, int l1,
+                  double *mean) {
+  Ctx *c = static_cast<Ctx *>(h);
+  cd *U = reinterpret_cast<cd *>(Ud);
+  cd *RK = reinterpret_cast<cd *>(RKd);
+  const cd *vd = reinterpret_cast<const cd *>(vdd);
+  const int n = c->ny;
+  const size_t nl = (size_t)c->nxh * c->nz;
+  const pb_config &g = c->cfg;
+#pragma omp parallel
+  {
+    Line L;
+    L.c = c; L.n = n; L.bw = c->bw;
+    L.D0lu = c->d0lu.data(); L.D0 = c->d0.data(); L.D1 = c->d1.data(); L.D2 = c->d2.data();
+    L.Dbuf.assign((size_t)n * c->W, 0.0);
+    L.dt = dt; L.s = s;
+    std::vector<cd> b1(n), b2(n);
+#pragma omp for schedule(dynamic, 16)
+    for (int l = l0; l < l1; ++l) {
+      const int k = l % c->nxh, j = l / c->nxh;
+      auto slot = [&](int f) { return U + ix(*c, f, j, k, 0); };
+      auto rk = [&](int f) { return RK + ix(*c, f, j, k, 0); };
+      if (c->kx[k] == 0 && c->kz[j] == 0) {
+        MeanOut mo;
+        wave00(L, slot(0), slot(1), slot(2), slot(3), slot(4), slot(6), slot(7), slot(8), rk(0), rk(1), rk(2),
+               b1.data(), g.nu, g.pr, g.bulk_u, g.bulk_w, g.bulk_t, g.u_gal, mo);
+        if (mean) { mean[0] = mo.phi_u; mean[1] = mo.phi_t; mean[2] = mo.tau_w; mean[3] = mo.q_w; }
+      } else {
+        const double kxp = c->kx[k] * c->lmfxr, kzp = c->kz[j] * c->lmfzr;
+        hvhg(L, slot(0), slot(1), slot(2), slot(3), slot(4), slot(6), slot(7), slot(8), kxp, kzp);
+        solve_gt(L, slot(0), rk(0), b1.data(), b2.data(), kxp, kzp, g.nu, g.pr, false);
+        cd wlo = C(0.0, 0.0), whi = C(0.0, 0.0);
+        if (g.opp_alpha != 0.0 && vd) {
+          wlo = mulr(vd[(size_t)j * c->nxh + k], -g.opp_alpha);
+          whi = mulr(vd[nl + (size_t)j * c->nxh + k], -g.opp_alpha);
+        }
+        solve_v_f(L, slot(1), slot(2), slot(4), rk(1), slot(3), b1.data(), b2.data(), kxp, kzp, g.nu, wlo, whi);
+        solve_u_w(n, slot(0), slot(2), kxp, kzp);
+        solve_gt(L, slot(6), rk(2), b1.data(), b2.data(), kxp, kzp, g.nu, g.pr, true);
+      }
+    }
+  }
+  return 0;
+}
+
+// Bounded CPU-baseline sample at full grid size without full-size arrays:
+// the nonlinear stage on nplanes y planes (plane-local buffers) and the
+// wavespace on nlines (kx,kz) systems spread over the grid, on synthetic
+// data.  Returns wall seconds of each part (OpenMP over planes / lines).
+int orc_sample(void *h, int nplanes, int nlines, int s, double dt, double *t_nonlin, double *t_wave) {
+  Ctx *c = static_cast<Ctx *>(h);
+  const size_t nxh = c->nxh, nz = c->nz, ny = c->ny;
+  uint64_t st = 0x9E3779B97F4A7C15ull;
+  auto rnd = [&]() {
+    st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+    return (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+  };
+  // --- nonlinear: each thread owns one plane of input and output
+  const int nt = omp_get_max_threads();
+  std::vector<std::vector<cd>> Sp(nt), Pp(nt);
+  for (int t = 0; t < nt; ++t) {
+    Sp[t].resize(4 * nz * nxh);
+    Pp[t].resize(10 * nz * nxh);
+    for (auto &z : Sp[t]) z = C(rnd(), rnd());
+  }
+  double t0 = omp_get_wtime();
+#pragma omp parallel
+  {
+    const int t = omp_get_thread_num();
+    std::vector<cd> G((size_t)4 * c->nz32 * nxh), H((size_t)10 * c->nz32 * nxh);
+    View S{Sp[t].data(), nz * nxh, nxh, 1, 0}, P{Pp[t].data(), nz * nxh, nxh, 1, 0};
+#pragma omp for schedule(dynamic, 1)
+    for (int q = 0; q < nplanes; ++q) nonlinear_plane(c, S, P, 0, (int)((size_t)q * ny / nplanes), G, H);
+  }
+  *t_nonlin = omp_get_wtime() - t0;
+  // --- wavespace lines: (kx,kz) spread over the grid, skipping (0,0)
+  std::vector<cd> U((size_t)10 * nlines * ny), RK((size_t)3 * nlines * ny);
+  for (auto &z : U) z = C(1e-3 * rnd(), 1e-3 * rnd());
+  for (auto &z : RK) z = C(1e-3 * rnd(), 1e-3 * rnd());
+  t0 = omp_get_wtime();
+#pragma omp parallel
+  {
+    Line L;
+    L.c = c; L.n = (int)ny; L.bw = c->bw;
+    L.D0lu = c->d0lu.data(); L.D0 = c->d0.data(); L.D1 = c->d1.data(); L.D2 = c->d2.data();
+    L.Dbuf.assign(ny * c->W, 0.0);
+    L.dt = dt; L.s = s;
+    std::vector<cd> b1(ny), b2(ny);
+    const pb_config &g = c->cfg;
+#pragma omp for schedule(dynamic, 4)
+    for (int l = 0; l < nlines; ++l) {
+      const size_t sys = 1 + (size_t)l * (nxh * nz - 1) / nlines;
+      const int k = (int)(sys % nxh), j = (int)(sys / nxh);
+      const double kxp = c->kx[k] * c->lmfxr, kzp = c->kz[j] * c->lmfzr;
+      auto sl = [&](int f) { return U.data() + ((size_t)f * nlines + l) * ny; };
+      auto rk = [&](int f) { return RK.data() + ((size_t)f * nlines + l) * ny; };
+      hvhg(L, sl(0), sl(1), sl(2), sl(3), sl(4), sl(6), sl(7), sl(8), kxp, kzp);
+      solve_gt(L, sl(0), rk(0), b1.data(), b2.data(), kxp, kzp, g.nu, g.pr, false);
+      solve_v_f(L, sl(1), sl(2), sl(4), rk(1), sl(3), b1.data(), b2.data(), kxp, kzp, g.nu, C(0, 0), C(0, 0));
+      solve_u_w((int)ny, sl(0), sl(2), kxp, kzp);
+      solve_gt(L, sl(6), rk(2), b1.data(), b2.data(), kxp, kzp, g.nu, g.pr, true);
+    }
+  }
+  *t_wave = omp_get_wtime() - t0;
+  return nt;
+}
+
+// One full substep on reference-layout arrays.  S(ny,nxh,nz,4) state in/out,
+// RK(ny,nxh,nz,3) in/out.  out6 = dt, velmax, phi_u, phi_t, tau_w, q_w.
+int orc_substep(void *h, double *Sd, double *RKd, int s, double *dt_inout, double *out6) {
+  Ctx *c = static_cast<Ctx *>(h);
+  const size_t N = (size_t)c->ny * c->nxh * c->nz;
+  std::vector<double> P(N * 10 * 2);
+  std::vector<double> vd;
+  if (c->cfg.opp_alpha != 0.0) {
+    vd.resize((size_t)c->nxh * c->nz * 2 * 2);
+    orc_opp_planes(h, Sd, vd.data());
+  }
+  double velmax = 0.0;
+  orc_nonlinear(h, Sd, P.data(), 0, 0, c->ny, &velmax);
+  double dt = *dt_inout;
+  if (s == 1) {
+    dt = c->cfg.dt_factor * dt + (1.0 - c->cfg.dt_factor) * c->cflx / velmax;
+    if (dt <= 1e-14 || dt > 1e5) return PB_E_NUMERIC;
+  }
+  double mean[4] = {0, 0, 0, 0};
+  orc_wavespace(h, s, dt, P.data(), RKd, vd.empty() ? nullptr : vd.data(), 0, (int)(c->nxh * c->nz), mean);
+  // copy u,v,w,theta back into the state
+  const int map[4] = {0, 1, 2, 6};
+  for (int f = 0; f < 4; ++f) std::memcpy(Sd + (size_t)f * N * 2, P.data() + (size_t)map[f] * N * 2, N * 16);
+  *dt_inout = dt;
+  if (out6) {
+    out6[0] = dt; out6[1] = velmax; out6[2] = mean[0]; out6[3] = mean[1];
+    out6[4] = mean[2]; out6[5] = mean[3];
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,195 @@
+"""ctypes front end of the parity oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg.  The product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from channel_passive_amd.config import ChannelConfig, PbConfig
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.POINTER(PbConfig)]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)] + [_dp] * 7
+        L.orc_init_stores.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_nonlinear.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_opp_planes.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_wavespace.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp,
+                                    ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_substep.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_labs_setup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
+        L.orc_bspline_row.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, _dp, ctypes.POINTER(ctypes.c_int)]
+        L.orc_full_to_band.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
+        L.orc_band_lu.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
+        L.orc_band_solve.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_band_matvec.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_fft.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+# ------------------------------------------------------------ LABS helpers
+def labs_setup(ny: int, labs_k: int, knot_map: int, stretch: float, ly: float = 2.0):
+    k = labs_k + 1
+    nbreak = ny - k + 2
+    y = np.zeros(ny); brk = np.zeros(nbreak); integ = np.zeros(ny); center = np.zeros(ny)
+    full = np.zeros((5, ny, ny))
+    bw = ctypes.c_int(0)
+    r = lib().orc_labs_setup(ny, labs_k, knot_map, stretch, ly, _p(y), _p(brk), _p(integ), _p(center),
+                             _p(full), ctypes.byref(bw))
+    if r:
+        raise RuntimeError(f"orc_labs_setup failed: {r}")
+    return dict(y=y, brk=brk, integ=integ, center=center, full=full, bw=bw.value)
+
+
+def bspline_row(ny, labs_k, knot_map, stretch, yv, ly=2.0):
+    vals = np.zeros(labs_k + 1)
+    first = ctypes.c_int(0)
+    r = lib().orc_bspline_row(ny, labs_k, knot_map, stretch, ly, yv, _p(vals), ctypes.byref(first))
+    if r:
+        raise RuntimeError(f"orc_bspline_row failed: {r}")
+    return first.value, vals
+
+
+def full_to_band(full: np.ndarray, bw: int) -> np.ndarray:
+    n = full.shape[0]
+    band = np.zeros((n, 2 * bw + 1))
+    lib().orc_full_to_band(_p(np.ascontiguousarray(full)), _p(band), bw, n)
+    return band
+
+
+def band_lu(band: np.ndarray, bw: int) -> np.ndarray:
+    A = np.ascontiguousarray(band).copy()
+    lib().orc_band_lu(_p(A), A.shape[0], bw)
+    return A
+
+
+def band_solve(lu: np.ndarray, bw: int, b: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(b, dtype=np.complex128).copy()
+    lib().orc_band_solve(_p(np.ascontiguousarray(lu)), lu.shape[0], bw, x.ctypes.data_as(_dp))
+    return x
+
+
+def band_matvec(A: np.ndarray, bw: int, x: np.ndarray) -> np.ndarray:
+    xx = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(xx)
+    lib().orc_band_matvec(_p(np.ascontiguousarray(A)), A.shape[0], bw, xx.ctypes.data_as(_dp),
+                          out.ctypes.data_as(_dp))
+    return out
+
+
+def fft(x: np.ndarray, sign: int) -> np.ndarray:
+    xx = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(xx)
+    lib().orc_fft(xx.ctypes.data_as(_dp), out.ctypes.data_as(_dp), xx.shape[0], sign)
+    return out
+
+
+# ----------------------------------------------------------------- solver
+class Oracle:
+    """One channel configuration on the CPU, reference layout throughout."""
+
+    def __init__(self, cfg: ChannelConfig):
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        self.h = lib().orc_create(ctypes.byref(self._c))
+        if not self.h:
+            raise RuntimeError("orc_create failed")
+        bw = ctypes.c_int(0)
+        lib().orc_info(self.h, ctypes.byref(bw), *([None] * 7))
+        self.bw = bw.value
+        n, W = cfg.ny, 2 * self.bw + 1
+        self.y = np.zeros(n); self.d0 = np.zeros((n, W)); self.d1 = np.zeros((n, W)); self.d2 = np.zeros((n, W))
+        self.d0lu = np.zeros((n, W)); self.integ = np.zeros(n); self.dyr = np.zeros(n)
+        lib().orc_info(self.h, ctypes.byref(bw), _p(self.y), _p(self.d0), _p(self.d1), _p(self.d2),
+                       _p(self.d0lu), _p(self.integ), _p(self.dyr))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def init_stores(self, S: np.ndarray) -> np.ndarray:
+        cfg = self.cfg
+        RK = np.zeros((3, cfg.nz, cfg.nxh, cfg.ny), dtype=np.complex128)
+        lib().orc_init_stores(self.h, S.ctypes.data_as(_dp), RK.ctypes.data_as(_dp))
+        return RK
+
+    def nonlinear(self, S: np.ndarray, want_vv: bool = False, y0: int = 0, y1: int | None = None):
+        cfg = self.cfg
+        P = np.zeros((10, cfg.nz, cfg.nxh, cfg.ny), dtype=np.complex128)
+        vm = ctypes.c_double(0.0)
+        lib().orc_nonlinear(self.h, S.ctypes.data_as(_dp), P.ctypes.data_as(_dp), int(want_vv), y0,
+                            cfg.ny if y1 is None else y1, ctypes.byref(vm))
+        return P, vm.value
+
+    def opp_planes(self, S: np.ndarray) -> np.ndarray:
+        cfg = self.cfg
+        out = np.zeros((2, cfg.nz, cfg.nxh), dtype=np.complex128)
+        lib().orc_opp_planes(self.h, S.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+        return out
+
+    def wavespace(self, s: int, dt: float, P: np.ndarray, RK: np.ndarray, vd: np.ndarray | None = None):
+        cfg = self.cfg
+        mean = np.zeros(4)
+        lib().orc_wavespace(self.h, s, dt, P.ctypes.data_as(_dp), RK.ctypes.data_as(_dp),
+                            None if vd is None else vd.ctypes.data_as(_dp), 0, cfg.nxh * cfg.nz, _p(mean))
+        return mean
+
+    def substep(self, S: np.ndarray, RK: np.ndarray, s: int, dt: float):
+        out = np.zeros(6)
+        d = ctypes.c_double(dt)
+        r = lib().orc_substep(self.h, S.ctypes.data_as(_dp), RK.ctypes.data_as(_dp), s, ctypes.byref(d), _p(out))
+        if r:
+            raise RuntimeError(f"orc_substep failed: {r}")
+        return dict(zip(["dt", "velmax", "phi_u", "phi_t", "tau_w", "q_w"], out))
+
+    def step(self, S, RK, dt):
+        outs = [self.substep(S, RK, s, dt) for s in (1, 2, 3)]
+        return outs
+
+
+def sample_times(cfg: ChannelConfig, nplanes: int, nlines: int, s: int = 2):
+    """Bounded CPU baseline: seconds for the nonlinear stage on `nplanes`
+    y planes and for the wavespace on `nlines` (kx,kz) systems of the full
+    grid; returns (t_nonlin, t_wave, threads)."""
+    L = lib()
+    L.orc_sample.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]
+    L.orc_sample.restype = ctypes.c_int
+    o = Oracle(cfg)
+    a, b = ctypes.c_double(0), ctypes.c_double(0)
+    nt = L.orc_sample(o.h, nplanes, nlines, s, cfg.dt, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value, nt

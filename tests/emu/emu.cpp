@@ -1,0 +1,82 @@
+// CPU build of the product's set-up code and per-system solver
+// (channel_passive_amd/csrc/bspline.cpp, solver.h), used only by tests to
+// check the kernel arithmetic bit-for-bit against the oracle without a GPU.
+// The product never loads this library.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../channel_passive_amd/csrc/bspline.h"
+#include "../../channel_passive_amd/csrc/solver.h"
+#include "../../include/pb_capi.h"
+
+namespace {
+struct Setup {
+  int n, bw, W, order;
+  pb::Collocation col;
+  std::vector<double> d0, d1, d2, d0lu;
+};
+int make_setup(const pb_config *cfg, Setup &S) {
+  S.order = cfg->labs_k + 1;
+  S.n = (int)cfg->ny;
+  if (pb::build_collocation(S.order, S.n, cfg->knot_map, cfg->stretch, 2.0 * cfg->lmfy, S.col)) return -1;
+  S.bw = pb::eval_bandwidth(S.col);
+  S.W = 2 * S.bw + 1;
+  pb::to_band(S.col.D[0], S.n, S.bw, S.d0);
+  pb::to_band(S.col.D[1], S.n, S.bw, S.d1);
+  pb::to_band(S.col.D[2], S.n, S.bw, S.d2);
+  S.d0lu = S.d0;
+  pb::OwnBand ob{S.d0lu.data(), 1, S.W};
+  pb::band_lu(pb::BandGeom{S.n, S.bw}, ob);
+  return 0;
+}
+pb::LineParams params(const pb_config *cfg, const Setup &S, int s, double dt) {
+  pb::LineParams P;
+  P.g = pb::BandGeom{S.n, S.bw};
+  P.D0 = pb::SharedBand{S.d0.data(), S.W};
+  P.D1 = pb::SharedBand{S.d1.data(), S.W};
+  P.D2 = pb::SharedBand{S.d2.data(), S.W};
+  P.D0LU = pb::SharedBand{S.d0lu.data(), S.W};
+  P.integ = S.col.integ.data();
+  pb::RK3 rk = pb::rk3_coefficients();
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 3; ++b) P.rk[a][b] = rk.c[a][b];
+  P.dt = dt; P.nu = cfg->nu; P.pr = cfg->pr; P.bulk_u = cfg->bulk_u; P.bulk_w = cfg->bulk_w;
+  P.bulk_t = cfg->bulk_t; P.u_gal = cfg->u_gal; P.s = s; P.S = cfg->rk_step;
+  return P;
+}
+}  // namespace
+
+extern "C" {
+int emu_setup(const pb_config *cfg, int *bw, double *y, double *d0, double *d1, double *d2, double *d0lu,
+              double *integ) {
+  Setup S;
+  if (make_setup(cfg, S)) return -1;
+  *bw = S.bw;
+  const size_t nb = (size_t)S.n * S.W;
+  if (y) std::memcpy(y, S.col.y.data(), S.n * 8);
+  if (d0) std::memcpy(d0, S.d0.data(), nb * 8);
+  if (d1) std::memcpy(d1, S.d1.data(), nb * 8);
+  if (d2) std::memcpy(d2, S.d2.data(), nb * 8);
+  if (d0lu) std::memcpy(d0lu, S.d0lu.data(), nb * 8);
+  if (integ) std::memcpy(integ, S.col.integ.data(), S.n * 8);
+  return 0;
+}
+
+// U: 9 slots, RK: 3 slots, device layout [slot][y][kz][kx]; vd: 2 planes or NULL
+int emu_wavespace(const pb_config *cfg, int s, double dt, double *U, double *RK, const double *vd, double *mean4) {
+  Setup S;
+  if (make_setup(cfg, S)) return -1;
+  pb::LineParams P = params(cfg, S, s, dt);
+  const int nxh = (int)cfg->nx / 2, nz = (int)cfg->nz;
+  const size_t plane = (size_t)nz * nxh, field = plane * S.n;
+  std::vector<pb::cplx> buf(2 * field);
+  std::vector<double> dbuf((size_t)S.W * field);
+  pb::WsArgs A;
+  A.U = (pb::cplx *)U; A.RK = (pb::cplx *)RK; A.buf = buf.data(); A.dbuf = dbuf.data();
+  A.vd = (const pb::cplx *)vd; A.plane = plane; A.field = field; A.nxh = nxh; A.nz = nz; A.nh = nz / 2;
+  A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha; A.mean_out = mean4;
+  for (size_t sys = 0; sys < plane; ++sys) pb::wavespace_line(P, A, sys);
+  return 0;
+}
+}

@@ -1,0 +1,213 @@
+"""GPU parity: the HIP library (through the C ABI) against the oracle.
+
+Bars: the banded solves reproduce the oracle's operation order, so the
+wavespace stage and the stores are compared bit-for-bit on identical inputs;
+the transforms differ from the oracle's FFT only by rounding (tolerance
+1e-13 relative L2 per slot); a whole step is held to the 1e-11 relative L2
+per field of the specification.  At the full Re_tau=1000 size, where the
+oracle does not fit a test, size-independent invariants are checked.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from channel_passive_amd.config import preset
+from channel_passive_amd.initial import make_state
+
+pytestmark = pytest.mark.gpu
+
+STEP_TOL = 1e-11
+FFT_TOL = 1e-13
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def configs():
+    small = preset("re180_64")
+    opp = preset("re180_192")
+    thin385 = dataclasses.replace(preset("re1000"), nx=16, nz=12, name="re1000_thin")  # ny = 385
+    tiny_sin = dataclasses.replace(preset("tiny"), knot_map=0, stretch=0.9, labs_k=6, name="tiny_sin")
+    return {"re180_64": small, "re180_192": opp, "re1000_thin": thin385, "tiny_sin": tiny_sin}
+
+
+CFGS = configs()
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from channel_passive_amd import capi
+    return capi
+
+
+def developed(cfg, orc, nsteps=1, seed=11):
+    S = make_state(cfg, seed=seed)
+    RK = orc.init_stores(S)
+    for _ in range(nsteps):
+        orc.step(S, RK, cfg.dt)
+    return S, RK
+
+
+@pytest.mark.parametrize("name", ["re180_64", "re1000_thin", "tiny_sin"])
+def test_labs_bitwise(lib, oracle_lib, name):
+    cfg = CFGS[name]
+    orc = oracle_lib.Oracle(cfg)
+    ch = lib.Channel(cfg)
+    L = ch.labs()
+    assert ch.bw == orc.bw
+    for k, ref in (("y", orc.y), ("d0", orc.d0), ("d1", orc.d1), ("d2", orc.d2), ("d0lu", orc.d0lu),
+                   ("integ", orc.integ)):
+        assert np.array_equal(L[k], ref), k
+    ch.close()
+
+
+@pytest.mark.parametrize("name", ["re180_64", "re1000_thin"])
+def test_init_stores_bitwise(lib, oracle_lib, name):
+    cfg = CFGS[name]
+    orc = oracle_lib.Oracle(cfg)
+    S = make_state(cfg, seed=3)
+    RK = orc.init_stores(S)
+    ch = lib.Channel(cfg)
+    ch.set_state(S)
+    ch.init_stores()
+    S2, RK2 = ch.get_state()
+    assert np.array_equal(S2, S)
+    for i in range(3):
+        assert np.array_equal(RK2[i], RK[i]), i
+    ch.close()
+
+
+@pytest.mark.parametrize("name", ["re180_64", "re180_192", "tiny_sin"])
+def test_nonlinear_stage(lib, oracle_lib, name):
+    cfg = CFGS[name]
+    orc = oracle_lib.Oracle(cfg)
+    S, RK = developed(cfg, orc)
+    P_ref, vm_ref = orc.nonlinear(S)
+    ch = lib.Channel(cfg)
+    P, vm = ch.nonlinear_host(S)
+    for slot in (0, 1, 2, 3, 4, 6, 7, 8):
+        e = rel(P[slot], P_ref[slot])
+        print(f"{name} slot {slot}: {e:.2e}")
+        assert e <= FFT_TOL, (slot, e)
+    # kz Nyquist slot is exactly zero
+    assert not np.any(P[:9, cfg.nz // 2])
+    assert abs(vm - vm_ref) <= 1e-13 * vm_ref
+    ch.close()
+
+
+@pytest.mark.parametrize("name", ["re180_64", "re180_192", "re1000_thin", "tiny_sin"])
+@pytest.mark.parametrize("s", [1, 2, 3])
+def test_wavespace_bitwise(lib, oracle_lib, name, s):
+    cfg = CFGS[name]
+    orc = oracle_lib.Oracle(cfg)
+    S, RK = developed(cfg, orc)
+    P, _ = orc.nonlinear(S)
+    vd = orc.opp_planes(S) if cfg.opp_alpha else None
+    v_old = np.ascontiguousarray(S[1:2]) if cfg.opp_alpha else None
+    Pg, RKg = P.copy(), RK.copy()
+    mean = orc.wavespace(s, cfg.dt, P, RK, vd)
+    ch = lib.Channel(cfg)
+    out = ch.wavespace_host(s, cfg.dt, Pg, RKg, v_old)
+    for slot in (0, 1, 2, 6):
+        assert np.array_equal(Pg[slot], P[slot]), (slot, rel(Pg[slot], P[slot]))
+    for i in range(3):
+        assert np.array_equal(RKg[i], RK[i]), (i, rel(RKg[i], RK[i]))
+    assert out["phi_u"] == mean[0] and out["phi_t"] == mean[1]
+    if s == 3:
+        assert out["tau_w"] == mean[2] and out["q_w"] == mean[3]
+    ch.close()
+
+
+@pytest.mark.parametrize("name", ["re180_64", "re180_192", "re1000_thin"])
+def test_step_parity(lib, oracle_lib, name):
+    cfg = CFGS[name]
+    orc = oracle_lib.Oracle(cfg)
+    S, RK = developed(cfg, orc)
+    ch = lib.Channel(cfg)
+    ch.set_state(S, RK)
+    outs = ch.step(cfg.dt)
+    Sg, RKg = ch.get_state()
+    oouts = orc.step(S, RK, cfg.dt)
+    errs = [rel(Sg[i], S[i]) for i in range(4)] + [rel(RKg[i], RK[i]) for i in range(3)]
+    print(name, "step rel L2 u v w th g phi th*:", ["%.2e" % e for e in errs])
+    assert max(errs) <= STEP_TOL
+    # phi = (bulk - Re(I)/2) / (Im(I)/2) cancels in its numerator (wave00.f90:710): its
+    # rounding is set by the bulk value, not by phi itself
+    for a, b in zip(outs, oouts):
+        assert abs(a["phi_u"] - b["phi_u"]) <= 1e-11 * abs(b["phi_u"]) + 1e-14 * cfg.bulk_u_value()
+        assert abs(a["phi_t"] - b["phi_t"]) <= 1e-11 * abs(b["phi_t"]) + 1e-14 * cfg.bulk_t
+    assert abs(outs[-1]["tau_w"] - oouts[-1]["tau_w"]) <= 1e-11 * abs(oouts[-1]["tau_w"])
+    ch.close()
+
+
+def test_multistep_parity(lib, oracle_lib):
+    """Five steps: trajectories of this random initial state separate by about
+    x5 per step from rounding alone, so the multi-step bar is 1e-9 (the 1e-11
+    bar of the specification is per step, see test_step_parity)."""
+    cfg = CFGS["re180_64"]
+    orc = oracle_lib.Oracle(cfg)
+    S, RK = developed(cfg, orc, nsteps=0, seed=5)
+    ch = lib.Channel(cfg)
+    ch.set_state(S, RK)
+    for _ in range(5):
+        ch.step(cfg.dt)
+        orc.step(S, RK, cfg.dt)
+    Sg, RKg = ch.get_state()
+    errs = [rel(Sg[i], S[i]) for i in range(4)]
+    print("5-step rel L2:", ["%.2e" % e for e in errs])
+    assert max(errs) <= 1e-9
+    ch.close()
+
+
+def test_full_size_invariants(lib, oracle_lib):
+    """Re_tau = 1000, 1024 x 385 x 768 with opposition control: one step on the
+    GPU, then divergence, wall derivative, Hermitian symmetry, the empty kz
+    Nyquist slot and the flow rate on a sample of (kx,kz) columns."""
+    from oracle import oracle as O
+    cfg = preset("re1000")
+    print("[full] make_state", flush=True)
+    S = make_state(cfg, seed=0xC0FFEE + 2)
+    ch = lib.Channel(cfg)
+    ch.set_state(S)
+    del S
+    ch.init_stores()
+    print("[full] step", flush=True)
+    out = ch.step(cfg.dt)
+    assert all(np.isfinite(list(o.values())).all() for o in out)
+    print("[full] get_state", out[-1], flush=True)
+    Sg, RKg = ch.get_state()
+    ch.close()
+    nh = cfg.nz // 2
+    # sample columns only: the full arrays are 17 GB on the host
+    orc = oracle_lib.Oracle(dataclasses.replace(cfg, nx=16, nz=12))  # same wall-normal operators
+    kxp = np.arange(cfg.nxh) / cfg.lmfx
+    kzi = np.concatenate([np.arange(nh + 1), np.arange(nh + 2, cfg.nz + 1) - 2 * nh - 1])
+    kzp = kzi / cfg.lmfz
+    rng = np.random.default_rng(0)
+    cols = [(int(rng.integers(0, 40)), int(rng.integers(0, cfg.nz))) for _ in range(200)]
+    print("[full] column checks", flush=True)
+    for k, j in cols:
+        assert np.isfinite(Sg[:, j, k]).all() and np.isfinite(RKg[:, j, k]).all()
+        if k == 0 and kzi[j] == 0:
+            continue
+        u, v, w = Sg[0, j, k], Sg[1, j, k], Sg[2, j, k]
+        c = O.band_solve(orc.d0lu, orc.bw, v)
+        dv = O.band_matvec(orc.d1, orc.bw, c)
+        div = 1j * kxp[k] * u + dv + 1j * kzp[j] * w
+        scale = np.max(np.abs(dv)) + np.max(np.abs(kxp[k] * u)) + np.max(np.abs(kzp[j] * w)) + 1e-300
+        assert np.max(np.abs(div)) <= 1e-9 * scale, (k, j)
+        assert abs(dv[0]) <= 1e-9 * scale and abs(dv[-1]) <= 1e-9 * scale
+    # Hermitian symmetry on the kx = 0 plane
+    plane0 = Sg[:, :, 0, :]
+    amax = np.abs(plane0).max()
+    for j in range(1, nh):
+        assert np.allclose(plane0[:, j], np.conj(plane0[:, cfg.nz - j]), rtol=0, atol=1e-12 * amax)
+    # kz Nyquist stays empty
+    assert not np.any(Sg[:, nh])
+    # flow rate held: 0.5 * integral of U = bulk_u
+    cU = O.band_solve(orc.d0lu, orc.bw, Sg[0, 0, 0].copy())
+    assert abs(0.5 * np.dot(orc.integ, cU.real) - cfg.bulk_u_value()) <= 1e-12 * cfg.bulk_u_value()
+    print("[full] done", flush=True)
