@@ -1,0 +1,41 @@
+"""The product's per-system solver (channel_passive_amd/csrc/solver.h) built
+for the CPU equals the oracle bit for bit, on every substep, with and
+without opposition control.  This is the same source the GPU kernel runs."""
+import ctypes
+import dataclasses
+
+import numpy as np
+import pytest
+
+from channel_passive_amd.config import preset
+from channel_passive_amd.initial import make_state
+
+dp = ctypes.POINTER(ctypes.c_double)
+
+
+@pytest.mark.parametrize("name,alpha", [("tiny", 0.0), ("tiny", 0.8), ("re180_64", 0.0)])
+def test_wavespace_bitwise_cpu(oracle_lib, emu_lib, name, alpha):
+    cfg = dataclasses.replace(preset(name), opp_alpha=alpha, opp_yplus=20.0)
+    orc = oracle_lib.Oracle(cfg)
+    c = cfg.to_c()
+    S = make_state(cfg, seed=13)
+    RK = orc.init_stores(S)
+    orc.step(S, RK, cfg.dt)
+    for s in (1, 2, 3):
+        P, _ = orc.nonlinear(S)
+        vd = orc.opp_planes(S) if alpha else None
+        Ud = np.ascontiguousarray(np.transpose(P[:9], (0, 3, 1, 2)))
+        RKd = np.ascontiguousarray(np.transpose(RK, (0, 3, 1, 2)))
+        m_e = np.zeros(4)
+        assert emu_lib.emu_wavespace(ctypes.byref(c), s, ctypes.c_double(cfg.dt), Ud.ctypes.data_as(dp),
+                                     RKd.ctypes.data_as(dp), None if vd is None else vd.ctypes.data_as(dp),
+                                     m_e.ctypes.data_as(dp)) == 0
+        m_o = orc.wavespace(s, cfg.dt, P, RK, vd)
+        Ue, RKe = np.transpose(Ud, (0, 2, 3, 1)), np.transpose(RKd, (0, 2, 3, 1))
+        for f in (0, 1, 2, 6):
+            assert np.array_equal(Ue[f], P[f]), (s, f)
+        for i in range(3):
+            assert np.array_equal(RKe[i], RK[i]), (s, i)
+        assert np.array_equal(m_e, m_o)
+        for i, f in enumerate((0, 1, 2, 6)):
+            S[i] = P[f]

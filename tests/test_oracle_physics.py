@@ -1,0 +1,117 @@
+"""Closed forms and invariants of the substep, pinned on the oracle.
+
+* laminar Poiseuille flow with a uniform heat source is an exact steady
+  state: U and Theta unchanged, tau_w = 3 nu U_b, q_w = 3 (nu/Pr) Theta_b
+  (wave00.f90:210-211, 621-622), mean pressure gradient sum(phi)/dt = 3 nu U_b
+* the flow rate and the bulk scalar are held exactly (calculate_phi)
+* one step keeps i kx u + dv/dy + i kz w = 0, v = dv/dy = 0 at the walls,
+  Hermitian symmetry on kx = 0 and an empty kz Nyquist slot
+* opposition control sets v(+-1) = -alpha v(y_d) exactly
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from channel_passive_amd.config import preset, physical_wavenumbers
+from channel_passive_amd.initial import make_state
+
+
+def laminar(cfg, orc):
+    S = np.zeros((4, cfg.nz, cfg.nxh, cfg.ny), complex)
+    y = orc.y
+    S[0, 0, 0] = 1.5 * cfg.bulk_u_value() * (1 - y * y)
+    S[3, 0, 0] = 1.5 * cfg.bulk_t * (1 - y * y)
+    return S
+
+
+@pytest.mark.parametrize("name", ["re180_64", "tiny"])
+def test_laminar_steady_state(oracle_lib, name):
+    cfg = preset(name)
+    orc = oracle_lib.Oracle(cfg)
+    S = laminar(cfg, orc)
+    S0 = S.copy()
+    RK = orc.init_stores(S)
+    outs = orc.step(S, RK, cfg.dt)
+    ub, nu = cfg.bulk_u_value(), cfg.nu
+    assert np.abs(S[0, 0, 0] - S0[0, 0, 0]).max() <= 1e-12 * ub
+    assert np.abs(S[3, 0, 0] - S0[3, 0, 0]).max() <= 1e-12 * cfg.bulk_t
+    assert np.abs(S[1:3]).max() <= 1e-15 * ub  # FFT roundoff only
+    assert abs(outs[-1]["tau_w"] - 3 * nu * ub) <= 1e-10 * 3 * nu * ub
+    assert abs(outs[-1]["q_w"] - 3 * nu / cfg.pr * cfg.bulk_t) <= 1e-10 * 3 * nu / cfg.pr
+    dpdx = sum(o["phi_u"] for o in outs) / cfg.dt
+    assert abs(dpdx - 3 * nu * ub) <= 1e-8 * 3 * nu * ub
+
+
+def divergence_and_walls(orc, cfg, S):
+    from oracle import oracle as O
+    _, kzi, kxp, kzp = physical_wavenumbers(cfg)
+    worst = 0.0
+    for j in range(cfg.nz):
+        for k in range(cfg.nxh):
+            if k == 0 and j == 0:
+                continue
+            u, v, w = S[0, j, k], S[1, j, k], S[2, j, k]
+            dv = O.band_matvec(orc.d1, orc.bw, O.band_solve(orc.d0lu, orc.bw, v))
+            scale = max(np.abs(dv).max(), np.abs(kxp[k] * u).max(), np.abs(kzp[j] * w).max(), 1e-300)
+            div = 1j * kxp[k] * u + dv + 1j * kzp[j] * w
+            worst = max(worst, np.abs(div).max() / scale, abs(dv[0]) / scale, abs(dv[-1]) / scale)
+    return worst
+
+
+@pytest.mark.parametrize("name", ["tiny", "re180_64"])
+def test_step_invariants(oracle_lib, name):
+    cfg = preset(name)
+    orc = oracle_lib.Oracle(cfg)
+    S = make_state(cfg, seed=21)
+    RK = orc.init_stores(S)
+    orc.step(S, RK, cfg.dt)
+    assert divergence_and_walls(orc, cfg, S) <= 1e-11
+    assert np.abs(S[1][..., 0]).max() == 0.0 and np.abs(S[1][..., -1]).max() == 0.0
+    nh = cfg.nz // 2
+    for j in range(1, nh):
+        assert np.allclose(S[:, j, 0], np.conj(S[:, cfg.nz - j, 0]), rtol=0, atol=1e-13 * np.abs(S).max())
+    assert not np.any(S[:, nh])
+    from oracle import oracle as O
+    cU = O.band_solve(orc.d0lu, orc.bw, S[0, 0, 0].copy())
+    assert abs(0.5 * orc.integ @ cU.real - cfg.bulk_u_value()) <= 1e-13 * cfg.bulk_u_value()
+    cT = O.band_solve(orc.d0lu, orc.bw, S[3, 0, 0].copy())
+    assert abs(0.5 * orc.integ @ cT.real - cfg.bulk_t) <= 1e-13
+
+
+def test_opposition_control_wall_values(oracle_lib):
+    cfg = dataclasses.replace(preset("tiny"), opp_alpha=0.7, opp_yplus=25.0)
+    orc = oracle_lib.Oracle(cfg)
+    S = make_state(cfg, seed=4)
+    RK = orc.init_stores(S)
+    for s in (1, 2, 3):
+        vd = orc.opp_planes(S)
+        orc.substep(S, RK, s, cfg.dt)
+        lo, hi = S[1][..., 0], S[1][..., -1]
+        mask = np.ones(lo.shape, bool)
+        mask[0, 0] = False  # the mean mode has no wall-normal velocity
+        assert np.array_equal(lo[mask], (-cfg.opp_alpha * vd[0])[mask])
+        assert np.array_equal(hi[mask], (-cfg.opp_alpha * vd[1])[mask])
+    assert divergence_and_walls.__name__  # divergence still holds with moving walls:
+    from oracle import oracle as O
+    _, _, kxp, kzp = physical_wavenumbers(cfg)
+    for j in range(cfg.nz):
+        for k in range(1, cfg.nxh):
+            u, v, w = S[0, j, k], S[1, j, k], S[2, j, k]
+            dv = O.band_matvec(orc.d1, orc.bw, O.band_solve(orc.d0lu, orc.bw, v))
+            div = 1j * kxp[k] * u + dv + 1j * kzp[j] * w
+            assert np.abs(div).max() <= 1e-11 * max(np.abs(dv).max(), 1e-300)
+
+
+def test_opposition_detection_plane(oracle_lib):
+    """v(y_d) is B(y_d) . D0^-1 v: for a polynomial profile of degree < order
+    the interpolation is exact."""
+    cfg = dataclasses.replace(preset("tiny"), opp_alpha=1.0, opp_yplus=18.0)
+    orc = oracle_lib.Oracle(cfg)
+    y = orc.y
+    S = np.zeros((4, cfg.nz, cfg.nxh, cfg.ny), complex)
+    S[1, 2, 3] = (1 - y * y) ** 2 * (0.3 + 0.1j)
+    vd = orc.opp_planes(S)
+    yd = 1.0 - cfg.opp_yplus / cfg.re_tau
+    exact = (1 - yd * yd) ** 2 * (0.3 + 0.1j)
+    assert abs(vd[0, 2, 3] - exact) <= 1e-14 and abs(vd[1, 2, 3] - exact) <= 1e-14
