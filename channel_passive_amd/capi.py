@@ -45,7 +45,11 @@ SIGNATURES = {
     "pb_get_labs": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 6),
     "pb_set_state": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "pb_get_state": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "pb_set_state_slots": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 5),
+    "pb_get_state_slots": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 5),
     "pb_init_stores": (ctypes.c_int, [ctypes.c_void_p]),
+    "pb_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "pb_timing_read": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "pb_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(PbSubstepOut)]),
     "pb_step": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(PbSubstepOut)]),
     "pb_sync": (ctypes.c_int, [ctypes.c_void_p]),

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -75,7 +76,7 @@ struct pb_ctx {
   hipStream_t stream = nullptr;
   // device memory
   double2 *U = nullptr, *RK = nullptr, *pad = nullptr, *buf = nullptr, *vd = nullptr;
-  double *dbuf = nullptr, *vpart = nullptr, *scal = nullptr;
+  double *dbuf = nullptr, *dbuf2 = nullptr, *vpart = nullptr, *scal = nullptr;
   double *dD0 = nullptr, *dD1 = nullptr, *dD2 = nullptr, *dD0LU = nullptr, *dinteg = nullptr, *ddyr = nullptr,
          *drows = nullptr;
   double2 *twz = nullptr, *twx = nullptr;
@@ -83,6 +84,7 @@ struct pb_ctx {
   double *hscal = nullptr;  // pinned host mirror of scal
   size_t bytes = 0;
   bool have_state = false;
+  bool fused = true;  // fused-sweep wall-normal solver (bit-identical to the statement-by-statement port)
   pb::RK3 rk;
   // per-stage event timing
   bool timing = false;
@@ -122,7 +124,7 @@ const char *pb_last_error(void) { return g_err.c_str(); }
 
 static int ctx_free(pb_ctx *c) {
   if (!c) return 0;
-  void *ptrs[] = {c->U, c->RK, c->pad, c->buf, c->vd, c->dbuf, c->vpart, c->scal, c->dD0, c->dD1, c->dD2,
+  void *ptrs[] = {c->U, c->RK, c->pad, c->buf, c->vd, c->dbuf, c->dbuf2, c->vpart, c->scal, c->dD0, c->dD1, c->dD2,
                   c->dD0LU, c->dinteg, c->ddyr, c->drows, c->twz, c->twx, c->slotmap};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -157,6 +159,7 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   c->lmfxr = 1.0 / cfg->lmfx;
   c->lmfzr = 1.0 / cfg->lmfz;
   c->rk = pb::rk3_coefficients();
+  if (const char *e = getenv("PB_WAVESPACE_LITERAL")) c->fused = (e[0] == '0');
   int r = pb::build_collocation(c->order, c->ny, cfg->knot_map, cfg->stretch, 2.0 * cfg->lmfy, c->col);
   if (r) { ctx_free(c); return fail(PB_E_ARG, "B-spline set-up failed (%d): check ny, labs_k, knots", r); }
   c->bw = pb::eval_bandwidth(c->col);
@@ -240,6 +243,7 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   rc = rc ? rc : dalloc((void **)&c->buf, 2 * c->field * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->vd, 2 * c->plane * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->dbuf, (size_t)c->W * c->field * sizeof(double));
+  rc = rc ? rc : dalloc((void **)&c->dbuf2, (size_t)c->W * c->field * sizeof(double));
   rc = rc ? rc : dalloc((void **)&c->vpart, (size_t)c->ny * c->nz32 * sizeof(double));
   rc = rc ? rc : dalloc((void **)&c->scal, SCAL_N * sizeof(double));
   rc = rc ? rc : dalloc((void **)&c->dD0, bandb);
@@ -341,6 +345,38 @@ int pb_get_state(pb_ctx *c, double *U, double *rk) {
   return PB_OK;
 }
 
+int pb_set_state_slots(pb_ctx *c, const double *u, const double *v, const double *w, const double *theta,
+                       const double *rk) {
+  if (!c || !u || !v || !w || !theta) return fail(PB_E_ARG, "null argument");
+  const double *f[4] = {u, v, w, theta};
+  for (int i = 0; i < 4; ++i) {
+    int r = upload_ref(c, f[i], 1, &kStateMap[i], c->U);
+    if (r) return r;
+  }
+  if (rk) {
+    int r = upload_ref(c, rk, 3, kRKMap, c->RK);
+    if (r) return r;
+  }
+  c->have_state = true;
+  return PB_OK;
+}
+
+int pb_get_state_slots(pb_ctx *c, double *u, double *v, double *w, double *theta, double *rk) {
+  if (!c) return fail(PB_E_ARG, "null context");
+  if (!c->have_state) return fail(PB_E_STATE, "no state set");
+  double *f[4] = {u, v, w, theta};
+  for (int i = 0; i < 4; ++i)
+    if (f[i]) {
+      int r = download_ref(c, f[i], 1, &kStateMap[i], c->U);
+      if (r) return r;
+    }
+  if (rk) {
+    int r = download_ref(c, rk, 3, kRKMap, c->RK);
+    if (r) return r;
+  }
+  return PB_OK;
+}
+
 static unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 // Record an event pair around a launch when timing is on.
@@ -420,6 +456,29 @@ static int launch_nonlinear(pb_ctx *c, int update_dt) {
   return PB_OK;
 }
 
+extern "C++" {
+template <int BW>
+static int launch_sweeps(pb_ctx *c, const pb::LineParams &P, const pb::WsArgs &A, int s) {
+  const dim3 grid(blocks_for(c->nsys, 64)), block(64);
+#define SWEEP(id)                                                                                          \
+  hipLaunchKernelGGL((pb::sweep_kernel<BW, id>), grid, block, 0, c->stream, P, A, c->dbuf2, c->nsys, c->scal); \
+  HIPCHK(hipGetLastError());
+  if (s == 1) {
+    SWEEP(pb::SW_A1);
+    SWEEP(pb::SW_B1);
+  }
+  SWEEP(pb::SW_A);
+  SWEEP(pb::SW_B);
+  SWEEP(pb::SW_C);
+  SWEEP(pb::SW_D);
+  SWEEP(pb::SW_E);
+  SWEEP(pb::SW_F);
+  SWEEP(pb::SW_G);
+#undef SWEEP
+  return PB_OK;
+}
+}  // extern "C++"
+
 static int launch_wavespace(pb_ctx *c, int s) {
   pb::LineParams P = c->line_params(s, 0.0);
   pb::WsArgs A;
@@ -438,6 +497,15 @@ static int launch_wavespace(pb_ctx *c, int s) {
   A.opp_alpha = c->cfg.opp_alpha;
   A.mean_out = c->scal + 2 + 4 * (s - 1);
   StageTimer tm(c, PB_STAGE_WAVESPACE);
+  if (c->fused && (c->bw >= 3 && c->bw <= 5)) {
+    hipLaunchKernelGGL(pb::wave00_kernel, dim3(1), dim3(64), 0, c->stream, P, A, c->scal);
+    HIPCHK(hipGetLastError());
+    switch (c->bw) {
+      case 3: return launch_sweeps<3>(c, P, A, s);
+      case 4: return launch_sweeps<4>(c, P, A, s);
+      case 5: return launch_sweeps<5>(c, P, A, s);
+    }
+  }
   hipLaunchKernelGGL(pb::wavespace_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P, A, c->nsys,
                      c->scal);
   HIPCHK(hipGetLastError());

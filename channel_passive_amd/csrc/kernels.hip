@@ -15,6 +15,7 @@
 
 #include "pb_common.h"
 #include "solver.h"
+#include "solver_fused.h"
 
 namespace pb {
 
@@ -325,6 +326,45 @@ __global__ void __launch_bounds__(64) wavespace_kernel(LineParams P, WsArgs A, s
   LineParams Q = P;
   Q.dt = scal[0];
   wavespace_line(Q, A, sys);
+}
+
+// The (kx,kz) = (0,0) system: mean flow, mean spanwise velocity and mean
+// scalar with the flux constraints (wave00.f90); one thread.
+__global__ void wave00_kernel(LineParams P, WsArgs A, const double *scal) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  LineParams Q = P;
+  Q.dt = scal[0];
+  wavespace_line(Q, A, 0);
+}
+
+// Fused sweeps over all (kx,kz) != (0,0) systems, one thread per system.
+enum SweepId { SW_A1, SW_B1, SW_A, SW_B, SW_C, SW_D, SW_E, SW_F, SW_G };
+
+template <int BW, int SW>
+__global__ void __launch_bounds__(64) sweep_kernel(LineParams P, WsArgs A, double *dbuf2, size_t nsys,
+                                                   const double *scal) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  SysCtx c;
+  if (!make_sys(A, P.g.n, sys, c)) return;
+  LineParams Q = P;
+  Q.dt = scal[0];
+  if (SW == SW_A1) sweep_A1<BW>(Q, A, c);
+  if (SW == SW_B1) sweep_B1<BW>(Q, A, c);
+  if (SW == SW_A) sweep_A<BW>(Q, A, c);
+  if (SW == SW_B) sweep_B<BW>(Q, A, c);
+  if (SW == SW_C) sweep_C<BW>(Q, A, c, dbuf2);
+  if (SW == SW_D) sweep_D<BW>(Q, A, c, dbuf2);
+  if (SW == SW_E) {
+    cplx vlo = mk(0.0, 0.0), vhi = mk(0.0, 0.0);
+    if (A.opp_alpha != 0.0 && A.vd) {
+      vlo = cscale(A.vd[sys], -A.opp_alpha);
+      vhi = cscale(A.vd[A.plane + sys], -A.opp_alpha);
+    }
+    sweep_E<BW>(Q, A, c, vlo, vhi);
+  }
+  if (SW == SW_F) sweep_F<BW>(Q, A, c);
+  if (SW == SW_G) sweep_G(Q, A, c);
 }
 
 // opposition-control detection planes: vd[0][sys], vd[1][sys]

@@ -98,7 +98,12 @@ struct LineParams {
   double rk[4][3];
   double dt, nu, pr, bulk_u, bulk_w, bulk_t, u_gal;
   int s, S;  // substep (1-based), steps per RK step
-  PB_HD double coef(int row, int sub) const { return rk[row][sub - 1]; }
+  // row is a literal at every call site; sub is selected without indexing so
+  // that the parameter block stays in registers (no private-memory copy)
+  PB_HD double coef(int row, int sub) const {
+    const double *c = rk[row];
+    return sub == 1 ? c[0] : (sub == 2 ? c[1] : c[2]);
+  }
 };
 
 // Helmholtz LHS with diagonal-only wall rows (solve_g.f90:42-71, wave00.f90:58-82)

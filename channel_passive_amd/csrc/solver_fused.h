@@ -1,0 +1,575 @@
+// solver_fused.h -- the per-(kx,kz) wall-normal solve of one substep,
+// reorganised into seven sweeps over y per system.
+//
+// Each value is produced by exactly the same floating-point expression as
+// in the statement-by-statement port (solver.h, which follows hvhg.f90,
+// solve_g.f90, solve_v_f.f90, solve_u_w.f90, solve_t.f90 and the packed band
+// kernels of module_labs.f90), so the results are bit-identical; only the
+// order in which independent values are produced changes:
+//
+//   A  (down) hvhg point-wise terms + forward substitution of 4 RHS with LU(D0)
+//   B  (up)   back substitution + D1/D2 products -> h_g, h_v, h_theta, the
+//             Helmholtz right-hand sides and the zeta*h part of the stores
+//   C  (down) Crout LU of H_nu and H_theta row by row (matrix entries formed
+//             on the fly from D0, D2 and k^2) fused with the forward
+//             substitution of g, phi_p, phi_h and theta
+//   D  (up)   back substitution + D0/D2 products -> g, theta, phi_p, phi_h,
+//             D2 phi_h and the stores of g and theta
+//   E  (down) Crout LU of the Poisson operator + forward substitution of v_p, v_h
+//   F  (up)   back substitution + D0/D1 products -> v_p, v_p', v_h', D0 c_h
+//   G  (mirror pairs) influence coefficients C1, C2, f = v', v, u, w, store_phi
+//
+// Vectors live in the product slots of the system (layout [y][kz][kx]); a
+// back substitution's last 2bw+1 results are held in a register shift
+// window, which is also exactly the operand of the band products of the row
+// that becomes complete (first(m) = r, last(m) = r + 2bw for every emitted
+// row m), so every window index is a compile-time constant.
+//
+// Slot use per system (0-based product slots; s = substep):
+//   after A: 0 yB, 1 P2, 2 yE, 3 P4, 4 yF, 6 u1a, 7 y8, 8 u7a
+//   after B: 0 rhs_g, 2 rhs_phi, 4 rhs_theta, RK = zeta*h (s < S)
+//   after C: 0 y_g, 1 y_phih, 2 y_phip, 4 y_theta; U factors in dbuf, dbuf2
+//   after D: 0 g, 1 phi_h, 2 phi_p, 3 D2 phi_h, 6 theta; RK updated
+//   after E: 2 y_vp, 5 y_vh; U factor in dbuf
+//   after F: 2 v_p, 5 v_h', 7 v_p', 8 D0 c_h
+//   after G: 0 u, 1 v, 2 w, 6 theta; RK2 = store_phi
+// s == 1 adds a down/up pair (A1/B1) that forms D2 LU(D0)^-1 of the three
+// stores in scratch columns buf0, buf1 and slot 5.
+#pragma once
+
+#include "solver.h"
+
+namespace pb {
+
+template <int N>
+struct CWin {
+  cplx w[N];
+  PB_HD void clear() {
+#pragma unroll
+    for (int q = 0; q < N; ++q) w[q] = mk(0.0, 0.0);
+  }
+  PB_HD void push(cplx x) {
+#pragma unroll
+    for (int q = N - 1; q > 0; --q) w[q] = w[q - 1];
+    w[0] = x;
+  }
+};
+
+template <int N>
+struct RWin {
+  double w[N];
+  PB_HD void clear() {
+#pragma unroll
+    for (int q = 0; q < N; ++q) w[q] = 0.0;
+  }
+  PB_HD void push(double x) {
+#pragma unroll
+    for (int q = N - 1; q > 0; --q) w[q] = w[q - 1];
+    w[0] = x;
+  }
+};
+
+// Per-system context shared by the sweeps.
+struct SysCtx {
+  size_t sys;
+  int n;
+  double kx, kz, k2;
+  PB_HD SV col(const WsArgs &A, int slot) const { return SV{A.U + (size_t)slot * A.field + sys, A.plane}; }
+  PB_HD SV rk(const WsArgs &A, int slot) const { return SV{A.RK + (size_t)slot * A.field + sys, A.plane}; }
+  PB_HD SV buf(const WsArgs &A, int slot) const { return SV{A.buf + (size_t)slot * A.field + sys, A.plane}; }
+};
+
+PB_HD bool make_sys(const WsArgs &A, int n, size_t sys, SysCtx &c) {
+  const int k = (int)(sys % A.nxh), j = (int)(sys / A.nxh);
+  const int kzi = (j <= A.nh) ? j : j + 1 - 2 * A.nh - 1;  // module_ppdfftw.f90:360-374
+  c.sys = sys;
+  c.n = n;
+  c.kx = k * A.lmfxr;
+  c.kz = kzi * A.lmfzr;
+  c.k2 = c.kx * c.kx + c.kz * c.kz;
+  return !(k == 0 && kzi == 0);
+}
+
+// forward substitution of one row: w.w[q] = y(r-1-q); farthest first
+template <int BW, class M>
+PB_HD cplx fwd_row(const BandGeom &g, const M &L, int r, const CWin<2 * BW> &w, cplx b) {
+  const int f = g.first(r);
+  cplx v = b;
+#pragma unroll
+  for (int q = 2 * BW - 1; q >= 0; --q) {
+    const int p = r - 1 - q;
+    if (p >= f) v = csub(v, cscale(w.w[q], L(r, g.slot(r, p))));
+  }
+  return cdivr(v, L(r, BW));
+}
+
+// back substitution of one row: w.w[q] = x(r+1+q); nearest first
+template <int BW, class M>
+PB_HD cplx bwd_row(const BandGeom &g, const M &U, int r, const CWin<2 * BW + 1> &w, cplx y) {
+  const int l = g.last(r);
+  cplx v = y;
+#pragma unroll
+  for (int q = 0; q < 2 * BW; ++q) {
+    const int c = r + 1 + q;
+    if (c <= l) v = csub(v, cscale(w.w[q], U(r, g.slot(r, c))));
+  }
+  return v;
+}
+template <int BW, class M>
+PB_HD double bwd_row_r(const BandGeom &g, const M &U, int r, const RWin<2 * BW + 1> &w, double y) {
+  const int l = g.last(r);
+  double v = y;
+#pragma unroll
+  for (int q = 0; q < 2 * BW; ++q) {
+    const int c = r + 1 + q;
+    if (c <= l) v = v - w.w[q] * U(r, g.slot(r, c));
+  }
+  return v;
+}
+
+// band product of an emitted row m whose columns are r .. r+2bw, window w.w[q] = x(r+q)
+template <int BW>
+PB_HD cplx mv_row(const BandGeom &g, const SharedBand &A, int m, int r, const CWin<2 * BW + 1> &w) {
+  cplx v = cscale(w.w[0], A(m, g.slot(m, r)));
+#pragma unroll
+  for (int q = 1; q <= 2 * BW; ++q) v = cadd(v, cscale(w.w[q], A(m, g.slot(m, r + q))));
+  return v;
+}
+template <int BW>
+PB_HD double mv_row_r(const BandGeom &g, const SharedBand &A, int m, int r, const RWin<2 * BW + 1> &w) {
+  double v = w.w[0] * A(m, g.slot(m, r));
+#pragma unroll
+  for (int q = 1; q <= 2 * BW; ++q) v = v + w.w[q] * A(m, g.slot(m, r + q));
+  return v;
+}
+
+// Rows whose band product becomes complete after x(r) is produced in an
+// upward sweep: calls emit(m) for each (middle row r+bw; the last bw rows at
+// r = n-1-2bw; the first bw rows at r = 0).
+template <int BW, class F>
+PB_HD void for_completed_rows(int n, int r, F &&emit) {
+  if (r <= n - 1 - 2 * BW) emit(r + BW);
+  if (r == n - 1 - 2 * BW) {
+#pragma unroll
+    for (int t = 0; t < BW; ++t) emit(n - BW + t);
+  }
+  if (r == 0) {
+#pragma unroll
+    for (int t = 0; t < BW; ++t) emit(t);
+  }
+}
+
+// A system's own band factor: U entries of row p at dbuf[(p*W + slot)*plane + sys]
+struct OwnU {
+  double *p;
+  size_t s;
+  int W;
+  PB_HD double operator()(int r, int slot) const { return p[(size_t)(r * W + slot) * s]; }
+  PB_HD void set(int r, int slot, double v) const { p[(size_t)(r * W + slot) * s] = v; }
+};
+
+// Matrix entries formed on the fly.
+struct HelmholtzA {  // (1 + beta k2) D0 - beta D2 with diagonal-only wall rows
+  SharedBand D0, D2;
+  double dummy, beta;
+  int n, bw;
+  PB_HD double operator()(int r, int slot) const {
+    if (r == 0 || r == n - 1) return slot == bw ? dummy * D0(r, bw) - beta * D2(r, bw) : 0.0;
+    return dummy * D0(r, slot) - beta * D2(r, slot);
+  }
+};
+struct PoissonA {  // D2 - k2 D0 with D0-diagonal wall rows
+  SharedBand D0, D2;
+  double k2;
+  int n, bw;
+  PB_HD double operator()(int r, int slot) const {
+    if (r == 0 || r == n - 1) return slot == bw ? D0(r, bw) : 0.0;
+    return -k2 * D0(r, slot) + D2(r, slot);
+  }
+};
+
+// One Crout row (fixed_bandec*): Lrow[q] = L(r, r-q); U(r, c) stored into Uo.
+template <int BW, class M>
+PB_HD void crout_row(const BandGeom &g, const M &A, const OwnU &Uo, int r, double (&Lrow)[2 * BW + 1]) {
+  const int f = g.first(r);
+#pragma unroll
+  for (int qd = 2 * BW; qd >= 0; --qd) {
+    const int c = r - qd;
+    if (c >= f) {
+      double v = A(r, g.slot(r, c));
+#pragma unroll
+      for (int qp = qd + 1; qp <= 2 * BW; ++qp) {
+        const int p = r - qp;
+        if (p >= f && c <= g.last(p)) v = v - Lrow[qp] * Uo(p, g.slot(p, c));
+      }
+      Lrow[qd] = v;
+    } else {
+      Lrow[qd] = 0.0;
+    }
+  }
+  const double inv = 1.0 / Lrow[0];
+#pragma unroll
+  for (int qc = 1; qc <= 2 * BW; ++qc) {
+    const int c = r + qc;
+    if (c <= g.last(r)) {
+      double v = A(r, g.slot(r, c));
+#pragma unroll
+      for (int qp = 1; qp <= 2 * BW; ++qp) {
+        const int p = r - qp;
+        if (p >= f && c <= g.last(p)) v = v - Lrow[qp] * Uo(p, g.slot(p, c));
+      }
+      Uo.set(r, g.slot(r, c), inv * v);
+    }
+  }
+}
+
+// forward substitution with the current row's L (Lrow[q] = L(r, r-q))
+template <int BW>
+PB_HD cplx fwd_lrow(const BandGeom &g, int r, const double (&Lrow)[2 * BW + 1], const CWin<2 * BW> &w, cplx b) {
+  const int f = g.first(r);
+  cplx v = b;
+#pragma unroll
+  for (int q = 2 * BW; q >= 1; --q) {
+    if (r - q >= f) v = csub(v, cscale(w.w[q - 1], Lrow[q]));
+  }
+  return cdivr(v, Lrow[0]);
+}
+template <int BW>
+PB_HD double fwd_lrow_r(const BandGeom &g, int r, const double (&Lrow)[2 * BW + 1], const RWin<2 * BW> &w, double b) {
+  const int f = g.first(r);
+  double v = b;
+#pragma unroll
+  for (int q = 2 * BW; q >= 1; --q) {
+    if (r - q >= f) v = v - w.w[q - 1] * Lrow[q];
+  }
+  return v / Lrow[0];
+}
+
+// ------------------------------------------------------------------ sweeps
+// s == 1 only: scratch d2S_i = D2 LU(D0)^-1 RK_i  (solve_g.f90:97-98, solve_v_f.f90:72-73, solve_t.f90:97-98)
+template <int BW>
+PB_HD void sweep_A1(const LineParams &P, const WsArgs &A, const SysCtx &c) {
+  const BandGeom g{c.n, BW};
+  SV r1 = c.rk(A, 0), r2 = c.rk(A, 1), r3 = c.rk(A, 2);
+  SV o1 = c.buf(A, 0), o2 = c.buf(A, 1), o3 = c.col(A, 5);
+  CWin<2 * BW> w1, w2, w3;
+  w1.clear(); w2.clear(); w3.clear();
+  for (int r = 0; r < c.n; ++r) {
+    const cplx y1 = fwd_row<BW>(g, P.D0LU, r, w1, r1[r]);
+    const cplx y2 = fwd_row<BW>(g, P.D0LU, r, w2, r2[r]);
+    const cplx y3 = fwd_row<BW>(g, P.D0LU, r, w3, r3[r]);
+    o1[r] = y1; o2[r] = y2; o3[r] = y3;
+    w1.push(y1); w2.push(y2); w3.push(y3);
+  }
+}
+
+template <int BW>
+PB_HD void sweep_B1(const LineParams &P, const WsArgs &A, const SysCtx &c) {
+  const BandGeom g{c.n, BW};
+  SV o1 = c.buf(A, 0), o2 = c.buf(A, 1), o3 = c.col(A, 5);
+  CWin<2 * BW + 1> w1, w2, w3;
+  w1.clear(); w2.clear(); w3.clear();
+  for (int r = c.n - 1; r >= 0; --r) {
+    const cplx x1 = bwd_row<BW>(g, P.D0LU, r, w1, o1[r]);
+    const cplx x2 = bwd_row<BW>(g, P.D0LU, r, w2, o2[r]);
+    const cplx x3 = bwd_row<BW>(g, P.D0LU, r, w3, o3[r]);
+    w1.push(x1); w2.push(x2); w3.push(x3);
+    for_completed_rows<BW>(c.n, r, [&](int m) {
+      o1[m] = mv_row<BW>(g, P.D2, m, r, w1);
+      o2[m] = mv_row<BW>(g, P.D2, m, r, w2);
+      o3[m] = mv_row<BW>(g, P.D2, m, r, w3);
+    });
+  }
+}
+
+// A: hvhg point-wise terms (hvhg.f90:169-204, 256-258) and forward
+// substitution of u3a, u3b, u4b, P8 with LU(D0)
+template <int BW>
+PB_HD void sweep_A(const LineParams &P, const WsArgs &A, const SysCtx &c) {
+  const BandGeom g{c.n, BW};
+  const double kx = c.kx, kz = c.kz;
+  const double kz2 = kz * kz, kx2 = kx * kx, kxkz = kx * kz;
+  const cplx ikx = iunit(kx), ikz = iunit(kz);
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s4 = c.col(A, 4), s6 = c.col(A, 6),
+     s7 = c.col(A, 7), s8 = c.col(A, 8);
+  CWin<2 * BW> wB, wE, wF, w8;
+  wB.clear(); wE.clear(); wF.clear(); w8.clear();
+  for (int r = 0; r < c.n; ++r) {
+    const cplx p1 = s0[r], p2 = s1[r], p3 = s2[r], p4 = s3[r], p5 = s4[r], p7 = s6[r], p8 = s7[r], p9 = s8[r];
+    const cplx u1a = cadd(cscale(csub(p1, p3), kxkz), cscale(p5, kz2 - kx2));
+    const cplx u3a = csub(csub(cscale(p1, -kx2), cscale(p5, 2.0 * kxkz)), cscale(p3, kz2));
+    const cplx u3b = csub(cmul(ikx, p4), cmul(ikz, p2));
+    const cplx u4b = cadd(cmul(ikz, p4), cmul(ikx, p2));
+    const cplx u7a = cadd(cmul(ikx, p7), cmul(ikz, p9));
+    const cplx yB = fwd_row<BW>(g, P.D0LU, r, wB, u3a);
+    const cplx yE = fwd_row<BW>(g, P.D0LU, r, wE, u3b);
+    const cplx yF = fwd_row<BW>(g, P.D0LU, r, wF, u4b);
+    const cplx y8 = fwd_row<BW>(g, P.D0LU, r, w8, p8);
+    s0[r] = yB; s2[r] = yE; s4[r] = yF; s7[r] = y8; s6[r] = u1a; s8[r] = u7a;
+    wB.push(yB); wE.push(yE); wF.push(yF); w8.push(y8);
+  }
+}
+
+// B: back substitution, hvhg's band products, the Helmholtz right-hand sides
+// (solve_g.f90:84-123, solve_v_f.f90:56-95, solve_t.f90:84-123) and the
+// zeta*h stores (solve_g.f90:138-143, solve_v_f.f90:115-120)
+template <int BW>
+PB_HD void sweep_B(const LineParams &P, const WsArgs &A, const SysCtx &c) {
+  const BandGeom g{c.n, BW};
+  const int n = c.n, s = P.s, S = P.S;
+  const double kx = c.kx, kz = c.kz, k2 = c.k2, dt = P.dt, nu = P.nu, pr = P.pr;
+  const cplx ikx3 = iunit(kx * kx * kx), ikxkz2 = iunit(kx * (kz * kz));
+  const cplx ikx2kz = iunit((kx * kx) * kz), ikz3 = iunit(kz * kz * kz);
+  const cplx a = cadd(ikx3, ikxkz2), b = cadd(ikx2kz, ikz3);
+  const double gamma = dt * P.coef(2, s);
+  const double al_g = dt * P.coef(0, s) * nu, al_t = dt * P.coef(0, s) * nu / pr;
+  const double du_g = 1.0 - al_g * k2, du_t = 1.0 - al_t * k2;
+  const double zeta = (s != S) ? dt * P.coef(3, s + 1) : 0.0;
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s4 = c.col(A, 4), s6 = c.col(A, 6),
+     s7 = c.col(A, 7), s8 = c.col(A, 8);
+  SV r1 = c.rk(A, 0), r2 = c.rk(A, 1), r3 = c.rk(A, 2);
+  SV d1 = c.buf(A, 0), d2 = c.buf(A, 1), d3 = c.col(A, 5);
+  CWin<2 * BW + 1> cB, cE, cF, c8;
+  cB.clear(); cE.clear(); cF.clear(); c8.clear();
+  for (int r = n - 1; r >= 0; --r) {
+    const cplx xB = bwd_row<BW>(g, P.D0LU, r, cB, s0[r]);
+    const cplx xE = bwd_row<BW>(g, P.D0LU, r, cE, s2[r]);
+    const cplx xF = bwd_row<BW>(g, P.D0LU, r, cF, s4[r]);
+    const cplx x8 = bwd_row<BW>(g, P.D0LU, r, c8, s7[r]);
+    cB.push(xB); cE.push(xE); cF.push(xF); c8.push(x8);
+    for_completed_rows<BW>(n, r, [&](int m) {
+      const cplx u5a = mv_row<BW>(g, P.D1, m, r, cB);
+      const cplx u2c = mv_row<BW>(g, P.D1, m, r, cE);
+      const cplx u3c = mv_row<BW>(g, P.D2, m, r, cF);
+      const cplx u9a = mv_row<BW>(g, P.D1, m, r, c8);
+      const cplx u5b = cadd(cadd(u5a, cmul(a, s1[m])), cmul(b, s3[m]));
+      const cplx hv = cadd(u5b, u3c);
+      const cplx hg = cadd(s6[m], u2c);
+      const cplx ht = cneg(cadd(s8[m], u9a));
+      cplx rg, rp, rt;
+      if (s == 1) {
+        rg = cadd(cadd(cscale(r1[m], du_g), cscale(hg, gamma)), cscale(d1[m], al_g));
+        rp = cadd(cadd(cscale(r2[m], du_g), cscale(hv, gamma)), cscale(d2[m], al_g));
+        rt = cadd(cadd(cscale(r3[m], du_t), cscale(ht, gamma)), cscale(d3[m], al_t));
+      } else {
+        rg = cadd(r1[m], cscale(hg, gamma));
+        rp = cadd(r2[m], cscale(hv, gamma));
+        rt = cadd(r3[m], cscale(ht, gamma));
+      }
+      if (m == 0 || m == n - 1) {
+        rg = mk(0.0, 0.0); rp = mk(0.0, 0.0); rt = mk(0.0, 0.0);
+      }
+      if (s != S) {
+        r1[m] = cscale(hg, zeta);
+        r2[m] = cscale(hv, zeta);
+        r3[m] = cscale(ht, zeta);
+      }
+      s0[m] = rg; s2[m] = rp; s4[m] = rt;
+    });
+  }
+}
+
+// C: LU of H_nu and H_theta fused with their forward substitutions
+// (solve_g.f90:42-71, 158; solve_v_f.f90:142-148; solve_t.f90)
+template <int BW>
+PB_HD void sweep_C(const LineParams &P, const WsArgs &A, const SysCtx &c, double *dbuf2) {
+  const BandGeom g{c.n, BW};
+  const int n = c.n, s = P.s;
+  const double dt = P.dt, nu = P.nu, pr = P.pr, k2 = c.k2;
+  const double be_g = dt * P.coef(1, s) * nu, be_t = dt * P.coef(1, s) * nu / pr;
+  const HelmholtzA Hg{P.D0, P.D2, 1.0 + be_g * k2, be_g, n, BW};
+  const HelmholtzA Ht{P.D0, P.D2, 1.0 + be_t * k2, be_t, n, BW};
+  const OwnU Ug{A.dbuf + c.sys, A.plane, 2 * BW + 1}, Ut{dbuf2 + c.sys, A.plane, 2 * BW + 1};
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s4 = c.col(A, 4);
+  CWin<2 * BW> wg, wp, wt;
+  RWin<2 * BW> wh;
+  wg.clear(); wp.clear(); wt.clear(); wh.clear();
+  double Lg[2 * BW + 1], Lt[2 * BW + 1];
+  double d00 = 0.0;
+  for (int r = 0; r < n; ++r) {
+    crout_row<BW>(g, Hg, Ug, r, Lg);
+    crout_row<BW>(g, Ht, Ut, r, Lt);
+    if (r == 0) d00 = Lg[0];
+    const cplx yg = fwd_lrow<BW>(g, r, Lg, wg, s0[r]);
+    const cplx yp = fwd_lrow<BW>(g, r, Lg, wp, s2[r]);
+    const double yh = fwd_lrow_r<BW>(g, r, Lg, wh, r == 0 ? d00 : 0.0);
+    const cplx yt = fwd_lrow<BW>(g, r, Lt, wt, s4[r]);
+    s0[r] = yg; s2[r] = yp; s1[r] = mk(yh, yh); s4[r] = yt;
+    wg.push(yg); wp.push(yp); wh.push(yh); wt.push(yt);
+  }
+}
+
+// D: back substitution with U of H_nu / H_theta and the D0/D2 products;
+// stores of g and theta (solve_g.f90:158-185), the phi_p part of store_phi
+// (solve_v_f.f90:150-181)
+template <int BW>
+PB_HD void sweep_D(const LineParams &P, const WsArgs &A, const SysCtx &c, double *dbuf2) {
+  const BandGeom g{c.n, BW};
+  const int n = c.n, s = P.s, S = P.S;
+  const double dt = P.dt, nu = P.nu, pr = P.pr, k2 = c.k2;
+  const OwnU Ug{A.dbuf + c.sys, A.plane, 2 * BW + 1}, Ut{dbuf2 + c.sys, A.plane, 2 * BW + 1};
+  double al_g = 0, du_g = 0, al_t = 0, du_t = 0, al_p = 0, du_p = 0;
+  if (s != S) {
+    al_g = dt * P.coef(0, s + 1) * nu;
+    du_g = 1.0 - al_g * k2;
+    al_t = dt * P.coef(0, s + 1) * nu / pr;
+    du_t = 1.0 - al_t * k2;
+    al_p = dt * nu * P.coef(0, s + 1);
+    du_p = 1.0 - al_p * k2;
+  }
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s4 = c.col(A, 4), s6 = c.col(A, 6);
+  SV r1 = c.rk(A, 0), r2 = c.rk(A, 1), r3 = c.rk(A, 2);
+  CWin<2 * BW + 1> wg, wp, wt;
+  RWin<2 * BW + 1> wh;
+  wg.clear(); wp.clear(); wt.clear(); wh.clear();
+  for (int r = n - 1; r >= 0; --r) {
+    const cplx xg = bwd_row<BW>(g, Ug, r, wg, s0[r]);
+    const cplx xp = bwd_row<BW>(g, Ug, r, wp, s2[r]);
+    const double xh = bwd_row_r<BW>(g, Ug, r, wh, s1[r].re);
+    const cplx xt = bwd_row<BW>(g, Ut, r, wt, s4[r]);
+    wg.push(xg); wp.push(xp); wh.push(xh); wt.push(xt);
+    for_completed_rows<BW>(n, r, [&](int m) {
+      const cplx gv = mv_row<BW>(g, P.D0, m, r, wg);
+      const cplx d2g = mv_row<BW>(g, P.D2, m, r, wg);
+      const cplx tv = mv_row<BW>(g, P.D0, m, r, wt);
+      const cplx d2t = mv_row<BW>(g, P.D2, m, r, wt);
+      const cplx pv = mv_row<BW>(g, P.D0, m, r, wp);
+      const cplx d2p = mv_row<BW>(g, P.D2, m, r, wp);
+      const double hv = mv_row_r<BW>(g, P.D0, m, r, wh);
+      const double d2h = mv_row_r<BW>(g, P.D2, m, r, wh);
+      if (s != S) {
+        r1[m] = cadd(cadd(r1[m], cscale(gv, du_g)), cscale(d2g, al_g));
+        r3[m] = cadd(cadd(r3[m], cscale(tv, du_t)), cscale(d2t, al_t));
+        r2[m] = cadd(cadd(r2[m], cscale(pv, du_p)), cscale(d2p, al_p));
+      } else {
+        r1[m] = gv;
+        r3[m] = tv;
+        r2[m] = pv;
+      }
+      s0[m] = gv; s6[m] = tv; s2[m] = pv; s1[m] = mk(hv, hv); s3[m] = mk(d2h, d2h);
+    });
+  }
+}
+
+// E: LU of the Poisson operator fused with the forward substitution of
+// v_p (wall values vlo/vhi) and v_h (solve_v_f.f90:205-262)
+template <int BW>
+PB_HD void sweep_E(const LineParams &P, const WsArgs &A, const SysCtx &c, cplx vlo, cplx vhi) {
+  const BandGeom g{c.n, BW};
+  const int n = c.n;
+  const PoissonA Q{P.D0, P.D2, c.k2, n, BW};
+  const OwnU Uq{A.dbuf + c.sys, A.plane, 2 * BW + 1};
+  SV s1 = c.col(A, 1), s2 = c.col(A, 2), s5 = c.col(A, 5);
+  CWin<2 * BW> wp;
+  RWin<2 * BW> wh;
+  wp.clear(); wh.clear();
+  double L[2 * BW + 1];
+  for (int r = 0; r < n; ++r) {
+    crout_row<BW>(g, Q, Uq, r, L);
+    const cplx bp = (r == 0) ? vlo : (r == n - 1 ? vhi : s2[r]);
+    const double bh = (r == 0 || r == n - 1) ? 0.0 : s1[r].re;
+    const cplx yp = fwd_lrow<BW>(g, r, L, wp, bp);
+    const double yh = fwd_lrow_r<BW>(g, r, L, wh, bh);
+    s2[r] = yp; s5[r] = mk(yh, yh);
+    wp.push(yp); wh.push(yh);
+  }
+}
+
+// F: back substitution with U of the Poisson operator and the D0/D1
+// products v_p, v_p', v_h', D0 c_h (solve_v_f.f90:258-271, 352)
+template <int BW>
+PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
+  const BandGeom g{c.n, BW};
+  const int n = c.n;
+  const OwnU Uq{A.dbuf + c.sys, A.plane, 2 * BW + 1};
+  SV s2 = c.col(A, 2), s5 = c.col(A, 5), s7 = c.col(A, 7), s8 = c.col(A, 8);
+  CWin<2 * BW + 1> wp;
+  RWin<2 * BW + 1> wh;
+  wp.clear(); wh.clear();
+  for (int r = n - 1; r >= 0; --r) {
+    const cplx xp = bwd_row<BW>(g, Uq, r, wp, s2[r]);
+    const double xh = bwd_row_r<BW>(g, Uq, r, wh, s5[r].re);
+    wp.push(xp); wh.push(xh);
+    for_completed_rows<BW>(n, r, [&](int m) {
+      const cplx vp = mv_row<BW>(g, P.D0, m, r, wp);
+      const cplx dvp = mv_row<BW>(g, P.D1, m, r, wp);
+      const double dvh = mv_row_r<BW>(g, P.D1, m, r, wh);
+      const double d0h = mv_row_r<BW>(g, P.D0, m, r, wh);
+      s2[m] = vp; s7[m] = dvp; s5[m] = mk(dvh, dvh); s8[m] = mk(d0h, d0h);
+    });
+  }
+}
+
+// G: influence coefficients and the final combination on mirror pairs
+// (solve_v_f.f90:293-355, solve_u_w.f90:27-38)
+PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
+  const int n = c.n, s = P.s, S = P.S;
+  const double dt = P.dt, nu = P.nu, k2 = c.k2;
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s5 = c.col(A, 5), s7 = c.col(A, 7),
+     s8 = c.col(A, 8);
+  SV r2 = c.rk(A, 1);
+  const cplx h0 = s5[0], hn = s5[n - 1], d0 = s7[0], dn = s7[n - 1];
+  const cplx den = csub(cmul(h0, h0), cmul(hn, hn));
+  const cplx c1 = cdiv(csub(cmul(hn, dn), cmul(h0, d0)), den);
+  const cplx c2 = cdiv(csub(cmul(h0, dn), cmul(hn, d0)), den);
+  cplx c1d = c1, c2d = c2, c1a = c1, c2a = c2;
+  if (s != S) {
+    const double alpha = dt * nu * P.coef(0, s + 1);
+    const double dummy = 1.0 - alpha * k2;
+    c1d = cscale(c1, dummy); c2d = cscale(c2, dummy); c1a = cscale(c1, alpha); c2a = cscale(c2, alpha);
+  }
+  const cplx ci = mk(0.0, 1.0);
+  const cplx kxx = cdivr(cmul(ci, mk(c.kx, 0.0)), k2), kzz = cdivr(cmul(ci, mk(c.kz, 0.0)), k2);
+  for (int i = 0; i <= (n - 1) / 2; ++i) {
+    const int j = n - 1 - i;
+    const cplx dvp_i = s7[i], dvp_j = s7[j], dvh_i = s5[i], dvh_j = s5[j];
+    const cplx sp_i = r2[i], sp_j = r2[j], ph_i = s1[i], ph_j = s1[j], d2h_i = s3[i], d2h_j = s3[j];
+    const cplx vp_i = s2[i], vp_j = s2[j], d0h_i = s8[i], d0h_j = s8[j], g_i = s0[i], g_j = s0[j];
+    auto one = [&](int row, cplx dvp, cplx dvh, cplx dvh_m, cplx sp, cplx ph, cplx ph_m, cplx d2h, cplx d2h_m,
+                   cplx vp, cplx d0h, cplx d0h_m, cplx gg) {
+      const cplx f = csub(cadd(dvp, cmul(c1, dvh)), cmul(c2, dvh_m));
+      cplx st;
+      if (s != S) {
+        st = cadd(sp, cmul(c1d, ph));
+        st = cadd(st, cmul(c2d, ph_m));
+        st = cadd(st, cmul(c1a, d2h));
+        st = cadd(st, cmul(c2a, d2h_m));
+      } else {
+        st = cadd(cadd(sp, cmul(c1, ph)), cmul(c2, ph_m));
+      }
+      const cplx v = cadd(cadd(vp, cmul(c1, d0h)), cmul(c2, d0h_m));
+      const cplx dxg = cmul(kxx, gg), dzg = cmul(kzz, gg), dxf = cmul(kxx, f), dzf = cmul(kzz, f);
+      s0[row] = csub(dxf, dzg);
+      s2[row] = cadd(dxg, dzf);
+      s1[row] = v;
+      r2[row] = st;
+    };
+    one(i, dvp_i, dvh_i, dvh_j, sp_i, ph_i, ph_j, d2h_i, d2h_j, vp_i, d0h_i, d0h_j, g_i);
+    if (j != i) one(j, dvp_j, dvh_j, dvh_i, sp_j, ph_j, ph_i, d2h_j, d2h_i, vp_j, d0h_j, d0h_i, g_j);
+  }
+}
+
+// Whole substep of one (kx,kz) != (0,0) system.
+template <int BW>
+PB_HD void fused_line(const LineParams &P, const WsArgs &A, double *dbuf2, size_t sys) {
+  SysCtx c;
+  if (!make_sys(A, P.g.n, sys, c)) return;
+  if (P.s == 1) {
+    sweep_A1<BW>(P, A, c);
+    sweep_B1<BW>(P, A, c);
+  }
+  sweep_A<BW>(P, A, c);
+  sweep_B<BW>(P, A, c);
+  sweep_C<BW>(P, A, c, dbuf2);
+  sweep_D<BW>(P, A, c, dbuf2);
+  cplx vlo = mk(0.0, 0.0), vhi = mk(0.0, 0.0);
+  if (A.opp_alpha != 0.0 && A.vd) {
+    vlo = cscale(A.vd[sys], -A.opp_alpha);
+    vhi = cscale(A.vd[A.plane + sys], -A.opp_alpha);
+  }
+  sweep_E<BW>(P, A, c, vlo, vhi);
+  sweep_F<BW>(P, A, c);
+  sweep_G(P, A, c);
+}
+
+}  // namespace pb

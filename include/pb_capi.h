@@ -100,6 +100,12 @@ int pb_get_labs(const pb_ctx *ctx, double *y, double *d0, double *d1, double *d2
 /* U: (ny, nx/2, nz, 4) slots u, v, w, theta; rk: (ny, nx/2, nz, 3). */
 int pb_set_state(pb_ctx *ctx, const double *U, const double *rk);
 int pb_get_state(pb_ctx *ctx, double *U, double *rk);
+/* The same with one pointer per field, as the Fortran driver holds them:
+ * u = U(:,1), v = U(:,2), w = U(:,3), theta = U(:,7), each (ny, nx/2, nz)
+ * complex, and rk = RK_STORE(ny, nx/2, nz, 3) (or NULL). */
+int pb_set_state_slots(pb_ctx *ctx, const double *u, const double *v, const double *w, const double *theta,
+                       const double *rk);
+int pb_get_state_slots(pb_ctx *ctx, double *u, double *v, double *w, double *theta, double *rk);
 /* Fill the RK stores from u, v, w, theta already set by pb_set_state with
  * rk == NULL (set_initial_data_uvw, initial_step.f90:139-179). */
 int pb_init_stores(pb_ctx *ctx);

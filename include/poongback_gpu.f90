@@ -1,0 +1,102 @@
+!--------------------------------------------------------------------
+! poongback_gpu -- iso_c_binding interface to the gfx950 time-step
+! library declared in include/pb_capi.h.
+!
+! A Fortran driver built on SOURCE_PASSOPB (main.f90, time_march.f90)
+! uses this module to replace the body of time_march (time_march.f90:
+! 63-120: y_to_x x4, dft_x_realspace, x_to_y x8, rescale_data, wavespace)
+! by one pb_substep call on device-resident state; see INTEGRATION.md.
+! Sizes and layouts are those of the reference: U(ny,nx/2,nz,slot) and
+! RK_STORE(ny,nx/2,nz,3) complex(c_double_complex), y fastest.
+!--------------------------------------------------------------------
+module poongback_gpu
+  use, intrinsic :: iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: PB_OK = 0, PB_E_ARG = -1, PB_E_HIP = -2, &
+       PB_E_NUMERIC = -3, PB_E_STATE = -4
+  integer(c_int32_t), parameter :: PB_KNOTS_REFERENCE = 0, PB_KNOTS_TANH = 1
+
+  type, bind(C) :: pb_config
+     integer(c_int64_t) :: nx, ny, nz
+     integer(c_int32_t) :: labs_k, knot_map
+     real(c_double) :: stretch
+     real(c_double) :: lmfx, lmfy, lmfz
+     real(c_double) :: nu, pr
+     real(c_double) :: bulk_u, bulk_w, bulk_t
+     real(c_double) :: u_gal
+     real(c_double) :: cfl, dt_factor, dy_factor
+     real(c_double) :: opp_alpha, opp_ydist
+     integer(c_int32_t) :: rk_step, reserved
+  end type pb_config
+
+  type, bind(C) :: pb_substep_out
+     real(c_double) :: dt, velmax, phi_u, phi_t, tau_w, q_w
+  end type pb_substep_out
+
+  interface
+     integer(c_int) function pb_create(cfg, ctx) bind(C, name="pb_create")
+       import :: c_int, c_ptr, pb_config
+       type(pb_config), intent(in) :: cfg
+       type(c_ptr), intent(out) :: ctx
+     end function pb_create
+
+     integer(c_int) function pb_destroy(ctx) bind(C, name="pb_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function pb_destroy
+
+     type(c_ptr) function pb_last_error() bind(C, name="pb_last_error")
+       import :: c_ptr
+     end function pb_last_error
+
+     integer(c_int) function pb_set_state_slots(ctx, u, v, w, theta, rk) bind(C, name="pb_set_state_slots")
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: u(*), v(*), w(*), theta(*), rk(*)
+     end function pb_set_state_slots
+
+     integer(c_int) function pb_get_state_slots(ctx, u, v, w, theta, rk) bind(C, name="pb_get_state_slots")
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(out) :: u(*), v(*), w(*), theta(*), rk(*)
+     end function pb_get_state_slots
+
+     integer(c_int) function pb_init_stores(ctx) bind(C, name="pb_init_stores")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function pb_init_stores
+
+     integer(c_int) function pb_substep(ctx, s, dt, out) bind(C, name="pb_substep")
+       import :: c_int, c_int32_t, c_double, c_ptr, pb_substep_out
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: s
+       real(c_double), intent(inout) :: dt
+       type(pb_substep_out), intent(out) :: out
+     end function pb_substep
+
+     integer(c_int) function pb_sync(ctx) bind(C, name="pb_sync")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function pb_sync
+
+     integer(c_int) function pb_wavespace_host(ctx, s, dt, U, rk, v_old, out) bind(C, name="pb_wavespace_host")
+       import :: c_int, c_int32_t, c_double, c_ptr, c_double_complex, pb_substep_out
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: s
+       real(c_double), value :: dt
+       complex(c_double_complex), intent(inout) :: U(*), rk(*)
+       complex(c_double_complex), intent(in) :: v_old(*)
+       type(pb_substep_out), intent(out) :: out
+     end function pb_wavespace_host
+
+     integer(c_int) function pb_nonlinear_host(ctx, U, P, want_vv, velmax) bind(C, name="pb_nonlinear_host")
+       import :: c_int, c_int32_t, c_double, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: U(*)
+       complex(c_double_complex), intent(inout) :: P(*)
+       integer(c_int32_t), value :: want_vv
+       real(c_double), intent(out) :: velmax
+     end function pb_nonlinear_host
+  end interface
+end module poongback_gpu

@@ -80,3 +80,29 @@ int emu_wavespace(const pb_config *cfg, int s, double dt, double *U, double *RK,
   return 0;
 }
 }
+
+#include "../../channel_passive_amd/csrc/solver_fused.h"
+extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, double *U, double *RK, const double *vd,
+                                   double *mean4) {
+  Setup S;
+  if (make_setup(cfg, S)) return -1;
+  pb::LineParams P = params(cfg, S, s, dt);
+  const int nxh = (int)cfg->nx / 2, nz = (int)cfg->nz;
+  const size_t plane = (size_t)nz * nxh, field = plane * S.n;
+  std::vector<pb::cplx> buf(2 * field);
+  std::vector<double> dbuf((size_t)S.W * field), dbuf2((size_t)S.W * field);
+  pb::WsArgs A;
+  A.U = (pb::cplx *)U; A.RK = (pb::cplx *)RK; A.buf = buf.data(); A.dbuf = dbuf.data();
+  A.vd = (const pb::cplx *)vd; A.plane = plane; A.field = field; A.nxh = nxh; A.nz = nz; A.nh = nz / 2;
+  A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha; A.mean_out = mean4;
+  pb::wavespace_line(P, A, 0);  // (0,0): mean modes
+  for (size_t sys = 1; sys < plane; ++sys) {
+    switch (S.bw) {
+      case 3: pb::fused_line<3>(P, A, dbuf2.data(), sys); break;
+      case 4: pb::fused_line<4>(P, A, dbuf2.data(), sys); break;
+      case 5: pb::fused_line<5>(P, A, dbuf2.data(), sys); break;
+      default: return -2;
+    }
+  }
+  return 0;
+}

@@ -36,7 +36,7 @@ def test_library_exports_every_symbol():
 def test_binding_covers_header():
     from channel_passive_amd import capi
     names = set(declared())
-    assert names <= set(capi.SIGNATURES) | {"pb_timing", "pb_timing_read"}
+    assert names <= set(capi.SIGNATURES)
 
 
 def test_config_layout_matches_header():

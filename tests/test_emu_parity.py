@@ -39,3 +39,36 @@ def test_wavespace_bitwise_cpu(oracle_lib, emu_lib, name, alpha):
         assert np.array_equal(m_e, m_o)
         for i, f in enumerate((0, 1, 2, 6)):
             S[i] = P[f]
+
+
+@pytest.mark.parametrize("name,alpha,extra", [("tiny", 0.0, {}), ("tiny", 0.8, {}), ("re180_64", 0.0, {}),
+                                              ("tiny", 0.5, {"knot_map": 0, "stretch": 0.0, "labs_k": 6, "ny": 34}),
+                                              ("tiny", 0.0, {"ny": 65, "labs_k": 9})])
+def test_fused_wavespace_bitwise_cpu(oracle_lib, emu_lib, name, alpha, extra):
+    """The seven-sweep organisation of the device solver (solver_fused.h)
+    reproduces the oracle bit for bit (bw = 4, 3 and 5 grids)."""
+    cfg = dataclasses.replace(preset(name), opp_alpha=alpha, opp_yplus=20.0, **extra)
+    orc = oracle_lib.Oracle(cfg)
+    assert orc.bw in (3, 4, 5)
+    c = cfg.to_c()
+    S = make_state(cfg, seed=17)
+    RK = orc.init_stores(S)
+    orc.step(S, RK, cfg.dt)
+    for s in (1, 2, 3):
+        P, _ = orc.nonlinear(S)
+        vd = orc.opp_planes(S) if alpha else None
+        Ud = np.ascontiguousarray(np.transpose(P[:9], (0, 3, 1, 2)))
+        RKd = np.ascontiguousarray(np.transpose(RK, (0, 3, 1, 2)))
+        m_e = np.zeros(4)
+        assert emu_lib.emu_wavespace_fused(ctypes.byref(c), s, ctypes.c_double(cfg.dt), Ud.ctypes.data_as(dp),
+                                           RKd.ctypes.data_as(dp), None if vd is None else vd.ctypes.data_as(dp),
+                                           m_e.ctypes.data_as(dp)) == 0
+        m_o = orc.wavespace(s, cfg.dt, P, RK, vd)
+        Ue, RKe = np.transpose(Ud, (0, 2, 3, 1)), np.transpose(RKd, (0, 2, 3, 1))
+        for f in (0, 1, 2, 6):
+            assert np.array_equal(Ue[f], P[f]), (s, f, np.abs(Ue[f] - P[f]).max())
+        for i in range(3):
+            assert np.array_equal(RKe[i], RK[i]), (s, i, np.abs(RKe[i] - RK[i]).max())
+        assert np.array_equal(m_e, m_o)
+        for i, f in enumerate((0, 1, 2, 6)):
+            S[i] = P[f]
