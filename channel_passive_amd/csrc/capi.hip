@@ -74,6 +74,9 @@ struct pb_ctx {
   pb::FftPlan pz, px;
   int Bz;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;           // the (0,0) mean line runs here, beside the sweeps
+  hipStream_t side2 = nullptr;          // the theta chain of the sweeps
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
   // device memory
   double2 *U = nullptr, *RK = nullptr, *pad = nullptr, *buf = nullptr, *vd = nullptr;
   double *dbuf = nullptr, *dbuf2 = nullptr, *vpart = nullptr, *scal = nullptr;
@@ -130,6 +133,12 @@ static int ctx_free(pb_ctx *c) {
     if (p) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_fork2) (void)hipEventDestroy(c->ev_fork2);
+  if (c->ev_join2) (void)hipEventDestroy(c->ev_join2);
+  if (c->side2) (void)hipStreamDestroy(c->side2);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -232,7 +241,13 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
     return 0;
   };
   const size_t bandb = (size_t)c->ny * c->W * sizeof(double);
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
     ctx_free(c);
     return fail(PB_E_HIP, "no HIP device / stream");
   }
@@ -428,52 +443,102 @@ static int launch_oppvd(pb_ctx *c, const double2 *vfield_base_minus_one) {
   return PB_OK;
 }
 
-static int launch_nonlinear(pb_ctx *c, int update_dt) {
+// Smallest elements-per-thread instantiation that holds n elements.
+static int pick_ept(size_t n) {
+  const int ept[] = {4, 8, 12, 16, 24, 32};
+  for (int e : ept)
+    if ((size_t)e * pb::FFT_THREADS >= n) return e;
+  return -1;
+}
+
+extern "C++" {
+template <int EPT>
+static int launch_transforms_t(pb_ctx *c, int update_dt) {
   const size_t ldsz = (size_t)c->nz32 * c->Bz * sizeof(double2);
   {
-  StageTimer tm(c, PB_STAGE_ZINV);
-  hipLaunchKernelGGL(pb::zinv_kernel, dim3(c->nxh / c->Bz, c->ny, 4), dim3(pb::FFT_THREADS), ldsz, c->stream, c->U,
-                     c->pad, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield);
-  HIPCHK(hipGetLastError());
+    StageTimer tm(c, PB_STAGE_ZINV);
+    hipLaunchKernelGGL(pb::zinv_kernel<EPT>, dim3(c->nxh / c->Bz, c->ny, 4), dim3(pb::FFT_THREADS), ldsz, c->stream,
+                       c->U, c->pad, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield);
+    HIPCHK(hipGetLastError());
   }
+  return PB_OK;
+}
+template <int EPT>
+static int launch_xphys_t(pb_ctx *c) {
   const size_t ldsx = (size_t)c->nx32 * 2 * sizeof(double2);
-  {
   StageTimer tm(c, PB_STAGE_XPHYS);
-  hipLaunchKernelGGL(pb::xphys_kernel, dim3(c->nz32, c->ny), dim3(pb::FFT_THREADS), ldsx, c->stream, c->pad, c->px,
-                     c->twx, c->nxh, c->nz32, c->pfield, c->ddyr, c->dxr, c->dzr, c->vpart);
-  HIPCHK(hipGetLastError());
-  }
-  {
-  StageTimer tm(c, PB_STAGE_VELMAX);
-  hipLaunchKernelGGL(pb::velmax_kernel, dim3(1), dim3(256), 0, c->stream, c->vpart, (size_t)c->ny * c->nz32, c->scal,
-                     update_dt, c->cfg.dt_factor, c->cflx);
-  HIPCHK(hipGetLastError());
-  }
-  StageTimer tm(c, PB_STAGE_ZFWD);
-  hipLaunchKernelGGL(pb::zfwd_kernel, dim3(c->nxh / c->Bz, c->ny, 8), dim3(pb::FFT_THREADS), ldsz, c->stream, c->pad,
-                     c->U, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield, c->rescale);
+  hipLaunchKernelGGL(pb::xphys_kernel<EPT>, dim3(c->nz32, c->ny), dim3(pb::FFT_THREADS), ldsx, c->stream, c->pad,
+                     c->px, c->twx, c->nxh, c->nz32, c->pfield, c->ddyr, c->dxr, c->dzr, c->vpart);
   HIPCHK(hipGetLastError());
   return PB_OK;
+}
+template <int EPT>
+static int launch_zfwd_t(pb_ctx *c) {
+  const size_t ldsz = (size_t)c->nz32 * c->Bz * sizeof(double2);
+  StageTimer tm(c, PB_STAGE_ZFWD);
+  hipLaunchKernelGGL(pb::zfwd_kernel<EPT>, dim3(c->nxh / c->Bz, c->ny, 8), dim3(pb::FFT_THREADS), ldsz, c->stream,
+                     c->pad, c->U, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield, c->rescale);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+}  // extern "C++"
+
+#define PB_EPT_DISPATCH(ept, call)                                      \
+  switch (ept) {                                                        \
+    case 4: { constexpr int E = 4; call; } break;                       \
+    case 8: { constexpr int E = 8; call; } break;                       \
+    case 12: { constexpr int E = 12; call; } break;                     \
+    case 16: { constexpr int E = 16; call; } break;                     \
+    case 24: { constexpr int E = 24; call; } break;                     \
+    case 32: { constexpr int E = 32; call; } break;                     \
+    default: return fail(PB_E_ARG, "transform too large for one block"); \
+  }
+
+static int launch_nonlinear(pb_ctx *c, int update_dt) {
+  int r = PB_OK;
+  const int ez = pick_ept((size_t)c->nz32 * c->Bz), ex = pick_ept((size_t)c->nx32 * 2);
+  PB_EPT_DISPATCH(ez, r = launch_transforms_t<E>(c, update_dt));
+  if (r) return r;
+  PB_EPT_DISPATCH(ex, r = launch_xphys_t<E>(c));
+  if (r) return r;
+  {
+    StageTimer tm(c, PB_STAGE_VELMAX);
+    hipLaunchKernelGGL(pb::velmax_kernel, dim3(1), dim3(256), 0, c->stream, c->vpart, (size_t)c->ny * c->nz32,
+                       c->scal, update_dt, c->cfg.dt_factor, c->cflx);
+    HIPCHK(hipGetLastError());
+  }
+  PB_EPT_DISPATCH(ez, r = launch_zfwd_t<E>(c));
+  return r;
 }
 
 extern "C++" {
 template <int BW>
 static int launch_sweeps(pb_ctx *c, const pb::LineParams &P, const pb::WsArgs &A, int s) {
   const dim3 grid(blocks_for(c->nsys, 64)), block(64);
-#define SWEEP(id)                                                                                          \
-  hipLaunchKernelGGL((pb::sweep_kernel<BW, id>), grid, block, 0, c->stream, P, A, c->dbuf2, c->nsys, c->scal); \
+#define SWEEP(id, strm)                                                                                      \
+  hipLaunchKernelGGL((pb::sweep_kernel<BW, id>), grid, block, 0, strm, P, A, c->dbuf2, c->nsys, c->scal); \
   HIPCHK(hipGetLastError());
   if (s == 1) {
-    SWEEP(pb::SW_A1);
-    SWEEP(pb::SW_B1);
+    SWEEP(pb::SW_A1, c->stream);
+    SWEEP(pb::SW_B1, c->stream);
   }
-  SWEEP(pb::SW_A);
-  SWEEP(pb::SW_B);
-  SWEEP(pb::SW_C);
-  SWEEP(pb::SW_D);
-  SWEEP(pb::SW_E);
-  SWEEP(pb::SW_F);
-  SWEEP(pb::SW_G);
+  SWEEP(pb::SW_A, c->stream);
+  SWEEP(pb::SW_BG, c->stream);  // reads u1a in slot 6 before D_t writes theta there
+  // theta chain (slots 5, 6, 7, 8, RK3, dbuf2) beside B_v, C_nu, D_nu (slots
+  // 0-4, RK1, RK2, dbuf); joined before E and F overwrite slots 5, 7, 8
+  HIPCHK(hipEventRecord(c->ev_fork2, c->stream));
+  HIPCHK(hipStreamWaitEvent(c->side2, c->ev_fork2, 0));
+  SWEEP(pb::SW_BT, c->side2);
+  SWEEP(pb::SW_CT, c->side2);
+  SWEEP(pb::SW_DT, c->side2);
+  HIPCHK(hipEventRecord(c->ev_join2, c->side2));
+  SWEEP(pb::SW_BV, c->stream);
+  SWEEP(pb::SW_CNU, c->stream);
+  SWEEP(pb::SW_DNU, c->stream);
+  HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
+  SWEEP(pb::SW_E, c->stream);
+  SWEEP(pb::SW_F, c->stream);
+  SWEEP(pb::SW_G, c->stream);
 #undef SWEEP
   return PB_OK;
 }
@@ -498,13 +563,21 @@ static int launch_wavespace(pb_ctx *c, int s) {
   A.mean_out = c->scal + 2 + 4 * (s - 1);
   StageTimer tm(c, PB_STAGE_WAVESPACE);
   if (c->fused && (c->bw >= 3 && c->bw <= 5)) {
-    hipLaunchKernelGGL(pb::wave00_kernel, dim3(1), dim3(64), 0, c->stream, P, A, c->scal);
+    // The mean line touches only its own column: fork it onto the side
+    // stream, run the sweeps of all other systems, join before returning.
+    HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    hipLaunchKernelGGL(pb::wave00_kernel, dim3(1), dim3(64), 0, c->side, P, A, c->scal);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev_join, c->side));
+    int r = PB_OK;
     switch (c->bw) {
-      case 3: return launch_sweeps<3>(c, P, A, s);
-      case 4: return launch_sweeps<4>(c, P, A, s);
-      case 5: return launch_sweeps<5>(c, P, A, s);
+      case 3: r = launch_sweeps<3>(c, P, A, s); break;
+      case 4: r = launch_sweeps<4>(c, P, A, s); break;
+      case 5: r = launch_sweeps<5>(c, P, A, s); break;
     }
+    HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    return r;
   }
   hipLaunchKernelGGL(pb::wavespace_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P, A, c->nsys,
                      c->scal);

@@ -140,15 +140,18 @@ __device__ __forceinline__ void stockham_pass(double2 *x, int N, int B, int Ns, 
 constexpr int FFT_MAX_ELEMS = 8192;
 constexpr int FFT_THREADS = 256;
 
+// EPT = upper bound of elements per thread (N*B/blockDim), a compile-time
+// constant so that every register array below is statically sized.
+template <int EPT>
 __device__ void lds_fft(double2 *x, int B, const FftPlan &P, const double2 *__restrict__ tw, int sign) {
   int Ns = 1;
   for (int p = 0; p < P.np; ++p) {
     switch (P.r[p]) {
-      case 2: stockham_pass<2, FFT_MAX_ELEMS / 2 / FFT_THREADS>(x, P.N, B, Ns, tw, sign); break;
-      case 3: stockham_pass<3, (FFT_MAX_ELEMS / 3 + FFT_THREADS - 1) / FFT_THREADS>(x, P.N, B, Ns, tw, sign); break;
-      case 4: stockham_pass<4, FFT_MAX_ELEMS / 4 / FFT_THREADS>(x, P.N, B, Ns, tw, sign); break;
-      case 5: stockham_pass<5, (FFT_MAX_ELEMS / 5 + FFT_THREADS - 1) / FFT_THREADS>(x, P.N, B, Ns, tw, sign); break;
-      case 8: stockham_pass<8, FFT_MAX_ELEMS / 8 / FFT_THREADS>(x, P.N, B, Ns, tw, sign); break;
+      case 2: stockham_pass<2, (EPT + 1) / 2>(x, P.N, B, Ns, tw, sign); break;
+      case 3: stockham_pass<3, (EPT + 2) / 3>(x, P.N, B, Ns, tw, sign); break;
+      case 4: stockham_pass<4, (EPT + 3) / 4>(x, P.N, B, Ns, tw, sign); break;
+      case 5: stockham_pass<5, (EPT + 4) / 5>(x, P.N, B, Ns, tw, sign); break;
+      case 8: stockham_pass<8, (EPT + 7) / 8>(x, P.N, B, Ns, tw, sign); break;
     }
     Ns *= P.r[p];
   }
@@ -157,6 +160,7 @@ __device__ void lds_fft(double2 *x, int B, const FftPlan &P, const double2 *__re
 // ------------------------------------------------------------------ z inverse
 // grid: (nxh/B, ny, nfield); the 4 state fields u, v, w, theta live in U
 // slots 0,1,2,6.  Output pad[f][y][z][kx].
+template <int EPT>
 __global__ void __launch_bounds__(FFT_THREADS) zinv_kernel(const double2 *__restrict__ U, double2 *__restrict__ pad,
                                                            FftPlan P, const double2 *__restrict__ tw, int nxh, int nz,
                                                            int ny, int B, size_t field, size_t pfield) {
@@ -173,7 +177,7 @@ __global__ void __launch_bounds__(FFT_THREADS) zinv_kernel(const double2 *__rest
     lds[e] = val;
   }
   __syncthreads();
-  lds_fft(lds, B, P, tw, +1);
+  lds_fft<EPT>(lds, B, P, tw, +1);
   double2 *dst = pad + (size_t)f * pfield + (size_t)y * N * nxh + kx0;
   for (int e = threadIdx.x; e < N * B; e += blockDim.x) {
     const int z = e / B, b = e % B;
@@ -184,6 +188,7 @@ __global__ void __launch_bounds__(FFT_THREADS) zinv_kernel(const double2 *__rest
 // ---------------------------------------------------------------- x physical
 // grid: (nz32, ny): one x line per block.  pad fields 0..3 = u,v,w,theta in,
 // 0..7 = spectra of u^2-v^2, uv, w^2-v^2, vw, uw, u theta, v theta, w theta out.
+template <int EPT>
 __global__ void __launch_bounds__(FFT_THREADS) xphys_kernel(double2 *__restrict__ pad, FftPlan P,
                                                             const double2 *__restrict__ tw, int nxh, int nz32,
                                                             size_t pfield, const double *__restrict__ dyr,
@@ -212,9 +217,9 @@ __global__ void __launch_bounds__(FFT_THREADS) xphys_kernel(double2 *__restrict_
     lds[2 * m + 1] = z2;
   }
   __syncthreads();
-  lds_fft(lds, 2, P, tw, +1);
+  lds_fft<EPT>(lds, 2, P, tw, +1);
   // products at this thread's points, kept in registers
-  constexpr int MAXP = FFT_MAX_ELEMS / 2 / FFT_THREADS;
+  constexpr int MAXP = (EPT + 1) / 2;
   double pu[MAXP], pv[MAXP], pw[MAXP], pt[MAXP];
   const double dy = dyr[y];
   double vmax = 0.0;
@@ -252,7 +257,7 @@ __global__ void __launch_bounds__(FFT_THREADS) xphys_kernel(double2 *__restrict_
       }
     }
     __syncthreads();
-    lds_fft(lds, 2, P, tw, -1);
+    lds_fft<EPT>(lds, 2, P, tw, -1);
     // unpack the two real spectra of each pair, keep modes 0..nxh-1
     for (int m = threadIdx.x; m < nxh; m += blockDim.x) {
       const int q = (m == 0) ? 0 : N - m;
@@ -273,6 +278,7 @@ __global__ void __launch_bounds__(FFT_THREADS) xphys_kernel(double2 *__restrict_
 // ------------------------------------------------------------------ z forward
 // grid: (nxh/B, ny, 8).  pad field g -> U slot (0,1,2,3,4,6,7,8)[g], with
 // truncation and rescale.
+template <int EPT>
 __global__ void __launch_bounds__(FFT_THREADS) zfwd_kernel(const double2 *__restrict__ pad, double2 *__restrict__ U,
                                                            FftPlan P, const double2 *__restrict__ tw, int nxh, int nz,
                                                            int ny, int B, size_t field, size_t pfield,
@@ -287,7 +293,7 @@ __global__ void __launch_bounds__(FFT_THREADS) zfwd_kernel(const double2 *__rest
     lds[e] = src[(size_t)z * nxh + b];
   }
   __syncthreads();
-  lds_fft(lds, B, P, tw, -1);
+  lds_fft<EPT>(lds, B, P, tw, -1);
   double2 *dst = U + (size_t)slot * field + (size_t)y * nz * nxh + kx0;
   for (int e = threadIdx.x; e < nz * B; e += blockDim.x) {
     const int j = e / B, b = e % B;
@@ -338,7 +344,7 @@ __global__ void wave00_kernel(LineParams P, WsArgs A, const double *scal) {
 }
 
 // Fused sweeps over all (kx,kz) != (0,0) systems, one thread per system.
-enum SweepId { SW_A1, SW_B1, SW_A, SW_B, SW_C, SW_D, SW_E, SW_F, SW_G };
+enum SweepId { SW_A1, SW_B1, SW_A, SW_BT, SW_CT, SW_DT, SW_BG, SW_BV, SW_CNU, SW_DNU, SW_E, SW_F, SW_G };
 
 template <int BW, int SW>
 __global__ void __launch_bounds__(64) sweep_kernel(LineParams P, WsArgs A, double *dbuf2, size_t nsys,
@@ -352,9 +358,13 @@ __global__ void __launch_bounds__(64) sweep_kernel(LineParams P, WsArgs A, doubl
   if (SW == SW_A1) sweep_A1<BW>(Q, A, c);
   if (SW == SW_B1) sweep_B1<BW>(Q, A, c);
   if (SW == SW_A) sweep_A<BW>(Q, A, c);
-  if (SW == SW_B) sweep_B<BW>(Q, A, c);
-  if (SW == SW_C) sweep_C<BW>(Q, A, c, dbuf2);
-  if (SW == SW_D) sweep_D<BW>(Q, A, c, dbuf2);
+  if (SW == SW_BT) sweep_Bt<BW>(Q, A, c);
+  if (SW == SW_CT) sweep_Ct<BW>(Q, A, c, dbuf2);
+  if (SW == SW_DT) sweep_Dt<BW>(Q, A, c, dbuf2);
+  if (SW == SW_BG) sweep_Bg<BW>(Q, A, c);
+  if (SW == SW_BV) sweep_Bv<BW>(Q, A, c);
+  if (SW == SW_CNU) sweep_Cnu<BW>(Q, A, c);
+  if (SW == SW_DNU) sweep_Dnu<BW>(Q, A, c);
   if (SW == SW_E) {
     cplx vlo = mk(0.0, 0.0), vhi = mk(0.0, 0.0);
     if (A.opp_alpha != 0.0 && A.vd) {

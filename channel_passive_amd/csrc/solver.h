@@ -21,11 +21,23 @@ struct SV {
   PB_HD cplx &operator[](int i) const { return p[(size_t)i * s]; }
 };
 
+// Read of data that every lane of a wave reads at the same address and that
+// no kernel writes (the shared band matrices): on the device the pointer is
+// taken in the constant address space, so the compiler emits scalar loads
+// into SGPRs instead of a vector load per lane.
+PB_HD double ld_uniform(const double *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *(const __attribute__((address_space(4))) double *)p;
+#else
+  return *p;
+#endif
+}
+
 // band read from a shared (k-independent) matrix: A[r*W + slot]
 struct SharedBand {
   const double *a;
   int W;
-  PB_HD double operator()(int r, int slot) const { return a[r * W + slot]; }
+  PB_HD double operator()(int r, int slot) const { return ld_uniform(a + r * W + slot); }
 };
 
 // per-system band: element (r, slot) at p[(r*W + slot) * s]

@@ -15,6 +15,7 @@
 //             substitution of g, phi_p, phi_h and theta
 //   D  (up)   back substitution + D0/D2 products -> g, theta, phi_p, phi_h,
 //             D2 phi_h and the stores of g and theta
+//   (B, C, D each run as one kernel per right-hand-side group)
 //   E  (down) Crout LU of the Poisson operator + forward substitution of v_p, v_h
 //   F  (up)   back substitution + D0/D1 products -> v_p, v_p', v_h', D0 c_h
 //   G  (mirror pairs) influence coefficients C1, C2, f = v', v, u, w, store_phi
@@ -26,13 +27,17 @@
 // row m), so every window index is a compile-time constant.
 //
 // Slot use per system (0-based product slots; s = substep):
-//   after A: 0 yB, 1 P2, 2 yE, 3 P4, 4 yF, 6 u1a, 7 y8, 8 u7a
-//   after B: 0 rhs_g, 2 rhs_phi, 4 rhs_theta, RK = zeta*h (s < S)
-//   after C: 0 y_g, 1 y_phih, 2 y_phip, 4 y_theta; U factors in dbuf, dbuf2
-//   after D: 0 g, 1 phi_h, 2 phi_p, 3 D2 phi_h, 6 theta; RK updated
-//   after E: 2 y_vp, 5 y_vh; U factor in dbuf
-//   after F: 2 v_p, 5 v_h', 7 v_p', 8 D0 c_h
-//   after G: 0 u, 1 v, 2 w, 6 theta; RK2 = store_phi
+//   after A : 0 yB, 1 P2, 2 yE, 3 P4, 4 yF, 6 u1a, 7 y8, 8 u7a
+//   after B : 0 rhs_phi, 2 rhs_g, 7 rhs_theta, RK = zeta*h (s < S)
+//   after C : 0 y_phip, 1 y_phih, 2 y_g, 7 y_theta; U factors in dbuf, dbuf2
+//   after D : 0 phi_p, 1 phi_h, 2 g, 3 D2 phi_h, 6 theta; RK updated
+//   after E : 0 y_vp, 5 y_vh; U factor in dbuf
+//   after F : 0 v_p, 5 v_h', 7 v_p', 8 D0 c_h
+//   after G : 0 u, 1 v, 2 w, 6 theta; RK2 = store_phi
+// B, C and D are split by right-hand-side group (theta | g | phi,v): the
+// groups read disjoint data, so the split costs no traffic and keeps the
+// register windows of each kernel small; the theta chain can run beside
+// the others.
 // s == 1 adds a down/up pair (A1/B1) that forms D2 LU(D0)^-1 of the three
 // stores in scratch columns buf0, buf1 and slot 5.
 #pragma once
@@ -149,12 +154,13 @@ PB_HD double mv_row_r(const BandGeom &g, const SharedBand &A, int m, int r, cons
 template <int BW, class F>
 PB_HD void for_completed_rows(int n, int r, F &&emit) {
   if (r <= n - 1 - 2 * BW) emit(r + BW);
+  // boundary rows: only m varies, the window indices stay compile-time constants
   if (r == n - 1 - 2 * BW) {
-#pragma unroll
+#pragma nounroll
     for (int t = 0; t < BW; ++t) emit(n - BW + t);
   }
   if (r == 0) {
-#pragma unroll
+#pragma nounroll
     for (int t = 0; t < BW; ++t) emit(t);
   }
 }
@@ -310,143 +316,212 @@ PB_HD void sweep_A(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   }
 }
 
-// B: back substitution, hvhg's band products, the Helmholtz right-hand sides
-// (solve_g.f90:84-123, solve_v_f.f90:56-95, solve_t.f90:84-123) and the
-// zeta*h stores (solve_g.f90:138-143, solve_v_f.f90:115-120)
+// B, split by right-hand-side group (disjoint inputs, so the split moves no
+// extra data): back substitution with LU(D0), hvhg's band products, the
+// Helmholtz right-hand sides (solve_g.f90:84-123, solve_v_f.f90:56-95,
+// solve_t.f90:84-123) and the zeta*h part of the stores (solve_g.f90:138-143,
+// solve_v_f.f90:115-120).
+//   B_g: h_g = u1a + D1 c_E          -> rhs_g in slot 2, store_g = zeta h_g
+//   B_t: h_t = -(u7a + D1 c_8)       -> rhs_t in slot 7, store_t = zeta h_t
+//   B_v: h_v = ((D1 c_B + a P2) + b P4) + D2 c_F -> rhs_phi in slot 0
+struct RhsCoef {
+  double gamma, al, du, zeta;
+};
+PB_HD RhsCoef rhs_coef(const LineParams &P, double k2, bool scalar) {
+  RhsCoef r;
+  const int s = P.s;
+  r.gamma = P.dt * P.coef(2, s);
+  r.al = scalar ? P.dt * P.coef(0, s) * P.nu / P.pr : P.dt * P.coef(0, s) * P.nu;
+  r.du = 1.0 - r.al * k2;
+  r.zeta = (s != P.S) ? P.dt * P.coef(3, s + 1) : 0.0;
+  return r;
+}
+// right-hand side of one row and the zeta*h store
+PB_HD cplx rhs_row(const LineParams &P, const RhsCoef &q, int m, int n, SV &store, const SV &d2s, cplx h) {
+  cplx rhs;
+  if (P.s == 1)
+    rhs = cadd(cadd(cscale(store[m], q.du), cscale(h, q.gamma)), cscale(d2s[m], q.al));
+  else
+    rhs = cadd(store[m], cscale(h, q.gamma));
+  if (m == 0 || m == n - 1) rhs = mk(0.0, 0.0);
+  if (P.s != P.S) store[m] = cscale(h, q.zeta);
+  return rhs;
+}
+
 template <int BW>
-PB_HD void sweep_B(const LineParams &P, const WsArgs &A, const SysCtx &c) {
+PB_HD void sweep_Bg(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
-  const int n = c.n, s = P.s, S = P.S;
-  const double kx = c.kx, kz = c.kz, k2 = c.k2, dt = P.dt, nu = P.nu, pr = P.pr;
-  const cplx ikx3 = iunit(kx * kx * kx), ikxkz2 = iunit(kx * (kz * kz));
-  const cplx ikx2kz = iunit((kx * kx) * kz), ikz3 = iunit(kz * kz * kz);
-  const cplx a = cadd(ikx3, ikxkz2), b = cadd(ikx2kz, ikz3);
-  const double gamma = dt * P.coef(2, s);
-  const double al_g = dt * P.coef(0, s) * nu, al_t = dt * P.coef(0, s) * nu / pr;
-  const double du_g = 1.0 - al_g * k2, du_t = 1.0 - al_t * k2;
-  const double zeta = (s != S) ? dt * P.coef(3, s + 1) : 0.0;
-  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s4 = c.col(A, 4), s6 = c.col(A, 6),
-     s7 = c.col(A, 7), s8 = c.col(A, 8);
-  SV r1 = c.rk(A, 0), r2 = c.rk(A, 1), r3 = c.rk(A, 2);
-  SV d1 = c.buf(A, 0), d2 = c.buf(A, 1), d3 = c.col(A, 5);
-  CWin<2 * BW + 1> cB, cE, cF, c8;
-  cB.clear(); cE.clear(); cF.clear(); c8.clear();
+  const int n = c.n;
+  const RhsCoef q = rhs_coef(P, c.k2, false);
+  SV s2 = c.col(A, 2), s6 = c.col(A, 6), r1 = c.rk(A, 0), d1 = c.buf(A, 0);
+  CWin<2 * BW + 1> cE;
+  cE.clear();
   for (int r = n - 1; r >= 0; --r) {
-    const cplx xB = bwd_row<BW>(g, P.D0LU, r, cB, s0[r]);
-    const cplx xE = bwd_row<BW>(g, P.D0LU, r, cE, s2[r]);
-    const cplx xF = bwd_row<BW>(g, P.D0LU, r, cF, s4[r]);
-    const cplx x8 = bwd_row<BW>(g, P.D0LU, r, c8, s7[r]);
-    cB.push(xB); cE.push(xE); cF.push(xF); c8.push(x8);
+    cE.push(bwd_row<BW>(g, P.D0LU, r, cE, s2[r]));
     for_completed_rows<BW>(n, r, [&](int m) {
-      const cplx u5a = mv_row<BW>(g, P.D1, m, r, cB);
-      const cplx u2c = mv_row<BW>(g, P.D1, m, r, cE);
-      const cplx u3c = mv_row<BW>(g, P.D2, m, r, cF);
-      const cplx u9a = mv_row<BW>(g, P.D1, m, r, c8);
-      const cplx u5b = cadd(cadd(u5a, cmul(a, s1[m])), cmul(b, s3[m]));
-      const cplx hv = cadd(u5b, u3c);
-      const cplx hg = cadd(s6[m], u2c);
-      const cplx ht = cneg(cadd(s8[m], u9a));
-      cplx rg, rp, rt;
-      if (s == 1) {
-        rg = cadd(cadd(cscale(r1[m], du_g), cscale(hg, gamma)), cscale(d1[m], al_g));
-        rp = cadd(cadd(cscale(r2[m], du_g), cscale(hv, gamma)), cscale(d2[m], al_g));
-        rt = cadd(cadd(cscale(r3[m], du_t), cscale(ht, gamma)), cscale(d3[m], al_t));
-      } else {
-        rg = cadd(r1[m], cscale(hg, gamma));
-        rp = cadd(r2[m], cscale(hv, gamma));
-        rt = cadd(r3[m], cscale(ht, gamma));
-      }
-      if (m == 0 || m == n - 1) {
-        rg = mk(0.0, 0.0); rp = mk(0.0, 0.0); rt = mk(0.0, 0.0);
-      }
-      if (s != S) {
-        r1[m] = cscale(hg, zeta);
-        r2[m] = cscale(hv, zeta);
-        r3[m] = cscale(ht, zeta);
-      }
-      s0[m] = rg; s2[m] = rp; s4[m] = rt;
+      const cplx hg = cadd(s6[m], mv_row<BW>(g, P.D1, m, r, cE));
+      s2[m] = rhs_row(P, q, m, n, r1, d1, hg);
     });
   }
 }
 
-// C: LU of H_nu and H_theta fused with their forward substitutions
-// (solve_g.f90:42-71, 158; solve_v_f.f90:142-148; solve_t.f90)
 template <int BW>
-PB_HD void sweep_C(const LineParams &P, const WsArgs &A, const SysCtx &c, double *dbuf2) {
+PB_HD void sweep_Bt(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
-  const int n = c.n, s = P.s;
-  const double dt = P.dt, nu = P.nu, pr = P.pr, k2 = c.k2;
-  const double be_g = dt * P.coef(1, s) * nu, be_t = dt * P.coef(1, s) * nu / pr;
-  const HelmholtzA Hg{P.D0, P.D2, 1.0 + be_g * k2, be_g, n, BW};
-  const HelmholtzA Ht{P.D0, P.D2, 1.0 + be_t * k2, be_t, n, BW};
-  const OwnU Ug{A.dbuf + c.sys, A.plane, 2 * BW + 1}, Ut{dbuf2 + c.sys, A.plane, 2 * BW + 1};
-  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s4 = c.col(A, 4);
-  CWin<2 * BW> wg, wp, wt;
-  RWin<2 * BW> wh;
-  wg.clear(); wp.clear(); wt.clear(); wh.clear();
-  double Lg[2 * BW + 1], Lt[2 * BW + 1];
-  double d00 = 0.0;
-  for (int r = 0; r < n; ++r) {
-    crout_row<BW>(g, Hg, Ug, r, Lg);
-    crout_row<BW>(g, Ht, Ut, r, Lt);
-    if (r == 0) d00 = Lg[0];
-    const cplx yg = fwd_lrow<BW>(g, r, Lg, wg, s0[r]);
-    const cplx yp = fwd_lrow<BW>(g, r, Lg, wp, s2[r]);
-    const double yh = fwd_lrow_r<BW>(g, r, Lg, wh, r == 0 ? d00 : 0.0);
-    const cplx yt = fwd_lrow<BW>(g, r, Lt, wt, s4[r]);
-    s0[r] = yg; s2[r] = yp; s1[r] = mk(yh, yh); s4[r] = yt;
-    wg.push(yg); wp.push(yp); wh.push(yh); wt.push(yt);
+  const int n = c.n;
+  const RhsCoef q = rhs_coef(P, c.k2, true);
+  SV s7 = c.col(A, 7), s8 = c.col(A, 8), r3 = c.rk(A, 2), d3 = c.col(A, 5);
+  CWin<2 * BW + 1> c8;
+  c8.clear();
+  for (int r = n - 1; r >= 0; --r) {
+    c8.push(bwd_row<BW>(g, P.D0LU, r, c8, s7[r]));
+    for_completed_rows<BW>(n, r, [&](int m) {
+      const cplx ht = cneg(cadd(s8[m], mv_row<BW>(g, P.D1, m, r, c8)));
+      s7[m] = rhs_row(P, q, m, n, r3, d3, ht);
+    });
   }
 }
 
-// D: back substitution with U of H_nu / H_theta and the D0/D2 products;
-// stores of g and theta (solve_g.f90:158-185), the phi_p part of store_phi
-// (solve_v_f.f90:150-181)
 template <int BW>
-PB_HD void sweep_D(const LineParams &P, const WsArgs &A, const SysCtx &c, double *dbuf2) {
+PB_HD void sweep_Bv(const LineParams &P, const WsArgs &A, const SysCtx &c) {
+  const BandGeom g{c.n, BW};
+  const int n = c.n;
+  const double kx = c.kx, kz = c.kz;
+  const cplx ikx3 = iunit(kx * kx * kx), ikxkz2 = iunit(kx * (kz * kz));
+  const cplx ikx2kz = iunit((kx * kx) * kz), ikz3 = iunit(kz * kz * kz);
+  const cplx a = cadd(ikx3, ikxkz2), b = cadd(ikx2kz, ikz3);
+  const RhsCoef q = rhs_coef(P, c.k2, false);
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s3 = c.col(A, 3), s4 = c.col(A, 4), r2 = c.rk(A, 1), d2 = c.buf(A, 1);
+  CWin<2 * BW + 1> cB, cF;
+  cB.clear(); cF.clear();
+  for (int r = n - 1; r >= 0; --r) {
+    cB.push(bwd_row<BW>(g, P.D0LU, r, cB, s0[r]));
+    cF.push(bwd_row<BW>(g, P.D0LU, r, cF, s4[r]));
+    for_completed_rows<BW>(n, r, [&](int m) {
+      const cplx u5b = cadd(cadd(mv_row<BW>(g, P.D1, m, r, cB), cmul(a, s1[m])), cmul(b, s3[m]));
+      const cplx hv = cadd(u5b, mv_row<BW>(g, P.D2, m, r, cF));
+      s0[m] = rhs_row(P, q, m, n, r2, d2, hv);
+    });
+  }
+}
+
+// C, split by matrix: Crout LU row by row fused with the forward
+// substitutions of that matrix's right-hand sides (solve_g.f90:42-71, 158;
+// solve_v_f.f90:142-148; solve_t.f90:42-71, 158).
+//   C_nu: H_nu; rhs_g (slot 2), rhs_phi (slot 0), phi_h seed -> slot 1
+//   C_t : H_theta; rhs_t (slot 7)
+template <int BW>
+PB_HD void sweep_Cnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
+  const BandGeom g{c.n, BW};
+  const int n = c.n;
+  const double be = P.dt * P.coef(1, P.s) * P.nu;
+  const HelmholtzA H{P.D0, P.D2, 1.0 + be * c.k2, be, n, BW};
+  const OwnU U{A.dbuf + c.sys, A.plane, 2 * BW + 1};
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2);
+  CWin<2 * BW> wg, wp;
+  RWin<2 * BW> wh;
+  wg.clear(); wp.clear(); wh.clear();
+  double L[2 * BW + 1];
+  double d00 = 0.0;
+  for (int r = 0; r < n; ++r) {
+    crout_row<BW>(g, H, U, r, L);
+    if (r == 0) d00 = L[0];
+    const cplx yg = fwd_lrow<BW>(g, r, L, wg, s2[r]);
+    const cplx yp = fwd_lrow<BW>(g, r, L, wp, s0[r]);
+    const double yh = fwd_lrow_r<BW>(g, r, L, wh, r == 0 ? d00 : 0.0);
+    s2[r] = yg; s0[r] = yp; s1[r] = mk(yh, yh);
+    wg.push(yg); wp.push(yp); wh.push(yh);
+  }
+}
+
+template <int BW>
+PB_HD void sweep_Ct(const LineParams &P, const WsArgs &A, const SysCtx &c, double *dbuf2) {
+  const BandGeom g{c.n, BW};
+  const int n = c.n;
+  const double be = P.dt * P.coef(1, P.s) * P.nu / P.pr;
+  const HelmholtzA H{P.D0, P.D2, 1.0 + be * c.k2, be, n, BW};
+  const OwnU U{dbuf2 + c.sys, A.plane, 2 * BW + 1};
+  SV s7 = c.col(A, 7);
+  CWin<2 * BW> wt;
+  wt.clear();
+  double L[2 * BW + 1];
+  for (int r = 0; r < n; ++r) {
+    crout_row<BW>(g, H, U, r, L);
+    const cplx yt = fwd_lrow<BW>(g, r, L, wt, s7[r]);
+    s7[r] = yt;
+    wt.push(yt);
+  }
+}
+
+// D, split by matrix: back substitution and the D0/D2 products
+// (solve_g.f90:158-185, solve_v_f.f90:150-181, solve_t.f90:158-185).
+//   D_nu: g -> slot 2, store_g; phi_p -> slot 0, phi_h -> slot 1,
+//         D2 phi_h -> slot 3, the phi_p part of store_phi
+//   D_t : theta -> slot 6, store_t
+template <int BW>
+PB_HD void sweep_Dnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n, s = P.s, S = P.S;
-  const double dt = P.dt, nu = P.nu, pr = P.pr, k2 = c.k2;
-  const OwnU Ug{A.dbuf + c.sys, A.plane, 2 * BW + 1}, Ut{dbuf2 + c.sys, A.plane, 2 * BW + 1};
-  double al_g = 0, du_g = 0, al_t = 0, du_t = 0, al_p = 0, du_p = 0;
+  const double dt = P.dt, nu = P.nu, k2 = c.k2;
+  const OwnU U{A.dbuf + c.sys, A.plane, 2 * BW + 1};
+  double al_g = 0, du_g = 0, al_p = 0, du_p = 0;
   if (s != S) {
     al_g = dt * P.coef(0, s + 1) * nu;
     du_g = 1.0 - al_g * k2;
-    al_t = dt * P.coef(0, s + 1) * nu / pr;
-    du_t = 1.0 - al_t * k2;
     al_p = dt * nu * P.coef(0, s + 1);
     du_p = 1.0 - al_p * k2;
   }
-  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s4 = c.col(A, 4), s6 = c.col(A, 6);
-  SV r1 = c.rk(A, 0), r2 = c.rk(A, 1), r3 = c.rk(A, 2);
-  CWin<2 * BW + 1> wg, wp, wt;
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3);
+  SV r1 = c.rk(A, 0), r2 = c.rk(A, 1);
+  CWin<2 * BW + 1> wg, wp;
   RWin<2 * BW + 1> wh;
-  wg.clear(); wp.clear(); wt.clear(); wh.clear();
+  wg.clear(); wp.clear(); wh.clear();
   for (int r = n - 1; r >= 0; --r) {
-    const cplx xg = bwd_row<BW>(g, Ug, r, wg, s0[r]);
-    const cplx xp = bwd_row<BW>(g, Ug, r, wp, s2[r]);
-    const double xh = bwd_row_r<BW>(g, Ug, r, wh, s1[r].re);
-    const cplx xt = bwd_row<BW>(g, Ut, r, wt, s4[r]);
-    wg.push(xg); wp.push(xp); wh.push(xh); wt.push(xt);
+    wg.push(bwd_row<BW>(g, U, r, wg, s2[r]));
+    wp.push(bwd_row<BW>(g, U, r, wp, s0[r]));
+    wh.push(bwd_row_r<BW>(g, U, r, wh, s1[r].re));
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx gv = mv_row<BW>(g, P.D0, m, r, wg);
       const cplx d2g = mv_row<BW>(g, P.D2, m, r, wg);
-      const cplx tv = mv_row<BW>(g, P.D0, m, r, wt);
-      const cplx d2t = mv_row<BW>(g, P.D2, m, r, wt);
       const cplx pv = mv_row<BW>(g, P.D0, m, r, wp);
       const cplx d2p = mv_row<BW>(g, P.D2, m, r, wp);
       const double hv = mv_row_r<BW>(g, P.D0, m, r, wh);
       const double d2h = mv_row_r<BW>(g, P.D2, m, r, wh);
       if (s != S) {
         r1[m] = cadd(cadd(r1[m], cscale(gv, du_g)), cscale(d2g, al_g));
-        r3[m] = cadd(cadd(r3[m], cscale(tv, du_t)), cscale(d2t, al_t));
         r2[m] = cadd(cadd(r2[m], cscale(pv, du_p)), cscale(d2p, al_p));
       } else {
         r1[m] = gv;
-        r3[m] = tv;
         r2[m] = pv;
       }
-      s0[m] = gv; s6[m] = tv; s2[m] = pv; s1[m] = mk(hv, hv); s3[m] = mk(d2h, d2h);
+      s2[m] = gv; s0[m] = pv; s1[m] = mk(hv, hv); s3[m] = mk(d2h, d2h);
+    });
+  }
+}
+
+template <int BW>
+PB_HD void sweep_Dt(const LineParams &P, const WsArgs &A, const SysCtx &c, double *dbuf2) {
+  const BandGeom g{c.n, BW};
+  const int n = c.n, s = P.s, S = P.S;
+  const OwnU U{dbuf2 + c.sys, A.plane, 2 * BW + 1};
+  double al = 0, du = 0;
+  if (s != S) {
+    al = P.dt * P.coef(0, s + 1) * P.nu / P.pr;
+    du = 1.0 - al * c.k2;
+  }
+  SV s6 = c.col(A, 6), s7 = c.col(A, 7), r3 = c.rk(A, 2);
+  CWin<2 * BW + 1> wt;
+  wt.clear();
+  for (int r = n - 1; r >= 0; --r) {
+    wt.push(bwd_row<BW>(g, U, r, wt, s7[r]));
+    for_completed_rows<BW>(n, r, [&](int m) {
+      const cplx tv = mv_row<BW>(g, P.D0, m, r, wt);
+      if (s != S) {
+        r3[m] = cadd(cadd(r3[m], cscale(tv, du)), cscale(mv_row<BW>(g, P.D2, m, r, wt), al));
+      } else {
+        r3[m] = tv;
+      }
+      s6[m] = tv;
     });
   }
 }
@@ -459,49 +534,49 @@ PB_HD void sweep_E(const LineParams &P, const WsArgs &A, const SysCtx &c, cplx v
   const int n = c.n;
   const PoissonA Q{P.D0, P.D2, c.k2, n, BW};
   const OwnU Uq{A.dbuf + c.sys, A.plane, 2 * BW + 1};
-  SV s1 = c.col(A, 1), s2 = c.col(A, 2), s5 = c.col(A, 5);
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s5 = c.col(A, 5);
   CWin<2 * BW> wp;
   RWin<2 * BW> wh;
   wp.clear(); wh.clear();
   double L[2 * BW + 1];
   for (int r = 0; r < n; ++r) {
     crout_row<BW>(g, Q, Uq, r, L);
-    const cplx bp = (r == 0) ? vlo : (r == n - 1 ? vhi : s2[r]);
+    const cplx bp = (r == 0) ? vlo : (r == n - 1 ? vhi : s0[r]);
     const double bh = (r == 0 || r == n - 1) ? 0.0 : s1[r].re;
     const cplx yp = fwd_lrow<BW>(g, r, L, wp, bp);
     const double yh = fwd_lrow_r<BW>(g, r, L, wh, bh);
-    s2[r] = yp; s5[r] = mk(yh, yh);
+    s0[r] = yp; s5[r] = mk(yh, yh);
     wp.push(yp); wh.push(yh);
   }
 }
 
 // F: back substitution with U of the Poisson operator and the D0/D1
-// products v_p, v_p', v_h', D0 c_h (solve_v_f.f90:258-271, 352)
+// products v_p -> slot 0, v_p' -> slot 7, v_h' -> slot 5, D0 c_h -> slot 8
+// (solve_v_f.f90:258-271, 352)
 template <int BW>
 PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const OwnU Uq{A.dbuf + c.sys, A.plane, 2 * BW + 1};
-  SV s2 = c.col(A, 2), s5 = c.col(A, 5), s7 = c.col(A, 7), s8 = c.col(A, 8);
+  SV s0 = c.col(A, 0), s5 = c.col(A, 5), s7 = c.col(A, 7), s8 = c.col(A, 8);
   CWin<2 * BW + 1> wp;
   RWin<2 * BW + 1> wh;
   wp.clear(); wh.clear();
   for (int r = n - 1; r >= 0; --r) {
-    const cplx xp = bwd_row<BW>(g, Uq, r, wp, s2[r]);
-    const double xh = bwd_row_r<BW>(g, Uq, r, wh, s5[r].re);
-    wp.push(xp); wh.push(xh);
+    wp.push(bwd_row<BW>(g, Uq, r, wp, s0[r]));
+    wh.push(bwd_row_r<BW>(g, Uq, r, wh, s5[r].re));
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx vp = mv_row<BW>(g, P.D0, m, r, wp);
       const cplx dvp = mv_row<BW>(g, P.D1, m, r, wp);
       const double dvh = mv_row_r<BW>(g, P.D1, m, r, wh);
       const double d0h = mv_row_r<BW>(g, P.D0, m, r, wh);
-      s2[m] = vp; s7[m] = dvp; s5[m] = mk(dvh, dvh); s8[m] = mk(d0h, d0h);
+      s0[m] = vp; s7[m] = dvp; s5[m] = mk(dvh, dvh); s8[m] = mk(d0h, d0h);
     });
   }
 }
 
 // G: influence coefficients and the final combination on mirror pairs
-// (solve_v_f.f90:293-355, solve_u_w.f90:27-38)
+// (solve_v_f.f90:293-355, solve_u_w.f90:27-38): u -> slot 0, v -> 1, w -> 2
 PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const int n = c.n, s = P.s, S = P.S;
   const double dt = P.dt, nu = P.nu, k2 = c.k2;
@@ -524,7 +599,7 @@ PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
     const int j = n - 1 - i;
     const cplx dvp_i = s7[i], dvp_j = s7[j], dvh_i = s5[i], dvh_j = s5[j];
     const cplx sp_i = r2[i], sp_j = r2[j], ph_i = s1[i], ph_j = s1[j], d2h_i = s3[i], d2h_j = s3[j];
-    const cplx vp_i = s2[i], vp_j = s2[j], d0h_i = s8[i], d0h_j = s8[j], g_i = s0[i], g_j = s0[j];
+    const cplx vp_i = s0[i], vp_j = s0[j], d0h_i = s8[i], d0h_j = s8[j], g_i = s2[i], g_j = s2[j];
     auto one = [&](int row, cplx dvp, cplx dvh, cplx dvh_m, cplx sp, cplx ph, cplx ph_m, cplx d2h, cplx d2h_m,
                    cplx vp, cplx d0h, cplx d0h_m, cplx gg) {
       const cplx f = csub(cadd(dvp, cmul(c1, dvh)), cmul(c2, dvh_m));
@@ -559,9 +634,13 @@ PB_HD void fused_line(const LineParams &P, const WsArgs &A, double *dbuf2, size_
     sweep_B1<BW>(P, A, c);
   }
   sweep_A<BW>(P, A, c);
-  sweep_B<BW>(P, A, c);
-  sweep_C<BW>(P, A, c, dbuf2);
-  sweep_D<BW>(P, A, c, dbuf2);
+  sweep_Bg<BW>(P, A, c);
+  sweep_Bt<BW>(P, A, c);
+  sweep_Ct<BW>(P, A, c, dbuf2);
+  sweep_Dt<BW>(P, A, c, dbuf2);
+  sweep_Bv<BW>(P, A, c);
+  sweep_Cnu<BW>(P, A, c);
+  sweep_Dnu<BW>(P, A, c);
   cplx vlo = mk(0.0, 0.0), vhi = mk(0.0, 0.0);
   if (A.opp_alpha != 0.0 && A.vd) {
     vlo = cscale(A.vd[sys], -A.opp_alpha);
