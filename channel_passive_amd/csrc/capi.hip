@@ -80,6 +80,8 @@ struct pb_ctx {
   // device memory
   double2 *U = nullptr, *RK = nullptr, *pad = nullptr, *buf = nullptr, *vd = nullptr;
   double *dbuf = nullptr, *dbuf2 = nullptr, *vpart = nullptr, *scal = nullptr;
+  size_t mean_lds = 0;        // dynamic LDS of mean_kernel (0: one-thread wave00_kernel)
+  double *mean_db = nullptr;  // mean_kernel band scratch when it does not fit in LDS
   double *dD0 = nullptr, *dD1 = nullptr, *dD2 = nullptr, *dD0LU = nullptr, *dinteg = nullptr, *ddyr = nullptr,
          *drows = nullptr;
   double2 *twz = nullptr, *twx = nullptr;
@@ -128,7 +130,7 @@ const char *pb_last_error(void) { return g_err.c_str(); }
 static int ctx_free(pb_ctx *c) {
   if (!c) return 0;
   void *ptrs[] = {c->U, c->RK, c->pad, c->buf, c->vd, c->dbuf, c->dbuf2, c->vpart, c->scal, c->dD0, c->dD1, c->dD2,
-                  c->dD0LU, c->dinteg, c->ddyr, c->drows, c->twz, c->twx, c->slotmap};
+                  c->dD0LU, c->dinteg, c->ddyr, c->drows, c->twz, c->twx, c->slotmap, c->mean_db};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -271,6 +273,20 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   rc = rc ? rc : dalloc((void **)&c->twz, c->nz32 * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->twx, c->nx32 * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->slotmap, 16 * sizeof(int));
+  // mean line: its columns (and, when they fit, its band) in LDS
+  {
+    const size_t cols = (size_t)pb::MEAN_COLS * c->ny * sizeof(double2), db = bandb;
+    if (cols + db <= (size_t)lds_max) {
+      c->mean_lds = cols + db;
+    } else if (cols <= (size_t)lds_max) {
+      c->mean_lds = cols;
+      rc = rc ? rc : dalloc((void **)&c->mean_db, 3 * bandb);
+    }
+    if (!rc && c->mean_lds > 65536 &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&pb::mean_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->mean_lds) != hipSuccess)
+      c->mean_lds = 0;
+  }
   if (rc) { ctx_free(c); return rc; }
   if (hipHostMalloc((void **)&c->hscal, SCAL_N * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     ctx_free(c);
@@ -563,20 +579,30 @@ static int launch_wavespace(pb_ctx *c, int s) {
   A.mean_out = c->scal + 2 + 4 * (s - 1);
   StageTimer tm(c, PB_STAGE_WAVESPACE);
   if (c->fused && (c->bw >= 3 && c->bw <= 5)) {
-    // The mean line touches only its own column: fork it onto the side
-    // stream, run the sweeps of all other systems, join before returning.
-    HIPCHK(hipEventRecord(c->ev_fork, c->stream));
-    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-    hipLaunchKernelGGL(pb::wave00_kernel, dim3(1), dim3(64), 0, c->side, P, A, c->scal);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev_join, c->side));
+    // mean_kernel touches only the columns of system 0 and its own LDS: fork
+    // it onto the side stream, run the sweeps of all other systems, join.
+    // The one-thread wave00_kernel also uses the scratch columns, which the
+    // sweeps use too, so it runs after them.
+    if (c->mean_lds) {
+      HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+      HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+      hipLaunchKernelGGL(pb::mean_kernel, dim3(3), dim3(pb::MEAN_THREADS), c->mean_lds, c->side, P, A, c->scal,
+                         c->mean_db);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c->ev_join, c->side));
+    }
     int r = PB_OK;
     switch (c->bw) {
       case 3: r = launch_sweeps<3>(c, P, A, s); break;
       case 4: r = launch_sweeps<4>(c, P, A, s); break;
       case 5: r = launch_sweeps<5>(c, P, A, s); break;
     }
-    HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    if (c->mean_lds) {
+      HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    } else {
+      hipLaunchKernelGGL(pb::wave00_kernel, dim3(1), dim3(64), 0, c->stream, P, A, c->scal);
+      HIPCHK(hipGetLastError());
+    }
     return r;
   }
   hipLaunchKernelGGL(pb::wavespace_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P, A, c->nsys,

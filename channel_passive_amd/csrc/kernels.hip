@@ -157,6 +157,26 @@ __device__ void lds_fft(double2 *x, int B, const FftPlan &P, const double2 *__re
   }
 }
 
+// XCD-aware block numbering.  Workgroups are dealt to the 8 XCDs in turn
+// and each XCD has its own L2; a z-kernel block touches B*16 = 64 bytes of
+// every 128-byte line, its x-neighbour the other half.  Renumbering so that
+// each XCD runs one contiguous range of logical blocks keeps both halves of
+// a line in one L2.  Identity when the grid does not split evenly.
+struct Blk {
+  int x, y, z;
+};
+__device__ inline Blk xcd_block() {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const size_t n = (size_t)gx * gy * gridDim.z;
+  const size_t h = blockIdx.x + (size_t)gx * (blockIdx.y + (size_t)gy * blockIdx.z);
+  const size_t L = (n % 8 == 0) ? (h % 8) * (n / 8) + h / 8 : h;
+  Blk b;
+  b.x = (int)(L % gx);
+  b.y = (int)((L / gx) % gy);
+  b.z = (int)(L / ((size_t)gx * gy));
+  return b;
+}
+
 // ------------------------------------------------------------------ z inverse
 // grid: (nxh/B, ny, nfield); the 4 state fields u, v, w, theta live in U
 // slots 0,1,2,6.  Output pad[f][y][z][kx].
@@ -165,7 +185,8 @@ __global__ void __launch_bounds__(FFT_THREADS) zinv_kernel(const double2 *__rest
                                                            FftPlan P, const double2 *__restrict__ tw, int nxh, int nz,
                                                            int ny, int B, size_t field, size_t pfield) {
   extern __shared__ double2 lds[];
-  const int kx0 = blockIdx.x * B, y = blockIdx.y, f = blockIdx.z;
+  const Blk bk = xcd_block();
+  const int kx0 = bk.x * B, y = bk.y, f = bk.z;
   const int slot = (f == 3) ? 6 : f;
   const int N = P.N, nh = nz / 2;
   const double2 *src = U + (size_t)slot * field + (size_t)y * nz * nxh + kx0;
@@ -284,7 +305,8 @@ __global__ void __launch_bounds__(FFT_THREADS) zfwd_kernel(const double2 *__rest
                                                            int ny, int B, size_t field, size_t pfield,
                                                            double rescale) {
   extern __shared__ double2 lds[];
-  const int kx0 = blockIdx.x * B, y = blockIdx.y, g = blockIdx.z;
+  const Blk bk = xcd_block();
+  const int kx0 = bk.x * B, y = bk.y, g = bk.z;
   const int slot = g < 5 ? g : g + 1;
   const int N = P.N, nh = nz / 2;
   const double2 *src = pad + (size_t)g * pfield + (size_t)y * N * nxh + kx0;
@@ -343,38 +365,80 @@ __global__ void wave00_kernel(LineParams P, WsArgs A, const double *scal) {
   wavespace_line(Q, A, 0);
 }
 
-// Fused sweeps over all (kx,kz) != (0,0) systems, one thread per system.
-enum SweepId { SW_A1, SW_B1, SW_A, SW_BT, SW_CT, SW_DT, SW_BG, SW_BV, SW_CNU, SW_DNU, SW_E, SW_F, SW_G };
+// The (0,0) system split by field, one block each: block 0 the mean
+// streamwise velocity, 1 the spanwise velocity, 2 the scalar (the three
+// solves of wave00.f90 are independent).  The block copies its columns of
+// system 0 into LDS, thread 0 runs the recurrences there (same functions,
+// same arithmetic as the one-thread path), and the block writes back what
+// the one-thread path leaves in the state.  Db lives in LDS when
+// db_scratch is null, otherwise in db_scratch + field * W * ny.
+constexpr int MEAN_COLS = 6;
+constexpr int MEAN_THREADS = 256;
+__global__ void __launch_bounds__(MEAN_THREADS) mean_kernel(LineParams P, WsArgs A, const double *scal,
+                                                            double *db_scratch) {
+  extern __shared__ double2 lds[];
+  const int n = P.g.n, W = P.g.W(), f = blockIdx.x;
+  cplx *col = reinterpret_cast<cplx *>(lds);
+  double *db = db_scratch ? db_scratch + (size_t)f * W * n : reinterpret_cast<double *>(lds + MEAN_COLS * n);
+  auto gU = [&](int slot) { return A.U + (size_t)slot * A.field; };
+  auto gR = [&](int slot) { return A.RK + (size_t)slot * A.field; };
+  // global source of each LDS column (nullptr: scratch, not loaded)
+  cplx *src[MEAN_COLS] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (f == 0) {         // u1 u2 u3 st1 buf1
+    src[1] = gU(1); src[3] = gR(0);
+  } else if (f == 1) {  // u2 u3 u4 u5 st2 buf1
+    src[2] = gU(3); src[4] = gR(1);
+  } else {              // u7 u8 u9 st3 buf1
+    src[1] = gU(7); src[3] = gR(2);
+  }
+  for (int q = 0; q < MEAN_COLS; ++q)
+    if (src[q])
+      for (int i = threadIdx.x; i < n; i += MEAN_THREADS) col[q * n + i] = src[q][(size_t)i * A.plane];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    LineParams Q = P;
+    Q.dt = scal[0];
+    const OwnBand Db{db, 1, W};
+    auto L = [&](int q) { return SV{col + q * n, 1}; };
+    double o4[4] = {0, 0, 0, 0};
+    if (f == 0) {
+      mean_u(Q, Db, L(0), L(1), L(2), L(3), L(4), o4);
+      if (A.mean_out) { A.mean_out[0] = o4[0]; A.mean_out[2] = o4[2]; }
+    } else if (f == 1) {
+      mean_w(Q, Db, true, L(0), L(1), L(2), L(3), L(4), L(5));
+    } else {
+      mean_t(Q, Db, L(0), L(1), L(2), L(3), L(4), o4);
+      if (A.mean_out) { A.mean_out[1] = o4[1]; A.mean_out[3] = o4[3]; }
+    }
+  }
+  __syncthreads();
+  // results: f 0 -> slot 0 (U), slot 1 (V = 0, written here because this
+  // block is the one that reads slot 1), RK 0; f 1 -> slots 2-4, RK 1;
+  // f 2 -> slots 6-8, RK 2
+  cplx *dst[MEAN_COLS] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (f == 0) {
+    dst[0] = gU(0); dst[3] = gR(0);
+  } else if (f == 1) {
+    dst[1] = gU(2); dst[2] = gU(3); dst[3] = gU(4); dst[4] = gR(1);
+  } else {
+    dst[0] = gU(6); dst[1] = gU(7); dst[2] = gU(8); dst[3] = gR(2);
+  }
+  for (int q = 0; q < MEAN_COLS; ++q)
+    if (dst[q])
+      for (int i = threadIdx.x; i < n; i += MEAN_THREADS) dst[q][(size_t)i * A.plane] = col[q * n + i];
+  if (f == 0)
+    for (int i = threadIdx.x; i < n; i += MEAN_THREADS) gU(1)[(size_t)i * A.plane] = mk(0.0, 0.0);
+}
 
+// Fused sweeps over all (kx,kz) != (0,0) systems, one thread per system.
 template <int BW, int SW>
 __global__ void __launch_bounds__(64) sweep_kernel(LineParams P, WsArgs A, double *dbuf2, size_t nsys,
                                                    const double *scal) {
   const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sys >= nsys) return;
-  SysCtx c;
-  if (!make_sys(A, P.g.n, sys, c)) return;
   LineParams Q = P;
   Q.dt = scal[0];
-  if (SW == SW_A1) sweep_A1<BW>(Q, A, c);
-  if (SW == SW_B1) sweep_B1<BW>(Q, A, c);
-  if (SW == SW_A) sweep_A<BW>(Q, A, c);
-  if (SW == SW_BT) sweep_Bt<BW>(Q, A, c);
-  if (SW == SW_CT) sweep_Ct<BW>(Q, A, c, dbuf2);
-  if (SW == SW_DT) sweep_Dt<BW>(Q, A, c, dbuf2);
-  if (SW == SW_BG) sweep_Bg<BW>(Q, A, c);
-  if (SW == SW_BV) sweep_Bv<BW>(Q, A, c);
-  if (SW == SW_CNU) sweep_Cnu<BW>(Q, A, c);
-  if (SW == SW_DNU) sweep_Dnu<BW>(Q, A, c);
-  if (SW == SW_E) {
-    cplx vlo = mk(0.0, 0.0), vhi = mk(0.0, 0.0);
-    if (A.opp_alpha != 0.0 && A.vd) {
-      vlo = cscale(A.vd[sys], -A.opp_alpha);
-      vhi = cscale(A.vd[A.plane + sys], -A.opp_alpha);
-    }
-    sweep_E<BW>(Q, A, c, vlo, vhi);
-  }
-  if (SW == SW_F) sweep_F<BW>(Q, A, c);
-  if (SW == SW_G) sweep_G(Q, A, c);
+  run_sweep<BW, SW>(Q, A, dbuf2, sys);
 }
 
 // opposition-control detection planes: vd[0][sys], vd[1][sys]

@@ -303,18 +303,23 @@ PB_HD cplx integrate(const LineParams &P, const SV &src) {
   return t;
 }
 
-// wave00.f90:41-685 with calculate_phi (704-714); out4 = phi_u, phi_t, tau_w, q_w
-PB_HD void wave00(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, SV u4, SV u5, SV u7, SV u8, SV u9,
-                  SV st1, SV st2, SV st3, SV buf1, double *out4) {
-  const int n = P.g.n, s = P.s, S = P.S, W = P.g.W(), m = P.g.bw;
-  const double dt = P.dt, nu = P.nu, pr = P.pr;
-  auto calc_phi = [](double bulk, cplx I) { return (bulk - 0.5 * I.re) / (0.5 * I.im); };
-  auto wall_d1 = [&](const SV &c) {
-    return (((P.D1(0, m) * c[0].re + P.D1(0, m + 1) * c[1].re) - P.D1(n - 1, m - 1) * c[n - 2].re) -
-            P.D1(n - 1, m) * c[n - 1].re);
-  };
-  (void)W;
-  // ---- U
+PB_HD double calc_phi(double bulk, cplx I) { return (bulk - 0.5 * I.re) / (0.5 * I.im); }  // wave00.f90:704-714
+// two D1 coefficients at each wall (wave00.f90:210-211, 621-622)
+PB_HD double wall_d1(const LineParams &P, const SV &c) {
+  const int n = P.g.n, m = P.g.bw;
+  return (((P.D1(0, m) * c[0].re + P.D1(0, m + 1) * c[1].re) - P.D1(n - 1, m - 1) * c[n - 2].re) -
+          P.D1(n - 1, m) * c[n - 1].re);
+}
+
+// The (0,0) system of wave00.f90 is three independent solves; each of the
+// three functions below touches only its own columns (and Db), so they may
+// run one after another on one scratch Db or side by side on three.
+//
+// Mean streamwise velocity (wave00.f90:58-257): u1 <- U, st1 updated,
+// u2/u3 left as scratch; out4[0] = phi_u, out4[2] = tau_w (last substep).
+PB_HD void mean_u(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, SV st1, SV buf1, double *out4) {
+  const int n = P.g.n, s = P.s, S = P.S;
+  const double dt = P.dt, nu = P.nu;
   helmholtz_lu(P, Db, 0.0, dt * P.coef(1, s) * nu, true);
   if (s == 1) {
     for (int i = 0; i < n; ++i) u1[i] = st1[i];
@@ -334,10 +339,10 @@ PB_HD void wave00(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, S
   for (int i = 1; i < n - 1; ++i) buf1[i] = mk(buf1[i].re + P.u_gal, 1.0);
   buf1[n - 1] = mk(0.0, 0.0);
   band_bs(P.g, Db, buf1);
-  double phi = calc_phi(P.bulk_u, integrate(P, buf1));
+  const double phi = calc_phi(P.bulk_u, integrate(P, buf1));
   for (int i = 0; i < n; ++i) buf1[i] = mk(buf1[i].re + phi * buf1[i].im, 0.0);
   out4[0] = phi;
-  if (s == S) out4[2] = 0.5 * nu * wall_d1(buf1);
+  if (s == S) out4[2] = 0.5 * nu * wall_d1(P, buf1);
   band_mv(P.g, P.D0, buf1, u1);
   for (int i = 0; i < n; ++i) u1[i] = mk(u1[i].re - P.u_gal, u1[i].im);
   if (s != S) {
@@ -347,7 +352,16 @@ PB_HD void wave00(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, S
   } else {
     for (int i = 0; i < n; ++i) st1[i] = u1[i];
   }
-  // ---- W
+}
+
+// Mean spanwise velocity (wave00.f90:260-443, phi forced to 0 at 396): u3 <- W,
+// u2 <- 0, st2 updated, u4/u5 left as scratch.  Its matrix is the one of
+// mean_u; factor = true forms it again (same values) for a solve that does
+// not follow mean_u on the same Db.
+PB_HD void mean_w(const LineParams &P, const OwnBand &Db, bool factor, SV u2, SV u3, SV u4, SV u5, SV st2, SV buf1) {
+  const int n = P.g.n, s = P.s, S = P.S;
+  const double dt = P.dt, nu = P.nu;
+  if (factor) helmholtz_lu(P, Db, 0.0, dt * P.coef(1, s) * nu, true);
   if (s == 1) {
     for (int i = 0; i < n; ++i) u2[i] = st2[i];
     band_bs(P.g, P.D0LU, u2);
@@ -366,7 +380,7 @@ PB_HD void wave00(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, S
   for (int i = 1; i < n - 1; ++i) buf1[i] = mk(buf1[i].re, 1.0);
   buf1[n - 1] = mk(0.0, 0.0);
   band_bs(P.g, Db, buf1);
-  phi = calc_phi(P.bulk_w, integrate(P, buf1));
+  double phi = calc_phi(P.bulk_w, integrate(P, buf1));
   phi = 0.0;  // wave00.f90:396
   for (int i = 0; i < n; ++i) buf1[i] = mk(buf1[i].re + phi * buf1[i].im, 0.0);
   band_mv(P.g, P.D0, buf1, u3);
@@ -378,7 +392,13 @@ PB_HD void wave00(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, S
     for (int i = 0; i < n; ++i) st2[i] = u3[i];
   }
   for (int i = 0; i < n; ++i) u2[i] = mk(0.0, 0.0);
-  // ---- Theta
+}
+
+// Mean scalar (wave00.f90:473-667): u7 <- Theta, st3 updated, u8/u9 left as
+// scratch; out4[1] = phi_t, out4[3] = q_w (last substep).
+PB_HD void mean_t(const LineParams &P, const OwnBand &Db, SV u7, SV u8, SV u9, SV st3, SV buf1, double *out4) {
+  const int n = P.g.n, s = P.s, S = P.S;
+  const double dt = P.dt, nu = P.nu, pr = P.pr;
   helmholtz_lu(P, Db, 0.0, dt * P.coef(1, s) * nu / pr, true);
   if (s == 1) {
     for (int i = 0; i < n; ++i) u7[i] = st3[i];
@@ -398,10 +418,10 @@ PB_HD void wave00(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, S
   for (int i = 1; i < n - 1; ++i) buf1[i] = mk(buf1[i].re, 1.0);
   buf1[n - 1] = mk(0.0, 0.0);
   band_bs(P.g, Db, buf1);
-  phi = calc_phi(P.bulk_t, integrate(P, buf1));
+  const double phi = calc_phi(P.bulk_t, integrate(P, buf1));
   for (int i = 0; i < n; ++i) buf1[i] = mk(buf1[i].re + phi * buf1[i].im, 0.0);
   out4[1] = phi;
-  if (s == S) out4[3] = 0.5 * nu / pr * wall_d1(buf1);
+  if (s == S) out4[3] = 0.5 * nu / pr * wall_d1(P, buf1);
   band_mv(P.g, P.D0, buf1, u7);
   if (s != S) {
     band_mv(P.g, P.D2, buf1, st3);
@@ -410,6 +430,14 @@ PB_HD void wave00(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, S
   } else {
     for (int i = 0; i < n; ++i) st3[i] = u7[i];
   }
+}
+
+// wave00.f90:41-685 with calculate_phi (704-714); out4 = phi_u, phi_t, tau_w, q_w
+PB_HD void wave00(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, SV u4, SV u5, SV u7, SV u8, SV u9,
+                  SV st1, SV st2, SV st3, SV buf1, double *out4) {
+  mean_u(P, Db, u1, u2, u3, st1, buf1, out4);
+  mean_w(P, Db, false, u2, u3, u4, u5, st2, buf1);
+  mean_t(P, Db, u7, u8, u9, st3, buf1, out4);
 }
 
 // Detection-plane value of a collocation column: c = D0^{-1} v (in buf),

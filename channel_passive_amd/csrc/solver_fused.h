@@ -29,11 +29,14 @@
 // Slot use per system (0-based product slots; s = substep):
 //   after A : 0 yB, 1 P2, 2 yE, 3 P4, 4 yF, 6 u1a, 7 y8, 8 u7a
 //   after B : 0 rhs_phi, 2 rhs_g, 7 rhs_theta, RK = zeta*h (s < S)
-//   after C : 0 y_phip, 1 y_phih, 2 y_g, 7 y_theta; U factors in dbuf, dbuf2
-//   after D : 0 phi_p, 1 phi_h, 2 g, 3 D2 phi_h, 6 theta; RK updated
-//   after E : 0 y_vp, 5 y_vh; U factor in dbuf
-//   after F : 0 v_p, 5 v_h', 7 v_p', 8 D0 c_h
+//   after C : 0 y_phip, 2 y_g, 7 y_theta, R0 y_phih; U factors in dbuf, dbuf2
+//   after D : 0 phi_p, 2 g, 6 theta, R1 phi_h, R2 D2 phi_h; RK updated
+//   after E : 0 y_vp, R3 y_vh; U factor in dbuf
+//   after F : 0 v_p, 7 v_p', R3 v_h', R0 D0 c_h
 //   after G : 0 u, 1 v, 2 w, 6 theta; RK2 = store_phi
+// R0..R3 are real columns (the reference's (1+i)-scaled homogeneous
+// solutions are (x, x): only x is stored) in the memory of the two scratch
+// columns, which hold D2 LU(D0)^-1 of the stores on s == 1 until B.
 // B, C and D are split by right-hand-side group (theta | g | phi,v): the
 // groups read disjoint data, so the split costs no traffic and keeps the
 // register windows of each kernel small; the theta chain can run beside
@@ -74,12 +77,24 @@ struct RWin {
   }
 };
 
+// strided real column
+struct RV {
+  double *p;
+  size_t s;
+  PB_HD double &operator[](int i) const { return p[(size_t)i * s]; }
+};
+
 // Per-system context shared by the sweeps.
 struct SysCtx {
   size_t sys;
   int n;
   double kx, kz, k2;
   PB_HD SV col(const WsArgs &A, int slot) const { return SV{A.U + (size_t)slot * A.field + sys, A.plane}; }
+  // real column k (0..3) in the memory of the two scratch columns; only
+  // the sweeps C_nu to G use it (the complex scratch view is dead by then)
+  PB_HD RV rcol(const WsArgs &A, int k) const {
+    return RV{reinterpret_cast<double *>(A.buf) + (size_t)k * A.field + sys, A.plane};
+  }
   PB_HD SV rk(const WsArgs &A, int slot) const { return SV{A.RK + (size_t)slot * A.field + sys, A.plane}; }
   PB_HD SV buf(const WsArgs &A, int slot) const { return SV{A.buf + (size_t)slot * A.field + sys, A.plane}; }
 };
@@ -95,15 +110,24 @@ PB_HD bool make_sys(const WsArgs &A, int n, size_t sys, SysCtx &c) {
   return !(k == 0 && kzi == 0);
 }
 
-// forward substitution of one row: w.w[q] = y(r-1-q); farthest first
+// Forward windows hold the last bw results, w.w[q] = y(r-1-q), q < bw.
+// Only the last bw rows reach further back (their packed entries start at
+// column n-1-2bw); those older values are read back from the column the
+// sweep stores y in.
+template <int BW, class W, class Col>
+PB_HD auto win_or(const W &w, int q, const Col &y, int p) -> decltype(w.w[0]) {
+  return q < BW ? w.w[q < BW ? q : 0] : y[p];
+}
+
+// forward substitution of one row, farthest first
 template <int BW, class M>
-PB_HD cplx fwd_row(const BandGeom &g, const M &L, int r, const CWin<2 * BW> &w, cplx b) {
+PB_HD cplx fwd_row(const BandGeom &g, const M &L, int r, const CWin<BW> &w, const SV &y, cplx b) {
   const int f = g.first(r);
   cplx v = b;
 #pragma unroll
   for (int q = 2 * BW - 1; q >= 0; --q) {
     const int p = r - 1 - q;
-    if (p >= f) v = csub(v, cscale(w.w[q], L(r, g.slot(r, p))));
+    if (p >= f) v = csub(v, cscale(win_or<BW>(w, q, y, p), L(r, g.slot(r, p))));
   }
   return cdivr(v, L(r, BW));
 }
@@ -229,24 +253,91 @@ PB_HD void crout_row(const BandGeom &g, const M &A, const OwnU &Uo, int r, doubl
   }
 }
 
+// Crout row of an interior row r in [2bw, n-bw): first(r) = r-bw,
+// last(r) = r+bw and the bw rows above are interior as well, so every U(p,c)
+// the row needs is in the register window uw[qp-1][d-1] = U(r-qp, r-qp+d),
+// qp, d = 1..bw.  The terms and their order are those of crout_row (the
+// terms left out are exactly the ones its conditions exclude).  The new U
+// row is stored and shifted into the window.
+template <int BW, class M>
+PB_HD void crout_row_mid(const M &A, const OwnU &Uo, int r, double (&Lrow)[2 * BW + 1], double (&uw)[BW][BW]) {
+#pragma unroll
+  for (int qd = 2 * BW; qd > BW; --qd) Lrow[qd] = 0.0;
+#pragma unroll
+  for (int qd = BW; qd >= 0; --qd) {
+    double v = A(r, BW - qd);
+#pragma unroll
+    for (int qp = qd + 1; qp <= BW; ++qp) v = v - Lrow[qp] * uw[qp - 1][qp - qd - 1];
+    Lrow[qd] = v;
+  }
+  const double inv = 1.0 / Lrow[0];
+  double un[BW];
+#pragma unroll
+  for (int qc = 1; qc <= BW; ++qc) {
+    double v = A(r, BW + qc);
+#pragma unroll
+    for (int qp = 1; qp + qc <= BW; ++qp) v = v - Lrow[qp] * uw[qp - 1][qp + qc - 1];
+    un[qc - 1] = inv * v;
+    Uo.set(r, BW + qc, un[qc - 1]);
+  }
+#pragma unroll
+  for (int qp = BW - 1; qp > 0; --qp)
+#pragma unroll
+    for (int d = 0; d < BW; ++d) uw[qp][d] = uw[qp - 1][d];
+#pragma unroll
+  for (int d = 0; d < BW; ++d) uw[0][d] = un[d];
+}
+
+// Crout LU of a whole system, row by row, calling row(r, Lrow) after each
+// row: the first 2bw and the last bw rows (whose packed entries reach
+// further) read U back from Uo, the rows between keep it in registers.
+template <int BW, class M, class F>
+PB_HD void crout_sweep(const BandGeom &g, const M &A, const OwnU &Uo, F &&row) {
+  const int n = g.n;
+  const bool mid = n > 3 * BW;
+  const int r1 = mid ? 2 * BW : n, r2 = mid ? n - BW : n;
+  double L[2 * BW + 1];
+  for (int r = 0; r < r1; ++r) {
+    crout_row<BW>(g, A, Uo, r, L);
+    row(r, L);
+  }
+  if (mid) {
+    double uw[BW][BW];
+#pragma unroll
+    for (int qp = 1; qp <= BW; ++qp)
+#pragma unroll
+      for (int d = 1; d <= BW; ++d) uw[qp - 1][d - 1] = Uo(r1 - qp, BW + d);
+    for (int r = r1; r < r2; ++r) {
+      crout_row_mid<BW>(A, Uo, r, L, uw);
+      row(r, L);
+    }
+  }
+  for (int r = r2; r < n; ++r) {
+    crout_row<BW>(g, A, Uo, r, L);
+    row(r, L);
+  }
+}
+
 // forward substitution with the current row's L (Lrow[q] = L(r, r-q))
 template <int BW>
-PB_HD cplx fwd_lrow(const BandGeom &g, int r, const double (&Lrow)[2 * BW + 1], const CWin<2 * BW> &w, cplx b) {
+PB_HD cplx fwd_lrow(const BandGeom &g, int r, const double (&Lrow)[2 * BW + 1], const CWin<BW> &w, const SV &y,
+                    cplx b) {
   const int f = g.first(r);
   cplx v = b;
 #pragma unroll
   for (int q = 2 * BW; q >= 1; --q) {
-    if (r - q >= f) v = csub(v, cscale(w.w[q - 1], Lrow[q]));
+    if (r - q >= f) v = csub(v, cscale(win_or<BW>(w, q - 1, y, r - q), Lrow[q]));
   }
   return cdivr(v, Lrow[0]);
 }
 template <int BW>
-PB_HD double fwd_lrow_r(const BandGeom &g, int r, const double (&Lrow)[2 * BW + 1], const RWin<2 * BW> &w, double b) {
+PB_HD double fwd_lrow_r(const BandGeom &g, int r, const double (&Lrow)[2 * BW + 1], const RWin<BW> &w, const RV &y,
+                        double b) {
   const int f = g.first(r);
   double v = b;
 #pragma unroll
   for (int q = 2 * BW; q >= 1; --q) {
-    if (r - q >= f) v = v - w.w[q - 1] * Lrow[q];
+    if (r - q >= f) v = v - win_or<BW>(w, q - 1, y, r - q) * Lrow[q];
   }
   return v / Lrow[0];
 }
@@ -258,12 +349,12 @@ PB_HD void sweep_A1(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   SV r1 = c.rk(A, 0), r2 = c.rk(A, 1), r3 = c.rk(A, 2);
   SV o1 = c.buf(A, 0), o2 = c.buf(A, 1), o3 = c.col(A, 5);
-  CWin<2 * BW> w1, w2, w3;
+  CWin<BW> w1, w2, w3;
   w1.clear(); w2.clear(); w3.clear();
   for (int r = 0; r < c.n; ++r) {
-    const cplx y1 = fwd_row<BW>(g, P.D0LU, r, w1, r1[r]);
-    const cplx y2 = fwd_row<BW>(g, P.D0LU, r, w2, r2[r]);
-    const cplx y3 = fwd_row<BW>(g, P.D0LU, r, w3, r3[r]);
+    const cplx y1 = fwd_row<BW>(g, P.D0LU, r, w1, o1, r1[r]);
+    const cplx y2 = fwd_row<BW>(g, P.D0LU, r, w2, o2, r2[r]);
+    const cplx y3 = fwd_row<BW>(g, P.D0LU, r, w3, o3, r3[r]);
     o1[r] = y1; o2[r] = y2; o3[r] = y3;
     w1.push(y1); w2.push(y2); w3.push(y3);
   }
@@ -298,7 +389,7 @@ PB_HD void sweep_A(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const cplx ikx = iunit(kx), ikz = iunit(kz);
   SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s4 = c.col(A, 4), s6 = c.col(A, 6),
      s7 = c.col(A, 7), s8 = c.col(A, 8);
-  CWin<2 * BW> wB, wE, wF, w8;
+  CWin<BW> wB, wE, wF, w8;
   wB.clear(); wE.clear(); wF.clear(); w8.clear();
   for (int r = 0; r < c.n; ++r) {
     const cplx p1 = s0[r], p2 = s1[r], p3 = s2[r], p4 = s3[r], p5 = s4[r], p7 = s6[r], p8 = s7[r], p9 = s8[r];
@@ -307,10 +398,10 @@ PB_HD void sweep_A(const LineParams &P, const WsArgs &A, const SysCtx &c) {
     const cplx u3b = csub(cmul(ikx, p4), cmul(ikz, p2));
     const cplx u4b = cadd(cmul(ikz, p4), cmul(ikx, p2));
     const cplx u7a = cadd(cmul(ikx, p7), cmul(ikz, p9));
-    const cplx yB = fwd_row<BW>(g, P.D0LU, r, wB, u3a);
-    const cplx yE = fwd_row<BW>(g, P.D0LU, r, wE, u3b);
-    const cplx yF = fwd_row<BW>(g, P.D0LU, r, wF, u4b);
-    const cplx y8 = fwd_row<BW>(g, P.D0LU, r, w8, p8);
+    const cplx yB = fwd_row<BW>(g, P.D0LU, r, wB, s0, u3a);
+    const cplx yE = fwd_row<BW>(g, P.D0LU, r, wE, s2, u3b);
+    const cplx yF = fwd_row<BW>(g, P.D0LU, r, wF, s4, u4b);
+    const cplx y8 = fwd_row<BW>(g, P.D0LU, r, w8, s7, p8);
     s0[r] = yB; s2[r] = yE; s4[r] = yF; s7[r] = y8; s6[r] = u1a; s8[r] = u7a;
     wB.push(yB); wE.push(yE); wF.push(yF); w8.push(y8);
   }
@@ -417,21 +508,20 @@ PB_HD void sweep_Cnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const double be = P.dt * P.coef(1, P.s) * P.nu;
   const HelmholtzA H{P.D0, P.D2, 1.0 + be * c.k2, be, n, BW};
   const OwnU U{A.dbuf + c.sys, A.plane, 2 * BW + 1};
-  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2);
-  CWin<2 * BW> wg, wp;
-  RWin<2 * BW> wh;
+  SV s0 = c.col(A, 0), s2 = c.col(A, 2);
+  RV h1 = c.rcol(A, 0);
+  CWin<BW> wg, wp;
+  RWin<BW> wh;
   wg.clear(); wp.clear(); wh.clear();
-  double L[2 * BW + 1];
   double d00 = 0.0;
-  for (int r = 0; r < n; ++r) {
-    crout_row<BW>(g, H, U, r, L);
+  crout_sweep<BW>(g, H, U, [&](int r, const double(&L)[2 * BW + 1]) __attribute__((always_inline)) {
     if (r == 0) d00 = L[0];
-    const cplx yg = fwd_lrow<BW>(g, r, L, wg, s2[r]);
-    const cplx yp = fwd_lrow<BW>(g, r, L, wp, s0[r]);
-    const double yh = fwd_lrow_r<BW>(g, r, L, wh, r == 0 ? d00 : 0.0);
-    s2[r] = yg; s0[r] = yp; s1[r] = mk(yh, yh);
+    const cplx yg = fwd_lrow<BW>(g, r, L, wg, s2, s2[r]);
+    const cplx yp = fwd_lrow<BW>(g, r, L, wp, s0, s0[r]);
+    const double yh = fwd_lrow_r<BW>(g, r, L, wh, h1, r == 0 ? d00 : 0.0);
+    s2[r] = yg; s0[r] = yp; h1[r] = yh;
     wg.push(yg); wp.push(yp); wh.push(yh);
-  }
+  });
 }
 
 template <int BW>
@@ -442,15 +532,13 @@ PB_HD void sweep_Ct(const LineParams &P, const WsArgs &A, const SysCtx &c, doubl
   const HelmholtzA H{P.D0, P.D2, 1.0 + be * c.k2, be, n, BW};
   const OwnU U{dbuf2 + c.sys, A.plane, 2 * BW + 1};
   SV s7 = c.col(A, 7);
-  CWin<2 * BW> wt;
+  CWin<BW> wt;
   wt.clear();
-  double L[2 * BW + 1];
-  for (int r = 0; r < n; ++r) {
-    crout_row<BW>(g, H, U, r, L);
-    const cplx yt = fwd_lrow<BW>(g, r, L, wt, s7[r]);
+  crout_sweep<BW>(g, H, U, [&](int r, const double(&L)[2 * BW + 1]) __attribute__((always_inline)) {
+    const cplx yt = fwd_lrow<BW>(g, r, L, wt, s7, s7[r]);
     s7[r] = yt;
     wt.push(yt);
-  }
+  });
 }
 
 // D, split by matrix: back substitution and the D0/D2 products
@@ -471,7 +559,8 @@ PB_HD void sweep_Dnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
     al_p = dt * nu * P.coef(0, s + 1);
     du_p = 1.0 - al_p * k2;
   }
-  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3);
+  SV s0 = c.col(A, 0), s2 = c.col(A, 2);
+  RV y1 = c.rcol(A, 0), h1 = c.rcol(A, 1), h3 = c.rcol(A, 2);
   SV r1 = c.rk(A, 0), r2 = c.rk(A, 1);
   CWin<2 * BW + 1> wg, wp;
   RWin<2 * BW + 1> wh;
@@ -479,7 +568,7 @@ PB_HD void sweep_Dnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   for (int r = n - 1; r >= 0; --r) {
     wg.push(bwd_row<BW>(g, U, r, wg, s2[r]));
     wp.push(bwd_row<BW>(g, U, r, wp, s0[r]));
-    wh.push(bwd_row_r<BW>(g, U, r, wh, s1[r].re));
+    wh.push(bwd_row_r<BW>(g, U, r, wh, y1[r]));
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx gv = mv_row<BW>(g, P.D0, m, r, wg);
       const cplx d2g = mv_row<BW>(g, P.D2, m, r, wg);
@@ -494,7 +583,7 @@ PB_HD void sweep_Dnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
         r1[m] = gv;
         r2[m] = pv;
       }
-      s2[m] = gv; s0[m] = pv; s1[m] = mk(hv, hv); s3[m] = mk(d2h, d2h);
+      s2[m] = gv; s0[m] = pv; h1[m] = hv; h3[m] = d2h;
     });
   }
 }
@@ -527,27 +616,35 @@ PB_HD void sweep_Dt(const LineParams &P, const WsArgs &A, const SysCtx &c, doubl
 }
 
 // E: LU of the Poisson operator fused with the forward substitution of
-// v_p (wall values vlo/vhi) and v_h (solve_v_f.f90:205-262)
+// v_p (wall values -alpha v(y_d) with opposition control, else 0) and v_h
+// (solve_v_f.f90:205-262)
 template <int BW>
-PB_HD void sweep_E(const LineParams &P, const WsArgs &A, const SysCtx &c, cplx vlo, cplx vhi) {
+PB_HD void sweep_E(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const PoissonA Q{P.D0, P.D2, c.k2, n, BW};
   const OwnU Uq{A.dbuf + c.sys, A.plane, 2 * BW + 1};
-  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s5 = c.col(A, 5);
-  CWin<2 * BW> wp;
-  RWin<2 * BW> wh;
+  const bool opp = A.opp_alpha != 0.0 && A.vd;
+  SV s0 = c.col(A, 0);
+  RV h1 = c.rcol(A, 1), h5 = c.rcol(A, 3);
+  CWin<BW> wp;
+  RWin<BW> wh;
   wp.clear(); wh.clear();
-  double L[2 * BW + 1];
-  for (int r = 0; r < n; ++r) {
-    crout_row<BW>(g, Q, Uq, r, L);
-    const cplx bp = (r == 0) ? vlo : (r == n - 1 ? vhi : s0[r]);
-    const double bh = (r == 0 || r == n - 1) ? 0.0 : s1[r].re;
-    const cplx yp = fwd_lrow<BW>(g, r, L, wp, bp);
-    const double yh = fwd_lrow_r<BW>(g, r, L, wh, bh);
-    s0[r] = yp; s5[r] = mk(yh, yh);
+  crout_sweep<BW>(g, Q, Uq, [&](int r, const double(&L)[2 * BW + 1]) __attribute__((always_inline)) {
+    cplx bp = s0[r];
+    double bh = h1[r];
+    if (r == 0 || r == n - 1) {
+      // wall rows: the value is read here rather than held in a variable
+      // chosen by r, which the compiler would turn into a private-memory
+      // address select
+      bp = opp ? cscale(A.vd[(r == 0 ? 0 : A.plane) + c.sys], -A.opp_alpha) : mk(0.0, 0.0);
+      bh = 0.0;
+    }
+    const cplx yp = fwd_lrow<BW>(g, r, L, wp, s0, bp);
+    const double yh = fwd_lrow_r<BW>(g, r, L, wh, h5, bh);
+    s0[r] = yp; h5[r] = yh;
     wp.push(yp); wh.push(yh);
-  }
+  });
 }
 
 // F: back substitution with U of the Poisson operator and the D0/D1
@@ -558,19 +655,20 @@ PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const OwnU Uq{A.dbuf + c.sys, A.plane, 2 * BW + 1};
-  SV s0 = c.col(A, 0), s5 = c.col(A, 5), s7 = c.col(A, 7), s8 = c.col(A, 8);
+  SV s0 = c.col(A, 0), s7 = c.col(A, 7);
+  RV h5 = c.rcol(A, 3), h8 = c.rcol(A, 0);
   CWin<2 * BW + 1> wp;
   RWin<2 * BW + 1> wh;
   wp.clear(); wh.clear();
   for (int r = n - 1; r >= 0; --r) {
     wp.push(bwd_row<BW>(g, Uq, r, wp, s0[r]));
-    wh.push(bwd_row_r<BW>(g, Uq, r, wh, s5[r].re));
+    wh.push(bwd_row_r<BW>(g, Uq, r, wh, h5[r]));
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx vp = mv_row<BW>(g, P.D0, m, r, wp);
       const cplx dvp = mv_row<BW>(g, P.D1, m, r, wp);
       const double dvh = mv_row_r<BW>(g, P.D1, m, r, wh);
       const double d0h = mv_row_r<BW>(g, P.D0, m, r, wh);
-      s0[m] = vp; s7[m] = dvp; s5[m] = mk(dvh, dvh); s8[m] = mk(d0h, d0h);
+      s0[m] = vp; s7[m] = dvp; h5[m] = dvh; h8[m] = d0h;
     });
   }
 }
@@ -580,10 +678,12 @@ PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
 PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const int n = c.n, s = P.s, S = P.S;
   const double dt = P.dt, nu = P.nu, k2 = c.k2;
-  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s5 = c.col(A, 5), s7 = c.col(A, 7),
-     s8 = c.col(A, 8);
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s7 = c.col(A, 7);
+  RV h1 = c.rcol(A, 1), h3 = c.rcol(A, 2), h5 = c.rcol(A, 3), h8 = c.rcol(A, 0);
   SV r2 = c.rk(A, 1);
-  const cplx h0 = s5[0], hn = s5[n - 1], d0 = s7[0], dn = s7[n - 1];
+  // the real columns stand for the complex (x, x) of the reference
+  auto cx = [](double x) { return mk(x, x); };
+  const cplx h0 = cx(h5[0]), hn = cx(h5[n - 1]), d0 = s7[0], dn = s7[n - 1];
   const cplx den = csub(cmul(h0, h0), cmul(hn, hn));
   const cplx c1 = cdiv(csub(cmul(hn, dn), cmul(h0, d0)), den);
   const cplx c2 = cdiv(csub(cmul(h0, dn), cmul(hn, d0)), den);
@@ -597,9 +697,9 @@ PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const cplx kxx = cdivr(cmul(ci, mk(c.kx, 0.0)), k2), kzz = cdivr(cmul(ci, mk(c.kz, 0.0)), k2);
   for (int i = 0; i <= (n - 1) / 2; ++i) {
     const int j = n - 1 - i;
-    const cplx dvp_i = s7[i], dvp_j = s7[j], dvh_i = s5[i], dvh_j = s5[j];
-    const cplx sp_i = r2[i], sp_j = r2[j], ph_i = s1[i], ph_j = s1[j], d2h_i = s3[i], d2h_j = s3[j];
-    const cplx vp_i = s0[i], vp_j = s0[j], d0h_i = s8[i], d0h_j = s8[j], g_i = s2[i], g_j = s2[j];
+    const cplx dvp_i = s7[i], dvp_j = s7[j], dvh_i = cx(h5[i]), dvh_j = cx(h5[j]);
+    const cplx sp_i = r2[i], sp_j = r2[j], ph_i = cx(h1[i]), ph_j = cx(h1[j]), d2h_i = cx(h3[i]), d2h_j = cx(h3[j]);
+    const cplx vp_i = s0[i], vp_j = s0[j], d0h_i = cx(h8[i]), d0h_j = cx(h8[j]), g_i = s2[i], g_j = s2[j];
     auto one = [&](int row, cplx dvp, cplx dvh, cplx dvh_m, cplx sp, cplx ph, cplx ph_m, cplx d2h, cplx d2h_m,
                    cplx vp, cplx d0h, cplx d0h_m, cplx gg) {
       const cplx f = csub(cadd(dvp, cmul(c1, dvh)), cmul(c2, dvh_m));
@@ -624,31 +724,37 @@ PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   }
 }
 
-// Whole substep of one (kx,kz) != (0,0) system.
-template <int BW>
-PB_HD void fused_line(const LineParams &P, const WsArgs &A, double *dbuf2, size_t sys) {
+// One sweep of one (kx,kz) != (0,0) system.  A substep runs, for all
+// systems, A1 and B1 (s == 1 only), A, B_g, then the theta chain B_t, C_t,
+// D_t and, in any order relative to it, B_phi, C_nu, D_nu; then E, F, G.
+enum SweepId { SW_A1, SW_B1, SW_A, SW_BT, SW_CT, SW_DT, SW_BG, SW_BV, SW_CNU, SW_DNU, SW_E, SW_F, SW_G, SW_COUNT };
+
+template <int BW, int SW>
+PB_HD void run_sweep(const LineParams &P, const WsArgs &A, double *dbuf2, size_t sys) {
   SysCtx c;
   if (!make_sys(A, P.g.n, sys, c)) return;
-  if (P.s == 1) {
-    sweep_A1<BW>(P, A, c);
-    sweep_B1<BW>(P, A, c);
-  }
-  sweep_A<BW>(P, A, c);
-  sweep_Bg<BW>(P, A, c);
-  sweep_Bt<BW>(P, A, c);
-  sweep_Ct<BW>(P, A, c, dbuf2);
-  sweep_Dt<BW>(P, A, c, dbuf2);
-  sweep_Bv<BW>(P, A, c);
-  sweep_Cnu<BW>(P, A, c);
-  sweep_Dnu<BW>(P, A, c);
-  cplx vlo = mk(0.0, 0.0), vhi = mk(0.0, 0.0);
-  if (A.opp_alpha != 0.0 && A.vd) {
-    vlo = cscale(A.vd[sys], -A.opp_alpha);
-    vhi = cscale(A.vd[A.plane + sys], -A.opp_alpha);
-  }
-  sweep_E<BW>(P, A, c, vlo, vhi);
-  sweep_F<BW>(P, A, c);
-  sweep_G(P, A, c);
+  if (SW == SW_A1) sweep_A1<BW>(P, A, c);
+  if (SW == SW_B1) sweep_B1<BW>(P, A, c);
+  if (SW == SW_A) sweep_A<BW>(P, A, c);
+  if (SW == SW_BT) sweep_Bt<BW>(P, A, c);
+  if (SW == SW_CT) sweep_Ct<BW>(P, A, c, dbuf2);
+  if (SW == SW_DT) sweep_Dt<BW>(P, A, c, dbuf2);
+  if (SW == SW_BG) sweep_Bg<BW>(P, A, c);
+  if (SW == SW_BV) sweep_Bv<BW>(P, A, c);
+  if (SW == SW_CNU) sweep_Cnu<BW>(P, A, c);
+  if (SW == SW_DNU) sweep_Dnu<BW>(P, A, c);
+  if (SW == SW_E) sweep_E<BW>(P, A, c);
+  if (SW == SW_F) sweep_F<BW>(P, A, c);
+  if (SW == SW_G) sweep_G(P, A, c);
+}
+
+// the sweeps of substep s in a serial order valid for the stream schedule
+inline int sweep_sequence(int s, int *ids) {
+  int k = 0;
+  if (s == 1) { ids[k++] = SW_A1; ids[k++] = SW_B1; }
+  const int rest[] = {SW_A, SW_BG, SW_BT, SW_CT, SW_DT, SW_BV, SW_CNU, SW_DNU, SW_E, SW_F, SW_G};
+  for (int q : rest) ids[k++] = q;
+  return k;
 }
 
 }  // namespace pb

@@ -82,6 +82,25 @@ int emu_wavespace(const pb_config *cfg, int s, double dt, double *U, double *RK,
 }
 
 #include "../../channel_passive_amd/csrc/solver_fused.h"
+template <int BW>
+static void run_id(int id, const pb::LineParams &P, const pb::WsArgs &A, double *dbuf2, size_t sys) {
+  switch (id) {
+    case pb::SW_A1: pb::run_sweep<BW, pb::SW_A1>(P, A, dbuf2, sys); break;
+    case pb::SW_B1: pb::run_sweep<BW, pb::SW_B1>(P, A, dbuf2, sys); break;
+    case pb::SW_A: pb::run_sweep<BW, pb::SW_A>(P, A, dbuf2, sys); break;
+    case pb::SW_BT: pb::run_sweep<BW, pb::SW_BT>(P, A, dbuf2, sys); break;
+    case pb::SW_CT: pb::run_sweep<BW, pb::SW_CT>(P, A, dbuf2, sys); break;
+    case pb::SW_DT: pb::run_sweep<BW, pb::SW_DT>(P, A, dbuf2, sys); break;
+    case pb::SW_BG: pb::run_sweep<BW, pb::SW_BG>(P, A, dbuf2, sys); break;
+    case pb::SW_BV: pb::run_sweep<BW, pb::SW_BV>(P, A, dbuf2, sys); break;
+    case pb::SW_CNU: pb::run_sweep<BW, pb::SW_CNU>(P, A, dbuf2, sys); break;
+    case pb::SW_DNU: pb::run_sweep<BW, pb::SW_DNU>(P, A, dbuf2, sys); break;
+    case pb::SW_E: pb::run_sweep<BW, pb::SW_E>(P, A, dbuf2, sys); break;
+    case pb::SW_F: pb::run_sweep<BW, pb::SW_F>(P, A, dbuf2, sys); break;
+    case pb::SW_G: pb::run_sweep<BW, pb::SW_G>(P, A, dbuf2, sys); break;
+  }
+}
+
 extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, double *U, double *RK, const double *vd,
                                    double *mean4) {
   Setup S;
@@ -95,14 +114,16 @@ extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, doubl
   A.U = (pb::cplx *)U; A.RK = (pb::cplx *)RK; A.buf = buf.data(); A.dbuf = dbuf.data();
   A.vd = (const pb::cplx *)vd; A.plane = plane; A.field = field; A.nxh = nxh; A.nz = nz; A.nh = nz / 2;
   A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha; A.mean_out = mean4;
+  if (S.bw < 3 || S.bw > 5) return -2;
   pb::wavespace_line(P, A, 0);  // (0,0): mean modes
-  for (size_t sys = 1; sys < plane; ++sys) {
-    switch (S.bw) {
-      case 3: pb::fused_line<3>(P, A, dbuf2.data(), sys); break;
-      case 4: pb::fused_line<4>(P, A, dbuf2.data(), sys); break;
-      case 5: pb::fused_line<5>(P, A, dbuf2.data(), sys); break;
-      default: return -2;
+  // sweep-major, as the device runs them: each sweep over all systems
+  int ids[pb::SW_COUNT];
+  const int nid = pb::sweep_sequence(s, ids);
+  for (int q = 0; q < nid; ++q)
+    for (size_t sys = 1; sys < plane; ++sys) {
+      if (S.bw == 3) run_id<3>(ids[q], P, A, dbuf2.data(), sys);
+      if (S.bw == 4) run_id<4>(ids[q], P, A, dbuf2.data(), sys);
+      if (S.bw == 5) run_id<5>(ids[q], P, A, dbuf2.data(), sys);
     }
-  }
   return 0;
 }
