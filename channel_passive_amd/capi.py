@@ -51,6 +51,8 @@ SIGNATURES = {
     "pb_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "pb_timing_read": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "pb_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(PbSubstepOut)]),
+    "pb_substep_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(PbSubstepOut), _dp,
+                                        _dp]),
     "pb_step": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(PbSubstepOut)]),
     "pb_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "pb_wavespace_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp,
@@ -147,6 +149,17 @@ class Channel:
         _check(load().pb_substep(self.h, s, ctypes.byref(d), ctypes.byref(o)))
         return o.as_dict()
 
+    def substep_stats(self, s: int, dt: float):
+        """Substep s with the statistics-time pressure (on s == 1): returns
+        (out, p (nz, nx/2, ny) complex128, <vv> (ny,))."""
+        d = ctypes.c_double(dt)
+        o = PbSubstepOut()
+        p = np.zeros(self.shape(1)[1:], np.complex128)
+        vv = np.zeros(self.cfg.ny)
+        _check(load().pb_substep_stats(self.h, s, ctypes.byref(d), ctypes.byref(o), _ptr(p),
+                                       vv.ctypes.data_as(_dp)))
+        return o.as_dict(), p, vv
+
     def step(self, dt: float):
         d = ctypes.c_double(dt)
         o = (PbSubstepOut * self.cfg.rk_step)()
@@ -156,10 +169,11 @@ class Channel:
     def sync(self):
         _check(load().pb_sync(self.h))
 
-    def nonlinear_host(self, S: np.ndarray):
+    def nonlinear_host(self, S: np.ndarray, want_vv: bool = False):
         P = np.zeros(self.shape(10), np.complex128)
         vm = ctypes.c_double(0)
-        _check(load().pb_nonlinear_host(self.h, _ptr(np.ascontiguousarray(S)), _ptr(P), 0, ctypes.byref(vm)))
+        _check(load().pb_nonlinear_host(self.h, _ptr(np.ascontiguousarray(S)), _ptr(P), int(want_vv),
+                                        ctypes.byref(vm)))
         return P, vm.value
 
     def wavespace_host(self, s: int, dt: float, P: np.ndarray, RK: np.ndarray, v_old: np.ndarray | None = None):

@@ -80,6 +80,8 @@ struct pb_ctx {
   // device memory
   double2 *U = nullptr, *RK = nullptr, *pad = nullptr, *buf = nullptr, *vd = nullptr;
   double *dbuf = nullptr, *dbuf2 = nullptr, *vpart = nullptr, *scal = nullptr;
+  double2 *ppe_bc = nullptr;  // statistics time: pressure wall values [2][kz][kx]
+  double *dvv = nullptr;      // statistics time: <vv>(y)
   size_t mean_lds = 0;        // dynamic LDS of mean_kernel (0: one-thread wave00_kernel)
   double *mean_db = nullptr;  // mean_kernel band scratch when it does not fit in LDS
   double *dD0 = nullptr, *dD1 = nullptr, *dD2 = nullptr, *dD0LU = nullptr, *dinteg = nullptr, *ddyr = nullptr,
@@ -90,6 +92,7 @@ struct pb_ctx {
   size_t bytes = 0;
   bool have_state = false;
   bool fused = true;  // fused-sweep wall-normal solver (bit-identical to the statement-by-statement port)
+  bool ct_fft = true;  // fixed-size transforms where one exists for the grid
   pb::RK3 rk;
   // per-stage event timing
   bool timing = false;
@@ -122,6 +125,8 @@ struct pb_ctx {
 
 // scal layout: [0] dt, [1] velmax, [2 + 4*(s-1) + q] mean outputs of substep s
 static constexpr int SCAL_N = 2 + 4 * 3;
+// padded z-pencil fields: 4 state fields in, 8 products out, + v*v at statistics time
+static constexpr int kPadFields = 9;
 
 extern "C" {
 
@@ -130,7 +135,8 @@ const char *pb_last_error(void) { return g_err.c_str(); }
 static int ctx_free(pb_ctx *c) {
   if (!c) return 0;
   void *ptrs[] = {c->U, c->RK, c->pad, c->buf, c->vd, c->dbuf, c->dbuf2, c->vpart, c->scal, c->dD0, c->dD1, c->dD2,
-                  c->dD0LU, c->dinteg, c->ddyr, c->drows, c->twz, c->twx, c->slotmap, c->mean_db};
+                  c->dD0LU, c->dinteg, c->ddyr, c->drows, c->twz, c->twx, c->slotmap, c->mean_db,
+                  c->ppe_bc, c->dvv};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -171,6 +177,7 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   c->lmfzr = 1.0 / cfg->lmfz;
   c->rk = pb::rk3_coefficients();
   if (const char *e = getenv("PB_WAVESPACE_LITERAL")) c->fused = (e[0] == '0');
+  if (const char *e = getenv("PB_FFT_GENERIC")) c->ct_fft = (e[0] == '0');
   int r = pb::build_collocation(c->order, c->ny, cfg->knot_map, cfg->stretch, 2.0 * cfg->lmfy, c->col);
   if (r) { ctx_free(c); return fail(PB_E_ARG, "B-spline set-up failed (%d): check ny, labs_k, knots", r); }
   c->bw = pb::eval_bandwidth(c->col);
@@ -256,7 +263,7 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   int rc = 0;
   rc = rc ? rc : dalloc((void **)&c->U, 9 * c->field * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->RK, 3 * c->field * sizeof(double2));
-  rc = rc ? rc : dalloc((void **)&c->pad, 8 * c->pfield * sizeof(double2));
+  rc = rc ? rc : dalloc((void **)&c->pad, kPadFields * c->pfield * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->buf, 2 * c->field * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->vd, 2 * c->plane * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->dbuf, (size_t)c->W * c->field * sizeof(double));
@@ -273,6 +280,8 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   rc = rc ? rc : dalloc((void **)&c->twz, c->nz32 * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->twx, c->nx32 * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->slotmap, 16 * sizeof(int));
+  rc = rc ? rc : dalloc((void **)&c->ppe_bc, 2 * c->plane * sizeof(double2));
+  rc = rc ? rc : dalloc((void **)&c->dvv, c->ny * sizeof(double));
   // mean line: its columns (and, when they fit, its band) in LDS
   {
     const size_t cols = (size_t)pb::MEAN_COLS * c->ny * sizeof(double2), db = bandb;
@@ -332,7 +341,7 @@ int pb_get_labs(const pb_ctx *c, double *y, double *d0, double *d1, double *d2, 
 // copy nslot reference-layout slots from host into device slots map[]
 static int upload_ref(pb_ctx *c, const double *host, int nslot, const int *map, double2 *dev_base) {
   const size_t bytes = (size_t)nslot * c->field * sizeof(double2);
-  if (bytes > 8 * c->pfield * sizeof(double2)) return fail(PB_E_ARG, "staging too small");
+  if (bytes > kPadFields * c->pfield * sizeof(double2)) return fail(PB_E_ARG, "staging too small");
   HIPCHK(hipMemcpyAsync(c->slotmap, map, nslot * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->pad, host, bytes, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(pb::ref_to_dev_kernel, dim3(c->nz, nslot), dim3(256), 0, c->stream, c->pad, dev_base, c->ny,
@@ -452,9 +461,20 @@ int pb_init_stores(pb_ctx *c) {
 static int launch_oppvd(pb_ctx *c, const double2 *vfield_base_minus_one) {
   pb::LineParams P = c->line_params(1, 0.0);
   StageTimer tm(c, PB_STAGE_OPPVD);
-  hipLaunchKernelGGL(pb::oppvd_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P,
-                     (const pb::cplx *)vfield_base_minus_one, (pb::cplx *)c->buf, (pb::cplx *)c->vd, c->plane,
-                     c->nsys, c->opp_first[0], c->opp_first[1], c->drows, c->order);
+  const dim3 grid(blocks_for(c->nsys, 64)), blk(64);
+  const pb::cplx *U = (const pb::cplx *)vfield_base_minus_one;
+  pb::cplx *buf = (pb::cplx *)c->buf, *vd = (pb::cplx *)c->vd;
+  const int lo = c->opp_first[0], hi = c->opp_first[1], k = c->order;
+  // windowed sweeps when the interpolation stencil fits the up-sweep window
+  const bool win = c->fused && k <= 2 * c->bw + 1 && lo <= hi;
+  if (win && c->bw == 3)
+    hipLaunchKernelGGL(pb::opp_kernel<3>, grid, blk, 0, c->stream, P, U, buf, vd, c->plane, c->nsys, lo, hi, c->drows, k);
+  else if (win && c->bw == 4)
+    hipLaunchKernelGGL(pb::opp_kernel<4>, grid, blk, 0, c->stream, P, U, buf, vd, c->plane, c->nsys, lo, hi, c->drows, k);
+  else if (win && c->bw == 5)
+    hipLaunchKernelGGL(pb::opp_kernel<5>, grid, blk, 0, c->stream, P, U, buf, vd, c->plane, c->nsys, lo, hi, c->drows, k);
+  else
+    hipLaunchKernelGGL(pb::oppvd_kernel, grid, blk, 0, c->stream, P, U, buf, vd, c->plane, c->nsys, lo, hi, c->drows, k);
   HIPCHK(hipGetLastError());
   return PB_OK;
 }
@@ -468,31 +488,29 @@ static int pick_ept(size_t n) {
 }
 
 extern "C++" {
-template <int EPT>
-static int launch_transforms_t(pb_ctx *c, int update_dt) {
+template <class F>
+static int launch_zinv_t(pb_ctx *c) {
   const size_t ldsz = (size_t)c->nz32 * c->Bz * sizeof(double2);
-  {
-    StageTimer tm(c, PB_STAGE_ZINV);
-    hipLaunchKernelGGL(pb::zinv_kernel<EPT>, dim3(c->nxh / c->Bz, c->ny, 4), dim3(pb::FFT_THREADS), ldsz, c->stream,
-                       c->U, c->pad, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield);
-    HIPCHK(hipGetLastError());
-  }
-  return PB_OK;
-}
-template <int EPT>
-static int launch_xphys_t(pb_ctx *c) {
-  const size_t ldsx = (size_t)c->nx32 * 2 * sizeof(double2);
-  StageTimer tm(c, PB_STAGE_XPHYS);
-  hipLaunchKernelGGL(pb::xphys_kernel<EPT>, dim3(c->nz32, c->ny), dim3(pb::FFT_THREADS), ldsx, c->stream, c->pad,
-                     c->px, c->twx, c->nxh, c->nz32, c->pfield, c->ddyr, c->dxr, c->dzr, c->vpart);
+  StageTimer tm(c, PB_STAGE_ZINV);
+  hipLaunchKernelGGL(pb::zinv_kernel<F>, dim3(c->nxh / c->Bz, c->ny, 4), dim3(F::T), ldsz, c->stream, c->U,
+                     c->pad, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield);
   HIPCHK(hipGetLastError());
   return PB_OK;
 }
-template <int EPT>
-static int launch_zfwd_t(pb_ctx *c) {
+template <class F>
+static int launch_xphys_t(pb_ctx *c, int want_vv) {
+  const size_t ldsx = (size_t)c->nx32 * 2 * sizeof(double2);
+  StageTimer tm(c, PB_STAGE_XPHYS);
+  hipLaunchKernelGGL(pb::xphys_kernel<F>, dim3(c->nz32, c->ny), dim3(F::T), ldsx, c->stream, c->pad, c->px, c->twx,
+                     c->nxh, c->nz32, c->pfield, c->ddyr, c->dxr, c->dzr, c->vpart, want_vv ? 3 : 2);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+template <class F>
+static int launch_zfwd_t(pb_ctx *c, int want_vv) {
   const size_t ldsz = (size_t)c->nz32 * c->Bz * sizeof(double2);
   StageTimer tm(c, PB_STAGE_ZFWD);
-  hipLaunchKernelGGL(pb::zfwd_kernel<EPT>, dim3(c->nxh / c->Bz, c->ny, 8), dim3(pb::FFT_THREADS), ldsz, c->stream,
+  hipLaunchKernelGGL(pb::zfwd_kernel<F>, dim3(c->nxh / c->Bz, c->ny, want_vv ? 9 : 8), dim3(F::T), ldsz, c->stream,
                      c->pad, c->U, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield, c->rescale);
   HIPCHK(hipGetLastError());
   return PB_OK;
@@ -501,21 +519,28 @@ static int launch_zfwd_t(pb_ctx *c) {
 
 #define PB_EPT_DISPATCH(ept, call)                                      \
   switch (ept) {                                                        \
-    case 4: { constexpr int E = 4; call; } break;                       \
-    case 8: { constexpr int E = 8; call; } break;                       \
-    case 12: { constexpr int E = 12; call; } break;                     \
-    case 16: { constexpr int E = 16; call; } break;                     \
-    case 24: { constexpr int E = 24; call; } break;                     \
-    case 32: { constexpr int E = 32; call; } break;                     \
+    case 4: { using F = pb::RtFft<4>; call; } break;                    \
+    case 8: { using F = pb::RtFft<8>; call; } break;                    \
+    case 12: { using F = pb::RtFft<12>; call; } break;                  \
+    case 16: { using F = pb::RtFft<16>; call; } break;                  \
+    case 24: { using F = pb::RtFft<24>; call; } break;                  \
+    case 32: { using F = pb::RtFft<32>; call; } break;                  \
     default: return fail(PB_E_ARG, "transform too large for one block"); \
   }
 
-static int launch_nonlinear(pb_ctx *c, int update_dt) {
+// fixed-size transforms for the sizes they were built for (PB_FFT_GENERIC=1
+// forces the run-time passes everywhere)
+static bool ct_z(const pb_ctx *c) { return c->ct_fft && c->nz32 == pb::FftZ1152::N && c->Bz == pb::FftZ1152::B; }
+static bool ct_x(const pb_ctx *c) { return c->ct_fft && c->nx32 == pb::FftX1536::N; }
+
+static int launch_nonlinear(pb_ctx *c, int update_dt, int want_vv) {
   int r = PB_OK;
   const int ez = pick_ept((size_t)c->nz32 * c->Bz), ex = pick_ept((size_t)c->nx32 * 2);
-  PB_EPT_DISPATCH(ez, r = launch_transforms_t<E>(c, update_dt));
+  if (ct_z(c)) r = launch_zinv_t<pb::FftZ1152>(c);
+  else PB_EPT_DISPATCH(ez, r = launch_zinv_t<F>(c));
   if (r) return r;
-  PB_EPT_DISPATCH(ex, r = launch_xphys_t<E>(c));
+  if (ct_x(c)) r = launch_xphys_t<pb::FftX1536>(c, want_vv);
+  else PB_EPT_DISPATCH(ex, r = launch_xphys_t<F>(c, want_vv));
   if (r) return r;
   {
     StageTimer tm(c, PB_STAGE_VELMAX);
@@ -523,7 +548,8 @@ static int launch_nonlinear(pb_ctx *c, int update_dt) {
                        c->scal, update_dt, c->cfg.dt_factor, c->cflx);
     HIPCHK(hipGetLastError());
   }
-  PB_EPT_DISPATCH(ez, r = launch_zfwd_t<E>(c));
+  if (ct_z(c)) r = launch_zfwd_t<pb::FftZ1152>(c, want_vv);
+  else PB_EPT_DISPATCH(ez, r = launch_zfwd_t<F>(c, want_vv));
   return r;
 }
 
@@ -560,8 +586,7 @@ static int launch_sweeps(pb_ctx *c, const pb::LineParams &P, const pb::WsArgs &A
 }
 }  // extern "C++"
 
-static int launch_wavespace(pb_ctx *c, int s) {
-  pb::LineParams P = c->line_params(s, 0.0);
+static pb::WsArgs ws_args(pb_ctx *c, int s) {
   pb::WsArgs A;
   A.U = (pb::cplx *)c->U;
   A.RK = (pb::cplx *)c->RK;
@@ -577,6 +602,12 @@ static int launch_wavespace(pb_ctx *c, int s) {
   A.lmfzr = c->lmfzr;
   A.opp_alpha = c->cfg.opp_alpha;
   A.mean_out = c->scal + 2 + 4 * (s - 1);
+  return A;
+}
+
+static int launch_wavespace(pb_ctx *c, int s) {
+  pb::LineParams P = c->line_params(s, 0.0);
+  const pb::WsArgs A = ws_args(c, s);
   StageTimer tm(c, PB_STAGE_WAVESPACE);
   if (c->fused && (c->bw >= 3 && c->bw <= 5)) {
     // mean_kernel touches only the columns of system 0 and its own LDS: fork
@@ -616,10 +647,41 @@ static int check_dt(double dt) {
   return PB_OK;
 }
 
+// Statistics time, before the transforms (get_statistics_1: <vv> and the
+// pressure wall values from the substep's input v).
+static int launch_stats_pre(pb_ctx *c) {
+  pb::LineParams P = c->line_params(1, 0.0);
+  hipLaunchKernelGGL(pb::vv_kernel, dim3(blocks_for(c->ny, 64)), dim3(64), 0, c->stream, (const pb::cplx *)c->U,
+                     c->plane, c->ny, c->nxh, c->dvv);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(pb::ppe_bc_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P,
+                     (const pb::cplx *)c->U, (pb::cplx *)c->buf, (pb::cplx *)c->ppe_bc, c->plane, c->nsys, c->cfg.nu);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+// Statistics time, after the transforms: p into product slot 5 (get_pressure.f90).
+static int launch_pressure(pb_ctx *c) {
+  pb::LineParams P = c->line_params(1, 0.0);
+  const pb::WsArgs A = ws_args(c, 1);
+  hipLaunchKernelGGL(pb::ppe_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P, A,
+                     (const pb::cplx *)c->ppe_bc, c->dvv);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+
+// transforms of substep s (with the statistics-time pressure when stats)
+static int substep_front(pb_ctx *c, int s, bool stats) {
+  int r;
+  if (stats && (r = launch_stats_pre(c))) return r;
+  if (c->cfg.opp_alpha != 0.0 && (r = launch_oppvd(c, c->U))) return r;
+  if ((r = launch_nonlinear(c, s == 1, stats))) return r;
+  if (stats && (r = launch_pressure(c))) return r;
+  return PB_OK;
+}
+
 static int substep_async(pb_ctx *c, int s) {
   int r;
-  if (c->cfg.opp_alpha != 0.0 && (r = launch_oppvd(c, c->U))) return r;
-  if ((r = launch_nonlinear(c, s == 1))) return r;
+  if ((r = substep_front(c, s, false))) return r;
   return launch_wavespace(c, s);
 }
 
@@ -648,6 +710,26 @@ int pb_substep(pb_ctx *c, int32_t s, double *dt_inout, pb_substep_out *out) {
   if (r && !(s == 1 && c->cfg.dt_factor != 1.0)) return r;
   HIPCHK(hipMemcpyAsync(c->scal, dt_inout, sizeof(double), hipMemcpyHostToDevice, c->stream));
   if ((r = substep_async(c, s))) return r;
+  return read_outputs(c, s, s, out, dt_inout);
+}
+
+static const int kSlot5[1] = {5};
+
+int pb_substep_stats(pb_ctx *c, int32_t s, double *dt_inout, pb_substep_out *out, double *p_out, double *vv_out) {
+  if (s != 1) return pb_substep(c, s, dt_inout, out);
+  if (!c || !dt_inout) return fail(PB_E_ARG, "null argument");
+  if (!c->have_state) return fail(PB_E_STATE, "pb_substep_stats before pb_set_state");
+  int r = check_dt(*dt_inout);
+  if (r && c->cfg.dt_factor == 1.0) return r;
+  HIPCHK(hipMemcpyAsync(c->scal, dt_inout, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if ((r = substep_front(c, s, true))) return r;
+  // p leaves through the staging buffer before the solver reuses slot 5
+  if (p_out && (r = download_ref(c, p_out, 1, kSlot5, c->U))) return r;
+  if (vv_out) {
+    HIPCHK(hipMemcpyAsync(vv_out, c->dvv, c->ny * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if ((r = launch_wavespace(c, s))) return r;
   return read_outputs(c, s, s, out, dt_inout);
 }
 
@@ -693,13 +775,12 @@ int pb_timing_read(pb_ctx *c, double *ms, int64_t *launches) {
 // ---- host-array drop-ins ---------------------------------------------------
 int pb_nonlinear_host(pb_ctx *c, const double *U, double *P, int32_t want_vv, double *velmax) {
   if (!c || !U || !P) return fail(PB_E_ARG, "null argument");
-  if (want_vv) return fail(PB_E_ARG, "the v*v statistics field is not produced on the device");
   int r = upload_ref(c, U, 4, kStateMap, c->U);
   if (r) return r;
   c->have_state = true;
-  if ((r = launch_nonlinear(c, 0))) return r;
+  if ((r = launch_nonlinear(c, 0, want_vv != 0))) return r;
   static const int map8[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
-  // reference slots 0-8 of P (slot 5 is left as the device holds it: unused)
+  // reference slots 0-8 of P (slot 5, v*v, only meaningful with want_vv)
   r = download_ref(c, P, 9, map8, c->U);
   if (r) return r;
   if (velmax) {

@@ -95,6 +95,97 @@ __device__ __forceinline__ void dft<8>(double2 *v, int sign) {
   }
 }
 
+// cos(2 pi m / R), sin(2 pi m / R) for the composite radices (long-double
+// values rounded to double; the quarter-turn points exact)
+template <int R>
+struct Wr;
+template <>
+struct Wr<6> {
+  static constexpr double c[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
+  static constexpr double s[6] = {0.0, 0.8660254037844386, 0.8660254037844386, 0.0, -0.8660254037844386,
+                                  -0.8660254037844386};
+};
+template <>
+struct Wr<9> {
+  static constexpr double c[9] = {1.0, 0.766044443118978, 0.17364817766693036, -0.5, -0.9396926207859084,
+                                  -0.9396926207859084, -0.5, 0.17364817766693036, 0.766044443118978};
+  static constexpr double s[9] = {0.0, 0.6427876096865394, 0.984807753012208, 0.8660254037844386,
+                                  0.3420201433256687, -0.3420201433256687, -0.8660254037844386, -0.984807753012208,
+                                  -0.6427876096865394};
+};
+template <>
+struct Wr<12> {
+  static constexpr double c[12] = {1.0, 0.8660254037844386, 0.5, 0.0, -0.5, -0.8660254037844386,
+                                   -1.0, -0.8660254037844386, -0.5, 0.0, 0.5, 0.8660254037844386};
+  static constexpr double s[12] = {0.0, 0.5, 0.8660254037844386, 1.0, 0.8660254037844386, 0.5,
+                                   0.0, -0.5, -0.8660254037844386, -1.0, -0.8660254037844386, -0.5};
+};
+template <>
+struct Wr<16> {
+  static constexpr double c[16] = {1.0, 0.9238795325112867, 0.7071067811865476, 0.3826834323650898,
+                                   0.0, -0.3826834323650898, -0.7071067811865476, -0.9238795325112867,
+                                   -1.0, -0.9238795325112867, -0.7071067811865476, -0.3826834323650898,
+                                   0.0, 0.3826834323650898, 0.7071067811865476, 0.9238795325112867};
+  static constexpr double s[16] = {0.0, 0.3826834323650898, 0.7071067811865476, 0.9238795325112867,
+                                   1.0, 0.9238795325112867, 0.7071067811865476, 0.3826834323650898,
+                                   0.0, -0.3826834323650898, -0.7071067811865476, -0.9238795325112867,
+                                   -1.0, -0.9238795325112867, -0.7071067811865476, -0.3826834323650898};
+};
+
+template <>
+struct Wr<18> {
+  static constexpr double c[18] = {1.0, 0.9396926207859084, 0.766044443118978, 0.5, 0.17364817766693036,
+                                   -0.17364817766693036, -0.5, -0.766044443118978, -0.9396926207859084, -1.0,
+                                   -0.9396926207859084, -0.766044443118978, -0.5, -0.17364817766693036,
+                                   0.17364817766693036, 0.5, 0.766044443118978, 0.9396926207859084};
+  static constexpr double s[18] = {0.0, 0.3420201433256687, 0.6427876096865394, 0.8660254037844386,
+                                   0.984807753012208, 0.984807753012208, 0.8660254037844386, 0.6427876096865394,
+                                   0.3420201433256687, 0.0, -0.3420201433256687, -0.6427876096865394,
+                                   -0.8660254037844386, -0.984807753012208, -0.984807753012208,
+                                   -0.8660254037844386, -0.6427876096865394, -0.3420201433256687};
+};
+
+// Composite in-register DFT, R = P*Q (Cooley-Tukey): P-point DFTs over
+// n1 of x[Q n1 + n2], twiddle W_R^(n2 k1), Q-point DFTs over n2, output
+// X[k1 + P k2].
+template <int P, int Q>
+__device__ __forceinline__ void dft_pq(double2 *v, int sign) {
+  constexpr int R = P * Q;
+  double2 a[Q][P];
+#pragma unroll
+  for (int n2 = 0; n2 < Q; ++n2) {
+#pragma unroll
+    for (int n1 = 0; n1 < P; ++n1) a[n2][n1] = v[Q * n1 + n2];
+    dft<P>(a[n2], sign);
+  }
+#pragma unroll
+  for (int n2 = 1; n2 < Q; ++n2)
+#pragma unroll
+    for (int k1 = 1; k1 < P; ++k1) {
+      const int m = (n2 * k1) % R;
+      a[n2][k1] = cmul2(a[n2][k1], c2(Wr<R>::c[m], sign * Wr<R>::s[m]));
+    }
+#pragma unroll
+  for (int k1 = 0; k1 < P; ++k1) {
+    double2 t[Q];
+#pragma unroll
+    for (int n2 = 0; n2 < Q; ++n2) t[n2] = a[n2][k1];
+    dft<Q>(t, sign);
+#pragma unroll
+    for (int k2 = 0; k2 < Q; ++k2) v[k1 + P * k2] = t[k2];
+  }
+}
+template <>
+__device__ __forceinline__ void dft<6>(double2 *v, int sign) { dft_pq<2, 3>(v, sign); }
+template <>
+__device__ __forceinline__ void dft<9>(double2 *v, int sign) { dft_pq<3, 3>(v, sign); }
+template <>
+__device__ __forceinline__ void dft<12>(double2 *v, int sign) { dft_pq<4, 3>(v, sign); }
+template <>
+__device__ __forceinline__ void dft<16>(double2 *v, int sign) { dft_pq<4, 4>(v, sign); }
+template <>
+__device__ __forceinline__ void dft<18>(double2 *v, int sign) { dft_pq<2, 9>(v, sign); }
+
 // One Stockham pass of radix R on x[n*B + b] (n < N, b < B), in place via
 // registers.  Ns = product of the radices already applied.
 template <int R, int MAXB>
@@ -177,20 +268,102 @@ __device__ inline Blk xcd_block() {
   return b;
 }
 
+// ------------------------------------------------- compile-time plans
+// The same Stockham passes with N, B, the radices and the direction known at
+// compile time: index arithmetic folds to shifts and multiplies, the first
+// pass needs no twiddles, and every register array has its exact size.
+template <int T, int N, int B, int SIGN, int Ns, int R>
+__device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ tw) {
+  constexpr int NR = N / R, NBF = NR * B, ROUNDS = (NBF + T - 1) / T;
+  constexpr int tstep = N / (Ns * R);
+  double2 v[ROUNDS][R];
+#pragma unroll
+  for (int q = 0; q < ROUNDS; ++q) {
+    const int t = threadIdx.x + q * T;
+    if (NBF % T == 0 || t < NBF) {
+      const int b = t % B, j = t / B;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = x[(j + r * NR) * B + b];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < ROUNDS; ++q) {
+    const int t = threadIdx.x + q * T;
+    if (NBF % T == 0 || t < NBF) {
+      const int b = t % B, j = t / B;
+      const int k = j % Ns;
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          double2 w = tw[r * k * tstep];
+          if (SIGN > 0) w.y = -w.y;
+          v[q][r] = cmul2(v[q][r], w);
+        }
+      }
+      dft<R>(v[q], SIGN);
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[(base + r * Ns) * B + b] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+template <int T, int N, int B, int SIGN, int Ns, int R, int... Rest>
+__device__ __forceinline__ void ct_passes(double2 *x, const double2 *__restrict__ tw) {
+  ct_pass<T, N, B, SIGN, Ns, R>(x, tw);
+  if constexpr (sizeof...(Rest) > 0) ct_passes<T, N, B, SIGN, Ns * R, Rest...>(x, tw);
+}
+
+// Transform policies for the three kernels: N = B = 0 means sizes at run
+// time (any 2,3,5-smooth length, generic passes); otherwise fixed sizes.
+template <int EPT>
+struct RtFft {
+  static constexpr int T = FFT_THREADS, N = 0, B = 0, ELEMS = EPT;
+  __device__ static void run(double2 *x, int B_, const FftPlan &P, const double2 *__restrict__ tw, int sign) {
+    lds_fft<EPT>(x, B_, P, tw, sign);
+  }
+};
+template <int T_, int N_, int B_, int... R>
+struct CtFft {
+  static constexpr int T = T_, N = N_, B = B_, ELEMS = (N_ * B_ + T_ - 1) / T_;
+  static_assert((R * ...) == N_, "radices must multiply to N");
+  static_assert(T_ % 64 == 0 && T_ <= 1024, "whole waves");
+  __device__ static void run(double2 *x, int, const FftPlan &, const double2 *__restrict__ tw, int sign) {
+    if (sign > 0)
+      ct_passes<T_, N_, B_, +1, 1, R...>(x, tw);
+    else
+      ct_passes<T_, N_, B_, -1, 1, R...>(x, tw);
+  }
+};
+// 3/2-padded production sizes.  nz32 = 1152 with 4 kx columns per block:
+// 576 threads hold 8 elements each, radix 18, 8, 8 (one butterfly per
+// thread per pass, 256 of 576 in the first).  nx32 = 1536, the two lines of
+// a pair: 256 threads, radix 16, 16, 6.
+#ifndef PB_FFTZ1152
+#define PB_FFTZ1152 CtFft<576, 1152, 4, 18, 8, 8>
+#endif
+#ifndef PB_FFTX1536
+#define PB_FFTX1536 CtFft<256, 1536, 2, 16, 16, 6>
+#endif
+using FftZ1152 = PB_FFTZ1152;
+using FftX1536 = PB_FFTX1536;
+
 // ------------------------------------------------------------------ z inverse
 // grid: (nxh/B, ny, nfield); the 4 state fields u, v, w, theta live in U
 // slots 0,1,2,6.  Output pad[f][y][z][kx].
-template <int EPT>
-__global__ void __launch_bounds__(FFT_THREADS) zinv_kernel(const double2 *__restrict__ U, double2 *__restrict__ pad,
+template <class F>
+__global__ void __launch_bounds__(F::T) zinv_kernel(const double2 *__restrict__ U, double2 *__restrict__ pad,
                                                            FftPlan P, const double2 *__restrict__ tw, int nxh, int nz,
-                                                           int ny, int B, size_t field, size_t pfield) {
+                                                           int ny, int B_, size_t field, size_t pfield) {
   extern __shared__ double2 lds[];
+  const int B = F::B ? F::B : B_;
   const Blk bk = xcd_block();
   const int kx0 = bk.x * B, y = bk.y, f = bk.z;
   const int slot = (f == 3) ? 6 : f;
-  const int N = P.N, nh = nz / 2;
+  const int N = F::N ? F::N : P.N, nh = nz / 2;
   const double2 *src = U + (size_t)slot * field + (size_t)y * nz * nxh + kx0;
-  for (int e = threadIdx.x; e < N * B; e += blockDim.x) {
+  for (int e = threadIdx.x; e < N * B; e += F::T) {
     const int z = e / B, b = e % B;
     double2 val = c2(0.0, 0.0);
     if (z < nh) val = src[(size_t)z * nxh + b];
@@ -198,9 +371,9 @@ __global__ void __launch_bounds__(FFT_THREADS) zinv_kernel(const double2 *__rest
     lds[e] = val;
   }
   __syncthreads();
-  lds_fft<EPT>(lds, B, P, tw, +1);
+  F::run(lds, B, P, tw, +1);
   double2 *dst = pad + (size_t)f * pfield + (size_t)y * N * nxh + kx0;
-  for (int e = threadIdx.x; e < N * B; e += blockDim.x) {
+  for (int e = threadIdx.x; e < N * B; e += F::T) {
     const int z = e / B, b = e % B;
     dst[(size_t)z * nxh + b] = lds[e];
   }
@@ -209,17 +382,18 @@ __global__ void __launch_bounds__(FFT_THREADS) zinv_kernel(const double2 *__rest
 // ---------------------------------------------------------------- x physical
 // grid: (nz32, ny): one x line per block.  pad fields 0..3 = u,v,w,theta in,
 // 0..7 = spectra of u^2-v^2, uv, w^2-v^2, vw, uw, u theta, v theta, w theta out.
-template <int EPT>
-__global__ void __launch_bounds__(FFT_THREADS) xphys_kernel(double2 *__restrict__ pad, FftPlan P,
+template <class F>
+__global__ void __launch_bounds__(F::T) xphys_kernel(double2 *__restrict__ pad, FftPlan P,
                                                             const double2 *__restrict__ tw, int nxh, int nz32,
                                                             size_t pfield, const double *__restrict__ dyr,
-                                                            double dxr, double dzr, double *__restrict__ vpart) {
+                                                            double dxr, double dzr, double *__restrict__ vpart,
+                                                            int nround) {
   extern __shared__ double2 lds[];  // [n][2]
-  __shared__ double red[FFT_THREADS];
-  const int z = blockIdx.x, y = blockIdx.y, N = P.N;
+  __shared__ double red[F::T];
+  const int z = blockIdx.x, y = blockIdx.y, N = F::N ? F::N : P.N;
   double2 *line = pad + (size_t)y * nz32 * nxh + (size_t)z * nxh;
   // Hermitian completion of (u + i v) and (w + i theta); FFTW c2r ignores Im of the DC bin
-  for (int m = threadIdx.x; m < N; m += blockDim.x) {
+  for (int m = threadIdx.x; m < N; m += F::T) {
     double2 z1 = c2(0.0, 0.0), z2 = c2(0.0, 0.0);
     if (m == 0) {
       z1 = c2(line[0].x, line[pfield].x);
@@ -238,15 +412,15 @@ __global__ void __launch_bounds__(FFT_THREADS) xphys_kernel(double2 *__restrict_
     lds[2 * m + 1] = z2;
   }
   __syncthreads();
-  lds_fft<EPT>(lds, 2, P, tw, +1);
+  F::run(lds, 2, P, tw, +1);
   // products at this thread's points, kept in registers
-  constexpr int MAXP = (EPT + 1) / 2;
+  constexpr int MAXP = (F::ELEMS + 1) / 2;
   double pu[MAXP], pv[MAXP], pw[MAXP], pt[MAXP];
   const double dy = dyr[y];
   double vmax = 0.0;
 #pragma unroll
   for (int q = 0; q < MAXP; ++q) {
-    const int m = threadIdx.x + q * blockDim.x;
+    const int m = threadIdx.x + q * F::T;
     if (m < N) {
       pu[q] = lds[2 * m].x; pv[q] = lds[2 * m].y; pw[q] = lds[2 * m + 1].x; pt[q] = lds[2 * m + 1].y;
       const double vel = (fabs(pu[q]) * dxr) + (fabs(pv[q]) * dy) + (fabs(pw[q]) * dzr);
@@ -255,32 +429,41 @@ __global__ void __launch_bounds__(FFT_THREADS) xphys_kernel(double2 *__restrict_
   }
   red[threadIdx.x] = vmax;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  constexpr int P2 = (F::T & (F::T - 1)) ? (1 << (31 - __builtin_clz(F::T))) : F::T;  // largest 2^j <= T
+  if (P2 < F::T) {
+    if ((int)threadIdx.x < F::T - P2) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + P2]);
+    __syncthreads();
+  }
+  for (int s = P2 / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0) vpart[(size_t)y * nz32 + z] = red[0];
-  // two pairs of products per round: (P1 + i P2, P3 + i P4), (P5 + i P7, P8 + i P9)
-  for (int round = 0; round < 2; ++round) {
+  // two pairs of products per round: (P1 + i P2, P3 + i P4), (P5 + i P7, P8 + i P9);
+  // a third round (statistics time) transforms v*v alone into pad field 8
+  for (int round = 0; round < nround; ++round) {
 #pragma unroll
     for (int q = 0; q < MAXP; ++q) {
-      const int m = threadIdx.x + q * blockDim.x;
+      const int m = threadIdx.x + q * F::T;
       if (m < N) {
         const double t1 = pu[q], t2 = pv[q], t3 = pw[q], t4 = pt[q];
         const double bv2 = t2 * t2;
         if (round == 0) {
           lds[2 * m] = c2(t1 * t1 - bv2, t1 * t2);
           lds[2 * m + 1] = c2(t3 * t3 - bv2, t2 * t3);
-        } else {
+        } else if (round == 1) {
           lds[2 * m] = c2(t1 * t3, t1 * t4);
           lds[2 * m + 1] = c2(t2 * t4, t3 * t4);
+        } else {
+          lds[2 * m] = c2(bv2, 0.0);
+          lds[2 * m + 1] = c2(0.0, 0.0);
         }
       }
     }
     __syncthreads();
-    lds_fft<EPT>(lds, 2, P, tw, -1);
+    F::run(lds, 2, P, tw, -1);
     // unpack the two real spectra of each pair, keep modes 0..nxh-1
-    for (int m = threadIdx.x; m < nxh; m += blockDim.x) {
+    for (int m = threadIdx.x; m < nxh; m += F::T) {
       const int q = (m == 0) ? 0 : N - m;
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
@@ -288,8 +471,12 @@ __global__ void __launch_bounds__(FFT_THREADS) xphys_kernel(double2 *__restrict_
         const double2 A = c2((Z.x + Zc.x) * 0.5, (Z.y - Zc.y) * 0.5);
         const double2 Bv = c2((Z.y + Zc.y) * 0.5, (Zc.x - Z.x) * 0.5);
         const int fa = round * 4 + 2 * b;
-        line[(size_t)fa * pfield + m] = A;
-        line[(size_t)(fa + 1) * pfield + m] = Bv;
+        if (round < 2) {
+          line[(size_t)fa * pfield + m] = A;
+          line[(size_t)(fa + 1) * pfield + m] = Bv;
+        } else if (b == 0) {
+          line[(size_t)8 * pfield + m] = A;
+        }
       }
     }
     __syncthreads();
@@ -297,27 +484,28 @@ __global__ void __launch_bounds__(FFT_THREADS) xphys_kernel(double2 *__restrict_
 }
 
 // ------------------------------------------------------------------ z forward
-// grid: (nxh/B, ny, 8).  pad field g -> U slot (0,1,2,3,4,6,7,8)[g], with
-// truncation and rescale.
-template <int EPT>
-__global__ void __launch_bounds__(FFT_THREADS) zfwd_kernel(const double2 *__restrict__ pad, double2 *__restrict__ U,
+// grid: (nxh/B, ny, 8 or 9).  pad field g -> U slot (0,1,2,3,4,6,7,8,5)[g],
+// with truncation and rescale.
+template <class F>
+__global__ void __launch_bounds__(F::T) zfwd_kernel(const double2 *__restrict__ pad, double2 *__restrict__ U,
                                                            FftPlan P, const double2 *__restrict__ tw, int nxh, int nz,
-                                                           int ny, int B, size_t field, size_t pfield,
+                                                           int ny, int B_, size_t field, size_t pfield,
                                                            double rescale) {
   extern __shared__ double2 lds[];
+  const int B = F::B ? F::B : B_;
   const Blk bk = xcd_block();
   const int kx0 = bk.x * B, y = bk.y, g = bk.z;
-  const int slot = g < 5 ? g : g + 1;
-  const int N = P.N, nh = nz / 2;
+  const int slot = g < 5 ? g : (g < 8 ? g + 1 : 5);
+  const int N = F::N ? F::N : P.N, nh = nz / 2;
   const double2 *src = pad + (size_t)g * pfield + (size_t)y * N * nxh + kx0;
-  for (int e = threadIdx.x; e < N * B; e += blockDim.x) {
+  for (int e = threadIdx.x; e < N * B; e += F::T) {
     const int z = e / B, b = e % B;
     lds[e] = src[(size_t)z * nxh + b];
   }
   __syncthreads();
-  lds_fft<EPT>(lds, B, P, tw, -1);
+  F::run(lds, B, P, tw, -1);
   double2 *dst = U + (size_t)slot * field + (size_t)y * nz * nxh + kx0;
-  for (int e = threadIdx.x; e < nz * B; e += blockDim.x) {
+  for (int e = threadIdx.x; e < nz * B; e += F::T) {
     const int j = e / B, b = e % B;
     double2 val;
     if (j < nh) val = lds[j * B + b];
@@ -450,6 +638,56 @@ __global__ void __launch_bounds__(64) oppvd_kernel(LineParams P, const cplx *U, 
   const SV b{buf + sys, plane};
   vd[sys] = interp_plane(P, v, b, first_lo, rows, k);
   vd[plane + sys] = interp_plane(P, v, b, first_hi, rows + k, k);
+}
+
+// the same with the windowed sweeps: reads v once, y written and read once
+template <int BW>
+__global__ void __launch_bounds__(64) opp_kernel(LineParams P, const cplx *U, cplx *buf, cplx *vd, size_t plane,
+                                                 size_t nsys, int first_lo, int first_hi, const double *rows, int k) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const SV v{const_cast<cplx *>(U) + (size_t)1 * plane * P.g.n + sys, plane};
+  const SV y{buf + sys, plane};
+  cplx lo, hi;
+  opp_line<BW>(P, v, y, first_lo, first_hi, rows, k, &lo, &hi);
+  vd[sys] = lo;
+  vd[plane + sys] = hi;
+}
+
+// --------------------------------------------- statistics-time pressure
+// <vv>(y): one thread per y plane, serial over the plane's modes
+__global__ void __launch_bounds__(64) vv_kernel(const cplx *U, size_t plane, int ny, int nxh, double *vv) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= ny) return;
+  vv[y] = mean_vv_row(U + (size_t)1 * plane * ny + (size_t)y * plane, nxh, plane);
+}
+// wall values of the pressure Poisson equation from v (slot 1), per system
+__global__ void __launch_bounds__(64) ppe_bc_kernel(LineParams P, const cplx *U, cplx *tmp, cplx *bc, size_t plane,
+                                                    size_t nsys, double nu) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const SV v{const_cast<cplx *>(U) + (size_t)1 * plane * P.g.n + sys, plane};
+  const SV t{tmp + sys, plane};
+  cplx b1, b2;
+  ppe_wall_values(P, v, t, nu, &b1, &b2);
+  bc[sys] = b1;
+  bc[plane + sys] = b2;
+}
+// pressure of every system into product slot 5 (v*v in)
+__global__ void __launch_bounds__(64) ppe_kernel(LineParams P, WsArgs A, const cplx *bc, const double *vv) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= A.plane) return;
+  const int k = (int)(sys % A.nxh), j = (int)(sys / A.nxh);
+  const int kzi = (j <= A.nh) ? j : j + 1 - 2 * A.nh - 1;  // module_ppdfftw.f90:360-374
+  auto col = [&](int slot) { return SV{A.U + (size_t)slot * A.field + sys, A.plane}; };
+  if (k == 0 && kzi == 0) {  // get_pressure.f90:45-48
+    const SV p = col(5);
+    for (int y = 0; y < P.g.n; ++y) p[y] = mk(-vv[y], 0.0);
+    return;
+  }
+  const OwnBand Db{A.dbuf + sys, A.plane, P.g.W()};
+  solve_ppe(P, Db, col(0), col(1), col(2), col(3), col(4), col(5), bc[sys], bc[A.plane + sys],
+            SV{A.buf + sys, A.plane}, SV{A.buf + A.field + sys, A.plane}, k * A.lmfxr, kzi * A.lmfzr);
 }
 
 // ------------------------------------------------------- layout conversions

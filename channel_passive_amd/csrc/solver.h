@@ -440,6 +440,65 @@ PB_HD void wave00(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, S
   mean_t(P, Db, u7, u8, u9, st3, buf1, out4);
 }
 
+// ------------------------------------------------- statistics-time pressure
+// Wall values of the pressure Poisson equation (get_statistics.f90:689-716):
+// c = LU(D0)^-1 v (in tmp), bc1 = nu*(D2 row 0 . c_0..2),
+// bc2 = nu*(D2 row n-1 . c_n-3..n-1).
+PB_HD void ppe_wall_values(const LineParams &P, const SV &v, const SV &tmp, double nu, cplx *bc1, cplx *bc2) {
+  const int n = P.g.n, bw = P.g.bw;
+  for (int i = 0; i < n; ++i) tmp[i] = v[i];
+  band_bs(P.g, P.D0LU, tmp);
+  *bc1 = cscale(cadd(cadd(cscale(tmp[0], P.D2(0, bw)), cscale(tmp[1], P.D2(0, bw + 1))), cscale(tmp[2], P.D2(0, bw + 2))),
+                nu);
+  *bc2 = cscale(cadd(cadd(cscale(tmp[n - 3], P.D2(n - 1, bw - 2)), cscale(tmp[n - 2], P.D2(n - 1, bw - 1))),
+                     cscale(tmp[n - 1], P.D2(n - 1, bw))),
+                nu);
+}
+
+// solve_ppe (get_pressure.f90:55-133) for one (kx,kz) != (0,0) system:
+// u1..u5 = u^2-v^2, uv, w^2-v^2, vw, uw are read; u6 = v^2 in, p out.
+PB_HD void solve_ppe(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, SV u4, SV u5, SV u6, cplx bc1,
+                     cplx bc2, SV buf1, SV buf2, double kx, double kz) {
+  const int n = P.g.n, W = P.g.W();
+  const cplx ikx = mk(0.0, kx), ikz = mk(0.0, kz);  // dcmplx(0d0, k)
+  const double k2 = kx * kx + kz * kz, kxkz = kx * kz;
+  for (int i = 1; i < n - 1; ++i)
+    buf1[i] = cadd(cadd(cadd(cscale(u1[i], kx * kx), cscale(u3[i], kz * kz)), cscale(u6[i], k2)),
+                   cscale(u5[i], 2.0 * kxkz));
+  band_bs(P.g, P.D0LU, u6);
+  band_mv(P.g, P.D2, u6, buf2);
+  for (int i = 1; i < n - 1; ++i) {
+    buf1[i] = csub(buf1[i], buf2[i]);
+    u6[i] = cadd(cmul(ikx, u2[i]), cmul(ikz, u4[i]));  // rows 0 and n-1 keep the coefficients above
+  }
+  band_bs(P.g, P.D0LU, u6);
+  band_mv(P.g, P.D1, u6, buf2);
+  buf1[0] = bc1;
+  for (int i = 1; i < n - 1; ++i) buf1[i] = csub(buf1[i], cscale(buf2[i], 2.0));
+  buf1[n - 1] = bc2;
+  for (int i = 0; i < W; ++i) Db.at(0, i) = P.D1(0, i);
+  for (int j = 1; j < n - 1; ++j)
+    for (int i = 0; i < W; ++i) Db.at(j, i) = -(k2 * P.D0(j, i)) + P.D2(j, i);
+  for (int i = 0; i < W; ++i) Db.at(n - 1, i) = P.D1(n - 1, i);
+  band_lu(P.g, Db);
+  band_bs(P.g, Db, buf1);
+  band_mv(P.g, P.D0, buf1, u6);
+}
+
+// <vv>(y) of get_uv (get_statistics.f90:982-1017) for one y: the modes of
+// the plane row in kz-major, kx-minor order, kx = 0 once, kx > 0 twice.
+PB_HD double mean_vv_row(const cplx *row, int nxh, size_t plane) {
+  double acc = 0.0;
+  for (size_t l = 0; l < plane; ++l) {
+    const cplx v = row[l];
+    if (l % nxh == 0)
+      acc = acc + v.re * v.re + v.im * v.im;
+    else
+      acc = acc + 2.0 * (v.re * v.re + v.im * v.im);
+  }
+  return acc;
+}
+
 // Detection-plane value of a collocation column: c = D0^{-1} v (in buf),
 // v(y_d) = sum_j B_j(y_d) c_j, ascending j.
 PB_HD cplx interp_plane(const LineParams &P, const SV &v, const SV &buf, int first, const double *vals, int k) {

@@ -724,6 +724,39 @@ PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   }
 }
 
+// Opposition-control detection planes of one system: c = LU(D0)^-1 v by a
+// down and an up sweep with the arithmetic of band_bs (y stored in the
+// scratch column), then v(y_d) = sum_j B_j(y_d) c_(first+j), ascending j,
+// from the up sweep's window when it reaches row first (needs k <= 2bw+1).
+template <int BW>
+PB_HD cplx opp_interp(const CWin<2 * BW + 1> &w, const double *vals, int k) {
+  cplx acc = cscale(w.w[0], ld_uniform(vals));
+#pragma unroll
+  for (int j = 1; j <= 2 * BW; ++j)
+    if (j < k) acc = cadd(acc, cscale(w.w[j], ld_uniform(vals + j)));
+  return acc;
+}
+template <int BW>
+PB_HD void opp_line(const LineParams &P, const SV &v, const SV &y, int first_lo, int first_hi, const double *rows,
+                    int k, cplx *lo, cplx *hi) {
+  const BandGeom g{P.g.n, BW};
+  const int n = g.n;
+  CWin<BW> wf;
+  wf.clear();
+  for (int r = 0; r < n; ++r) {
+    const cplx yr = fwd_row<BW>(g, P.D0LU, r, wf, y, v[r]);
+    y[r] = yr;
+    wf.push(yr);
+  }
+  CWin<2 * BW + 1> wb;
+  wb.clear();
+  for (int r = n - 1; r >= first_lo; --r) {
+    wb.push(bwd_row<BW>(g, P.D0LU, r, wb, y[r]));
+    if (r == first_hi) *hi = opp_interp<BW>(wb, rows + k, k);
+    if (r == first_lo) *lo = opp_interp<BW>(wb, rows, k);
+  }
+}
+
 // One sweep of one (kx,kz) != (0,0) system.  A substep runs, for all
 // systems, A1 and B1 (s == 1 only), A, B_g, then the theta chain B_t, C_t,
 // D_t and, in any order relative to it, B_phi, C_nu, D_nu; then E, F, G.

@@ -117,6 +117,17 @@ int pb_init_stores(pb_ctx *ctx);
  * (time_march.f90:63-120) in one call.  On substep 1 dt is updated from the
  * CFL condition when dt_factor != 1. */
 int pb_substep(pb_ctx *ctx, int32_t s, double *dt_inout, pb_substep_out *out);
+/* pb_substep with the statistics-time work of substep 1 (time_march.f90:
+ * 47-110 with stat_store_or_not set): <vv>(y) and the pressure wall values
+ * nu*D2(wall row).LU(D0)^-1 v from the substep's input state (get_uv,
+ * get_statistics.f90:982-1017; get_y_dir_deriv_of_field_with_PPE_BC,
+ * 689-716), the v*v product (x_to_y of field 6), and the pressure of
+ * get_pressure (get_pressure.f90:17-133; p = -<vv> on the mean line).
+ * p_out receives p in the reference y-pencil layout (ny, nx/2, nz) complex,
+ * vv_out receives <vv> (ny doubles); either may be NULL.  The state advances
+ * exactly as with pb_substep.  For s != 1 it is pb_substep. */
+int pb_substep_stats(pb_ctx *ctx, int32_t s, double *dt_inout, pb_substep_out *out, double *p_out,
+                     double *vv_out);
 /* rk_step substeps; out may be NULL or point to rk_step records. */
 int pb_step(pb_ctx *ctx, double *dt_inout, pb_substep_out *out);
 /* Wait for the context's stream. */
@@ -153,8 +164,9 @@ int pb_wavespace_host(pb_ctx *ctx, int32_t s, double dt, double *U, double *rk,
 
 /* y_to_x + dft_x_realspace + x_to_y + rescale_data for the four state
  * fields u, v, w, theta in U(ny,nx/2,nz,4): writes the product spectra
- * P(ny,nx/2,nz,10) in the reference slot order (slot 5 = v*v only when
- * want_vv != 0, slot 9 = v).  velmax receives the CFL maximum. */
+ * P(ny,nx/2,nz,10) in the reference slot order, slots 0-8 (slot 5 = v*v
+ * only when want_vv != 0; slot 9, the reference's copy of v, is not
+ * written).  velmax receives the CFL maximum. */
 int pb_nonlinear_host(pb_ctx *ctx, const double *U, double *P, int32_t want_vv,
                       double *velmax);
 

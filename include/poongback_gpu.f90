@@ -75,6 +75,16 @@ module poongback_gpu
        type(pb_substep_out), intent(out) :: out
      end function pb_substep
 
+     integer(c_int) function pb_substep_stats(ctx, s, dt, out, p, vv) bind(C, name="pb_substep_stats")
+       import :: c_int, c_int32_t, c_double, c_ptr, c_double_complex, pb_substep_out
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: s
+       real(c_double), intent(inout) :: dt
+       type(pb_substep_out), intent(out) :: out
+       complex(c_double_complex), intent(out) :: p(*)
+       real(c_double), intent(out) :: vv(*)
+     end function pb_substep_stats
+
      integer(c_int) function pb_sync(ctx) bind(C, name="pb_sync")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

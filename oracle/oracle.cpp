@@ -890,6 +890,56 @@ static cd interp_at(const Ctx &c, const cd *v, int first, const std::vector<doub
   return acc;
 }
 
+// solve_ppe (get_pressure.f90:55-133) for one (kx,kz) != (0,0) line:
+// u1..u5 = u^2-v^2, uv, w^2-v^2, vw, uw are read; u6 = v^2 on entry, p on
+// exit; bc1, bc2 = the Neumann values of the two walls.
+static void solve_ppe(Line &L, const cd *u1, const cd *u2, const cd *u3, const cd *u4, const cd *u5, cd *u6,
+                      cd bc1, cd bc2, cd *buf1, cd *buf2, double kx, double kz) {
+  const int n = L.n, bw = L.bw, W = 2 * bw + 1;
+  const cd ikx = C(0.0, kx), ikz = C(0.0, kz);  // dcmplx(0d0, k)
+  const double k2 = kx * kx + kz * kz, kxkz = kx * kz;
+  for (int i = 1; i < n - 1; ++i)
+    buf1[i] = add(add(add(mulr(u1[i], kx * kx), mulr(u3[i], kz * kz)), mulr(u6[i], k2)), mulr(u5[i], 2.0 * kxkz));
+  labs_bs(L.D0lu, n, bw, u6);
+  labs_mv(L.D2, n, bw, u6, buf2);
+  for (int i = 1; i < n - 1; ++i) {
+    buf1[i] = sub(buf1[i], buf2[i]);
+    u6[i] = add(mul(ikx, u2[i]), mul(ikz, u4[i]));  // rows 0 and n-1 keep the coefficients above
+  }
+  labs_bs(L.D0lu, n, bw, u6);
+  labs_mv(L.D1, n, bw, u6, buf2);
+  buf1[0] = bc1;
+  for (int i = 1; i < n - 1; ++i) buf1[i] = sub(buf1[i], mulr(buf2[i], 2.0));
+  buf1[n - 1] = bc2;
+  double *Db = L.Dbuf.data();
+  for (int i = 0; i < W; ++i) Db[i] = L.D1[i];
+  for (int j = 1; j < n - 1; ++j)
+    for (int i = 0; i < W; ++i) {
+      const size_t q = (size_t)j * W + i;
+      Db[q] = -(k2 * L.D0[q]) + L.D2[q];
+    }
+  const size_t e = (size_t)(n - 1) * W;
+  for (int i = 0; i < W; ++i) Db[e + i] = L.D1[e + i];
+  labs_lu(Db, n, bw);
+  labs_bs(Db, n, bw, buf1);
+  labs_mv(L.D0, n, bw, buf1, u6);
+}
+
+// Wall values of the pressure Poisson equation from v
+// (get_statistics.f90:689-716): c = LU(D0)^-1 v,
+// bc1 = nu*(D2 row 0 . c_0..2), bc2 = nu*(D2 row n-1 . c_n-3..n-1).
+static void ppe_wall_values(const Ctx &c, const cd *v, cd *bc1, cd *bc2) {
+  const int n = c.ny, bw = c.bw, W = 2 * bw + 1;
+  std::vector<cd> t(v, v + n);
+  labs_bs(c.d0lu.data(), n, bw, t.data());
+  const double *D2 = c.d2.data();
+  const double nu = c.cfg.nu;
+  *bc1 = mulr(add(add(mulr(t[0], D2[bw]), mulr(t[1], D2[bw + 1])), mulr(t[2], D2[bw + 2])), nu);
+  const size_t e = (size_t)(n - 1) * W;
+  *bc2 = mulr(add(add(mulr(t[n - 3], D2[e + bw - 2]), mulr(t[n - 2], D2[e + bw - 1])), mulr(t[n - 1], D2[e + bw])),
+              nu);
+}
+
 }  // namespace
 
 // ======================================================================= ABI
@@ -1295,6 +1345,54 @@ int orc_substep(void *h, double *Sd, double *RKd, int s, double *dt_inout, doubl
   if (out6) {
     out6[0] = dt; out6[1] = velmax; out6[2] = mean[0]; out6[3] = mean[1];
     out6[4] = mean[2]; out6[5] = mean[3];
+  }
+  return 0;
+}
+
+// Statistics-time pressure (get_statistics_1 -> get_uv(v, v) and the PPE
+// wall values, then get_pressure.f90 after the products).
+//   Sd: state S(ny,nxh,nz,4) at the start of the substep (v = slot 1)
+//   Pd: products P(ny,nxh,nz,10) of orc_nonlinear(want_vv = 1); slot 5
+//       (v^2) is replaced by p
+//   vv: <vv>(y), ny doubles (get_statistics.f90:982-1017: modes in
+//       kz-major, kx-minor order, kx = 0 once, kx > 0 twice)
+int orc_pressure(void *h, const double *Sd, double *Pd, double *vv) {
+  Ctx *c = static_cast<Ctx *>(h);
+  const int n = c->ny;
+  const size_t nl = (size_t)c->nxh * c->nz;
+  const cd *S = reinterpret_cast<const cd *>(Sd);
+  cd *P = reinterpret_cast<cd *>(Pd);
+  std::vector<double> acc(n, 0.0);
+  for (size_t l = 0; l < nl; ++l) {
+    const int k = (int)(l % c->nxh), j = (int)(l / c->nxh);
+    const cd *v = S + ix(*c, 1, j, k, 0);
+    for (int y = 0; y < n; ++y) {
+      if (k == 0)
+        acc[y] = acc[y] + v[y].re * v[y].re + v[y].im * v[y].im;
+      else
+        acc[y] = acc[y] + 2.0 * (v[y].re * v[y].re + v[y].im * v[y].im);
+    }
+  }
+  if (vv) std::memcpy(vv, acc.data(), n * sizeof(double));
+#pragma omp parallel
+  {
+    Line L{c, n, c->bw, c->d0lu.data(), c->d0.data(), c->d1.data(), c->d2.data(),
+           std::vector<double>((size_t)n * (2 * c->bw + 1)), 0.0, 1};
+    std::vector<cd> b1(n), b2(n);
+#pragma omp for schedule(static)
+    for (long l = 0; l < (long)nl; ++l) {
+      const int k = (int)(l % c->nxh), j = (int)(l / c->nxh);
+      cd *u6 = P + ix(*c, 5, j, k, 0);
+      if (c->kx[k] == 0 && c->kz[j] == 0) {
+        for (int y = 0; y < n; ++y) u6[y] = C(-acc[y], 0.0);
+        continue;
+      }
+      cd bc1, bc2;
+      ppe_wall_values(*c, S + ix(*c, 1, j, k, 0), &bc1, &bc2);
+      solve_ppe(L, P + ix(*c, 0, j, k, 0), P + ix(*c, 1, j, k, 0), P + ix(*c, 2, j, k, 0), P + ix(*c, 3, j, k, 0),
+                P + ix(*c, 4, j, k, 0), u6, bc1, bc2, b1.data(), b2.data(), c->kx[k] * c->lmfxr,
+                c->kz[j] * c->lmfzr);
+    }
   }
   return 0;
 }

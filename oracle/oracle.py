@@ -43,6 +43,7 @@ def lib():
         L.orc_wavespace.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp,
                                     ctypes.c_int, ctypes.c_int, _dp]
         L.orc_substep.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_pressure.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.orc_labs_setup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_double, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
         L.orc_bspline_row.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -168,6 +169,13 @@ class Oracle:
         lib().orc_wavespace(self.h, s, dt, P.ctypes.data_as(_dp), RK.ctypes.data_as(_dp),
                             None if vd is None else vd.ctypes.data_as(_dp), 0, cfg.nxh * cfg.nz, _p(mean))
         return mean
+
+    def pressure(self, S: np.ndarray, P: np.ndarray) -> np.ndarray:
+        """get_pressure on the products P (10 slots, v*v in slot 5) of the
+        state S: slot 5 of P becomes p; returns <vv>(y)."""
+        vv = np.zeros(self.cfg.ny)
+        lib().orc_pressure(self.h, S.ctypes.data_as(_dp), P.ctypes.data_as(_dp), _p(vv))
+        return vv
 
     def substep(self, S: np.ndarray, RK: np.ndarray, s: int, dt: float):
         out = np.zeros(6)

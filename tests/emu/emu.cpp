@@ -127,3 +127,66 @@ extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, doubl
     }
   return 0;
 }
+
+// Opposition-control detection planes with the windowed device sweep.
+// V: the v field in the device layout [y][kz][kx]; out: 2 planes [kz][kx].
+template <int BW>
+static void opp_all(const pb::LineParams &P, pb::cplx *V, pb::cplx *y, size_t plane, int f0, int f1,
+                    const double *rows, int k, pb::cplx *out) {
+  for (size_t sys = 0; sys < plane; ++sys) {
+    const pb::SV v{V + sys, plane}, ys{y + sys, plane};
+    pb::opp_line<BW>(P, v, ys, f0, f1, rows, k, out + sys, out + plane + sys);
+  }
+}
+extern "C" int emu_opp_planes(const pb_config *cfg, double *V, double *out) {
+  Setup S;
+  if (make_setup(cfg, S)) return -1;
+  pb::LineParams P = params(cfg, S, 1, 0.0);
+  int f0 = 0, f1 = 0;
+  std::vector<double> lo, hi;
+  if (pb::basis_row(S.col, -1.0 + cfg->opp_ydist, f0, lo) || pb::basis_row(S.col, 1.0 - cfg->opp_ydist, f1, hi))
+    return -3;
+  lo.insert(lo.end(), hi.begin(), hi.end());
+  const size_t plane = (size_t)(cfg->nz) * (cfg->nx / 2);
+  std::vector<pb::cplx> y(plane * S.n);
+  pb::cplx *Vc = (pb::cplx *)V, *o = (pb::cplx *)out;
+  if (S.order > 2 * S.bw + 1 || f0 > f1) return -4;
+  switch (S.bw) {
+    case 3: opp_all<3>(P, Vc, y.data(), plane, f0, f1, lo.data(), S.order, o); break;
+    case 4: opp_all<4>(P, Vc, y.data(), plane, f0, f1, lo.data(), S.order, o); break;
+    case 5: opp_all<5>(P, Vc, y.data(), plane, f0, f1, lo.data(), S.order, o); break;
+    default: return -2;
+  }
+  return 0;
+}
+
+// Statistics-time pressure with the product's per-system code, device
+// layout: V = v field [y][kz][kx]; U = 9 product slots (slot 5: v*v in, p
+// out); vv = <vv>(y).
+extern "C" int emu_pressure(const pb_config *cfg, double *V, double *U, double *vv) {
+  Setup S;
+  if (make_setup(cfg, S)) return -1;
+  pb::LineParams P = params(cfg, S, 1, 0.0);
+  const int nxh = (int)cfg->nx / 2, nz = (int)cfg->nz, nh = nz / 2;
+  const size_t plane = (size_t)nz * nxh, field = plane * S.n;
+  pb::cplx *Vc = (pb::cplx *)V, *Uc = (pb::cplx *)U;
+  for (int y = 0; y < S.n; ++y) vv[y] = pb::mean_vv_row(Vc + (size_t)y * plane, nxh, plane);
+  std::vector<pb::cplx> b1(field), b2(field);
+  std::vector<double> db((size_t)S.W * field);
+  for (size_t sys = 0; sys < plane; ++sys) {
+    const int k = (int)(sys % nxh), j = (int)(sys / nxh);
+    const int kzi = (j <= nh) ? j : j + 1 - 2 * nh - 1;
+    auto col = [&](int slot) { return pb::SV{Uc + (size_t)slot * field + sys, plane}; };
+    const pb::SV t1{b1.data() + sys, plane}, t2{b2.data() + sys, plane};
+    if (k == 0 && kzi == 0) {
+      for (int y = 0; y < S.n; ++y) col(5)[y] = pb::mk(-vv[y], 0.0);
+      continue;
+    }
+    pb::cplx bc1, bc2;
+    pb::ppe_wall_values(P, pb::SV{Vc + sys, plane}, t1, cfg->nu, &bc1, &bc2);
+    const pb::OwnBand Db{db.data() + sys, plane, S.W};
+    pb::solve_ppe(P, Db, col(0), col(1), col(2), col(3), col(4), col(5), bc1, bc2, t1, t2, k * (1.0 / cfg->lmfx),
+                  kzi * (1.0 / cfg->lmfz));
+  }
+  return 0;
+}

@@ -72,3 +72,42 @@ def test_fused_wavespace_bitwise_cpu(oracle_lib, emu_lib, name, alpha, extra):
         assert np.array_equal(m_e, m_o)
         for i, f in enumerate((0, 1, 2, 6)):
             S[i] = P[f]
+
+
+@pytest.mark.parametrize("name,extra", [("tiny", {}), ("re180_64", {}),
+                                        ("tiny", {"knot_map": 0, "stretch": 0.0, "labs_k": 6, "ny": 34}),
+                                        ("tiny", {"ny": 65, "labs_k": 9})])
+def test_opp_planes_bitwise_cpu(oracle_lib, emu_lib, name, extra):
+    """The windowed detection-plane sweep (solver_fused.h opp_line) gives the
+    oracle's v(y_d) (LU(D0) solve, then the B-spline row) bit for bit."""
+    cfg = dataclasses.replace(preset(name), opp_alpha=0.7, opp_yplus=20.0, **extra)
+    orc = oracle_lib.Oracle(cfg)
+    S = make_state(cfg, seed=29)
+    vd = orc.opp_planes(S)
+    V = np.ascontiguousarray(np.transpose(S[1], (2, 0, 1)))  # (nz, nxh, ny) -> [y][kz][kx]
+    out = np.zeros((2, cfg.nz, cfg.nxh), np.complex128)
+    c = cfg.to_c()
+    assert emu_lib.emu_opp_planes(ctypes.byref(c), V.ctypes.data_as(dp), out.ctypes.data_as(dp)) == 0
+    assert np.array_equal(out, vd), np.abs(out - vd).max()
+
+
+@pytest.mark.parametrize("name", ["tiny", "re180_64", "re180_192"])
+def test_pressure_bitwise_cpu(oracle_lib, emu_lib, name):
+    """The product's pressure code (solver.h ppe_wall_values, solve_ppe,
+    mean_vv_row) in the device layout equals the oracle bit for bit."""
+    cfg = preset(name)
+    orc = oracle_lib.Oracle(cfg)
+    S = make_state(cfg, seed=31)
+    RK = orc.init_stores(S)
+    orc.step(S, RK, cfg.dt)
+    P, _ = orc.nonlinear(S, want_vv=True)
+    Ud = np.ascontiguousarray(np.transpose(P[:9], (0, 3, 1, 2)))
+    V = np.ascontiguousarray(np.transpose(S[1], (2, 0, 1)))
+    vv_o = orc.pressure(S, P)
+    vv_e = np.zeros(cfg.ny)
+    c = cfg.to_c()
+    assert emu_lib.emu_pressure(ctypes.byref(c), V.ctypes.data_as(dp), Ud.ctypes.data_as(dp),
+                                vv_e.ctypes.data_as(dp)) == 0
+    assert np.array_equal(vv_e, vv_o)
+    pe = np.transpose(Ud[5], (1, 2, 0))
+    assert np.array_equal(pe, P[5]), np.abs(pe - P[5]).max()

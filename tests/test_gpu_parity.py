@@ -31,7 +31,8 @@ def configs():
     opp = preset("re180_192")
     thin385 = dataclasses.replace(preset("re1000"), nx=16, nz=12, name="re1000_thin")  # ny = 385
     tiny_sin = dataclasses.replace(preset("tiny"), knot_map=0, stretch=0.9, labs_k=6, name="tiny_sin")
-    return {"re180_64": small, "re180_192": opp, "re1000_thin": thin385, "tiny_sin": tiny_sin}
+    flat = dataclasses.replace(preset("re1000"), ny=33, name="re1000_flat")  # production x/z transform sizes
+    return {"re180_64": small, "re180_192": opp, "re1000_thin": thin385, "tiny_sin": tiny_sin, "re1000_flat": flat}
 
 
 CFGS = configs()
@@ -85,10 +86,10 @@ def test_nonlinear_stage(lib, oracle_lib, name):
     cfg = CFGS[name]
     orc = oracle_lib.Oracle(cfg)
     S, RK = developed(cfg, orc)
-    P_ref, vm_ref = orc.nonlinear(S)
+    P_ref, vm_ref = orc.nonlinear(S, want_vv=True)
     ch = lib.Channel(cfg)
-    P, vm = ch.nonlinear_host(S)
-    for slot in (0, 1, 2, 3, 4, 6, 7, 8):
+    P, vm = ch.nonlinear_host(S, want_vv=True)
+    for slot in (0, 1, 2, 3, 4, 5, 6, 7, 8):
         e = rel(P[slot], P_ref[slot])
         print(f"{name} slot {slot}: {e:.2e}")
         assert e <= FFT_TOL, (slot, e)
@@ -96,6 +97,68 @@ def test_nonlinear_stage(lib, oracle_lib, name):
     assert not np.any(P[:9, cfg.nz // 2])
     assert abs(vm - vm_ref) <= 1e-13 * vm_ref
     ch.close()
+
+
+def full_spectrum(cfg, seed):
+    """Random state with every retained mode populated (amplitude falling
+    off with |k|), Hermitian on kx = 0, kz Nyquist slot empty."""
+    rng = np.random.default_rng(seed)
+    nh = cfg.nz // 2
+    shape = (4, cfg.nz, cfg.nxh, cfg.ny)
+    S = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    kx = np.arange(cfg.nxh)[None, None, :, None]
+    kz = np.abs(np.fft.fftfreq(cfg.nz, 1.0 / cfg.nz))[None, :, None, None]
+    S *= 1.0 / (1.0 + 0.05 * (kx + kz))
+    S[:, nh] = 0.0
+    S[:, cfg.nz - 1:nh:-1, 0] = np.conj(S[:, 1:nh, 0])  # kz index nz-j holds -j
+    S[:, 0, 0] = S[:, 0, 0].real
+    return np.ascontiguousarray(S)
+
+
+@pytest.mark.parametrize("name", ["re180_192", "re1000_flat"])
+def test_nonlinear_full_spectrum(lib, oracle_lib, name):
+    """All modes populated: exercises every twiddle and radix pass, at the
+    production transform sizes 1536 (x) and 1152 (z) for re1000_flat."""
+    cfg = CFGS[name]
+    orc = oracle_lib.Oracle(cfg)
+    S = full_spectrum(cfg, seed=5)
+    P_ref, vm_ref = orc.nonlinear(S)
+    ch = lib.Channel(cfg)
+    P, vm = ch.nonlinear_host(S)
+    for slot in (0, 1, 2, 3, 4, 6, 7, 8):
+        e = rel(P[slot], P_ref[slot])
+        print(f"{name} full spectrum slot {slot}: {e:.2e}", flush=True)
+        assert e <= FFT_TOL, (slot, e)
+    assert not np.any(P[:9, cfg.nz // 2])
+    assert abs(vm - vm_ref) <= 1e-13 * vm_ref
+    ch.close()
+
+
+@pytest.mark.parametrize("name", ["re180_64", "re180_192", "re1000_thin"])
+def test_pressure_stats(lib, oracle_lib, name):
+    """Statistics-time substep: <vv> bitwise, p (get_pressure) against the
+    oracle on the oracle's own products, and the state advanced exactly as
+    by the plain substep."""
+    cfg = CFGS[name]
+    orc = oracle_lib.Oracle(cfg)
+    S, RK = developed(cfg, orc)
+    P, _ = orc.nonlinear(S, want_vv=True)
+    vv_ref = orc.pressure(S, P)
+    ch = lib.Channel(cfg)
+    ch.set_state(S, RK)
+    out, p, vv = ch.substep_stats(1, cfg.dt)
+    assert np.array_equal(vv, vv_ref)
+    e = rel(p, P[5])
+    print(f"{name} pressure rel L2 {e:.2e}", flush=True)
+    assert e <= STEP_TOL
+    S1, RK1 = ch.get_state()
+    ch.close()
+    ch = lib.Channel(cfg)
+    ch.set_state(S, RK)
+    out2 = ch.substep(1, cfg.dt)
+    S2, RK2 = ch.get_state()
+    ch.close()
+    assert np.array_equal(S1, S2) and np.array_equal(RK1, RK2) and out == out2
 
 
 @pytest.mark.parametrize("name", ["re180_64", "re180_192", "re1000_thin", "tiny_sin"])
