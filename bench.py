@@ -23,6 +23,26 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 STAGES = ["oppvd", "zinv", "xphys", "velmax", "zfwd", "wavespace"]
+# stages that are a single kernel each (the wall-normal stage is a chain of
+# sweep kernels on three streams); the roofline object names the single
+# kernel with the largest time per step
+KERNEL_OF_STAGE = {"oppvd": "opp_kernel", "zinv": "zinv_kernel", "xphys": "xphys_kernel", "zfwd": "zfwd_kernel"}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "re1000_1024x385x768_hbm_traffic.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (already scaled by the copy calibration),
+    or None when the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if parts and parts[0].startswith(kernel) and len(parts) >= 6:
+                    return (float(parts[-4]) + float(parts[-3])) * 1e9
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def algorithmic_bytes(cfg):
@@ -114,8 +134,11 @@ def main():
     stage = {STAGES[i]: {"ms_total": ms[i], "launches": int(cnt[i]),
                          "ms_mean": (ms[i] / cnt[i]) if cnt[i] else 0.0} for i in range(6)}
     alg = algorithmic_bytes(cfg)
-    dom = max(STAGES, key=lambda s: stage[s]["ms_total"])
+    dom = max(KERNEL_OF_STAGE, key=lambda s: stage[s]["ms_total"])
     achieved = alg[dom] / (stage[dom]["ms_mean"] * 1e-3) / 1e9
+    traffic = measured_traffic(KERNEL_OF_STAGE[dom]) if cfg.name == "re1000" else None
+    stage_frac = {s: round(alg[s] / (stage[s]["ms_mean"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                  for s in STAGES if alg[s] > 0 and stage[s]["ms_mean"] > 0}
     step_bytes = roofline_bytes_per_step(cfg)
     print(f"[bench] {args.steps} steps in {elapsed:.3f} s; stages: " +
           ", ".join(f"{s} {stage[s]['ms_mean']:.3f} ms" for s in STAGES), file=sys.stderr, flush=True)
@@ -124,7 +147,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         from oracle.oracle import sample_times
-        nplanes, nlines = 16, 4096
+        nplanes, nlines = 96, 131072
         tn, tw, nt = sample_times(cfg, nplanes, nlines)
         t_step = 3 * (tn * cfg.ny / nplanes + tw * (cfg.nxh * cfg.nz) / nlines)
         cpu = {"value": 1.0 / t_step, "unit": "steps/s", "cores": nt, "kind": "port",
@@ -148,10 +171,12 @@ def main():
         "config": {"workload": f"channel Re_tau={cfg.re_tau:g} {cfg.nx}x{cfg.ny}x{cfg.nz}, opposition control "
                                f"alpha={cfg.opp_alpha} at y+={cfg.opp_yplus:g}, Pr={cfg.pr}",
                    "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "dt": cfg.dt},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": KERNEL_OF_STAGE[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": alg[dom],
                      "step_frac": step_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS},
         "stages_ms": {s: round(stage[s]["ms_mean"], 4) for s in STAGES},
+        "stages_roofline_frac": stage_frac,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -272,7 +272,25 @@ __device__ inline Blk xcd_block() {
 // The same Stockham passes with N, B, the radices and the direction known at
 // compile time: index arithmetic folds to shifts and multiplies, the first
 // pass needs no twiddles, and every register array has its exact size.
-template <int T, int N, int B, int SIGN, int Ns, int R>
+// LDS element index of the fixed-size transforms: an XOR swizzle of the
+// low 3 bits of the 16-byte element index by bits 4..6, so that the strided
+// stores of the first Stockham pass (and every other access pattern of the
+// kernels, which are consecutive per lane) spread over all bank groups.
+__device__ __forceinline__ int lds_swz(int i) { return i ^ ((i >> 4) & 7); }
+
+// Layout of B lines of N elements in LDS: interleaved (element n of line b
+// at n*B + b, the z transforms, whose global rows hold B consecutive kx) or
+// separate (b*N + n, the pair of x lines).
+template <int N, int B, bool SEP>
+struct Lines {
+  __device__ static int at(int n, int b) { return lds_swz(SEP ? b * N + n : n * B + b); }
+  // butterfly t of a pass with N/R butterflies per line -> (line, index)
+  __device__ static void split(int t, int NR, int &b, int &j) {
+    if (SEP) { b = t / NR; j = t % NR; } else { b = t % B; j = t / B; }
+  }
+};
+
+template <int T, class L, int N, int B, int SIGN, int Ns, int R>
 __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ tw) {
   constexpr int NR = N / R, NBF = NR * B, ROUNDS = (NBF + T - 1) / T;
   constexpr int tstep = N / (Ns * R);
@@ -281,9 +299,10 @@ __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ 
   for (int q = 0; q < ROUNDS; ++q) {
     const int t = threadIdx.x + q * T;
     if (NBF % T == 0 || t < NBF) {
-      const int b = t % B, j = t / B;
+      int b, j;
+      L::split(t, NR, b, j);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = x[(j + r * NR) * B + b];
+      for (int r = 0; r < R; ++r) v[q][r] = x[L::at(j + r * NR, b)];
     }
   }
   __syncthreads();
@@ -291,7 +310,8 @@ __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ 
   for (int q = 0; q < ROUNDS; ++q) {
     const int t = threadIdx.x + q * T;
     if (NBF % T == 0 || t < NBF) {
-      const int b = t % B, j = t / B;
+      int b, j;
+      L::split(t, NR, b, j);
       const int k = j % Ns;
       if (Ns > 1) {
 #pragma unroll
@@ -304,36 +324,44 @@ __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ 
       dft<R>(v[q], SIGN);
       const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[(base + r * Ns) * B + b] = v[q][r];
+      for (int r = 0; r < R; ++r) x[L::at(base + r * Ns, b)] = v[q][r];
     }
   }
   __syncthreads();
 }
-template <int T, int N, int B, int SIGN, int Ns, int R, int... Rest>
+template <int T, class L, int N, int B, int SIGN, int Ns, int R, int... Rest>
 __device__ __forceinline__ void ct_passes(double2 *x, const double2 *__restrict__ tw) {
-  ct_pass<T, N, B, SIGN, Ns, R>(x, tw);
-  if constexpr (sizeof...(Rest) > 0) ct_passes<T, N, B, SIGN, Ns * R, Rest...>(x, tw);
+  ct_pass<T, L, N, B, SIGN, Ns, R>(x, tw);
+  if constexpr (sizeof...(Rest) > 0) ct_passes<T, L, N, B, SIGN, Ns * R, Rest...>(x, tw);
 }
 
 // Transform policies for the three kernels: N = B = 0 means sizes at run
-// time (any 2,3,5-smooth length, generic passes); otherwise fixed sizes.
+// time (any 2,3,5-smooth length, generic passes, lines interleaved, no
+// swizzle); otherwise fixed sizes.  at(n, b) is the LDS index of element n
+// of line b.
 template <int EPT>
 struct RtFft {
-  static constexpr int T = FFT_THREADS, N = 0, B = 0, ELEMS = EPT;
+  static constexpr int T = FFT_THREADS, N = 0, B = 0, ELEMS = EPT, MINW = 1;
+  __device__ static int at(int n, int b, int B_) { return n * B_ + b; }
   __device__ static void run(double2 *x, int B_, const FftPlan &P, const double2 *__restrict__ tw, int sign) {
     lds_fft<EPT>(x, B_, P, tw, sign);
   }
 };
-template <int T_, int N_, int B_, int... R>
+template <int T_, int MINW_, int N_, int B_, bool SEP, int... R>
 struct CtFft {
-  static constexpr int T = T_, N = N_, B = B_, ELEMS = (N_ * B_ + T_ - 1) / T_;
+  // MINW: waves per SIMD the register budget must allow (the LDS of one
+  // block limits how many blocks share a CU; more waves than that fit is
+  // register space wasted)
+  static constexpr int T = T_, N = N_, B = B_, ELEMS = (N_ * B_ + T_ - 1) / T_, MINW = MINW_;
+  using L = Lines<N_, B_, SEP>;
   static_assert((R * ...) == N_, "radices must multiply to N");
-  static_assert(T_ % 64 == 0 && T_ <= 1024, "whole waves");
+  static_assert(T_ % 32 == 0 && T_ <= 1024, "block size");
+  __device__ static int at(int n, int b, int) { return L::at(n, b); }
   __device__ static void run(double2 *x, int, const FftPlan &, const double2 *__restrict__ tw, int sign) {
     if (sign > 0)
-      ct_passes<T_, N_, B_, +1, 1, R...>(x, tw);
+      ct_passes<T_, L, N_, B_, +1, 1, R...>(x, tw);
     else
-      ct_passes<T_, N_, B_, -1, 1, R...>(x, tw);
+      ct_passes<T_, L, N_, B_, -1, 1, R...>(x, tw);
   }
 };
 // 3/2-padded production sizes.  nz32 = 1152 with 4 kx columns per block:
@@ -341,10 +369,10 @@ struct CtFft {
 // thread per pass, 256 of 576 in the first).  nx32 = 1536, the two lines of
 // a pair: 256 threads, radix 16, 16, 6.
 #ifndef PB_FFTZ1152
-#define PB_FFTZ1152 CtFft<576, 1152, 4, 18, 8, 8>
+#define PB_FFTZ1152 CtFft<576, 3, 1152, 4, false, 18, 8, 8>
 #endif
 #ifndef PB_FFTX1536
-#define PB_FFTX1536 CtFft<256, 1536, 2, 16, 16, 6>
+#define PB_FFTX1536 CtFft<256, 2, 1536, 2, true, 16, 16, 6>
 #endif
 using FftZ1152 = PB_FFTZ1152;
 using FftX1536 = PB_FFTX1536;
@@ -353,7 +381,7 @@ using FftX1536 = PB_FFTX1536;
 // grid: (nxh/B, ny, nfield); the 4 state fields u, v, w, theta live in U
 // slots 0,1,2,6.  Output pad[f][y][z][kx].
 template <class F>
-__global__ void __launch_bounds__(F::T) zinv_kernel(const double2 *__restrict__ U, double2 *__restrict__ pad,
+__global__ void __launch_bounds__(F::T, F::MINW) zinv_kernel(const double2 *__restrict__ U, double2 *__restrict__ pad,
                                                            FftPlan P, const double2 *__restrict__ tw, int nxh, int nz,
                                                            int ny, int B_, size_t field, size_t pfield) {
   extern __shared__ double2 lds[];
@@ -368,14 +396,14 @@ __global__ void __launch_bounds__(F::T) zinv_kernel(const double2 *__restrict__ 
     double2 val = c2(0.0, 0.0);
     if (z < nh) val = src[(size_t)z * nxh + b];
     else if (z > 2 * nh) val = src[(size_t)(z - nh) * nxh + b];
-    lds[e] = val;
+    lds[F::at(z, b, B)] = val;
   }
   __syncthreads();
   F::run(lds, B, P, tw, +1);
   double2 *dst = pad + (size_t)f * pfield + (size_t)y * N * nxh + kx0;
   for (int e = threadIdx.x; e < N * B; e += F::T) {
     const int z = e / B, b = e % B;
-    dst[(size_t)z * nxh + b] = lds[e];
+    dst[(size_t)z * nxh + b] = lds[F::at(z, b, B)];
   }
 }
 
@@ -383,12 +411,12 @@ __global__ void __launch_bounds__(F::T) zinv_kernel(const double2 *__restrict__ 
 // grid: (nz32, ny): one x line per block.  pad fields 0..3 = u,v,w,theta in,
 // 0..7 = spectra of u^2-v^2, uv, w^2-v^2, vw, uw, u theta, v theta, w theta out.
 template <class F>
-__global__ void __launch_bounds__(F::T) xphys_kernel(double2 *__restrict__ pad, FftPlan P,
+__global__ void __launch_bounds__(F::T, F::MINW) xphys_kernel(double2 *__restrict__ pad, FftPlan P,
                                                             const double2 *__restrict__ tw, int nxh, int nz32,
                                                             size_t pfield, const double *__restrict__ dyr,
                                                             double dxr, double dzr, double *__restrict__ vpart,
                                                             int nround) {
-  extern __shared__ double2 lds[];  // [n][2]
+  extern __shared__ double2 lds[];  // 2 lines of N, layout F::at
   __shared__ double red[F::T];
   const int z = blockIdx.x, y = blockIdx.y, N = F::N ? F::N : P.N;
   double2 *line = pad + (size_t)y * nz32 * nxh + (size_t)z * nxh;
@@ -408,8 +436,8 @@ __global__ void __launch_bounds__(F::T) xphys_kernel(double2 *__restrict__ pad, 
       z1 = c2(a.x + b.y, -a.y + b.x);
       z2 = c2(c.x + d.y, -c.y + d.x);
     }
-    lds[2 * m] = z1;
-    lds[2 * m + 1] = z2;
+    lds[F::at(m, 0, 2)] = z1;
+    lds[F::at(m, 1, 2)] = z2;
   }
   __syncthreads();
   F::run(lds, 2, P, tw, +1);
@@ -422,7 +450,8 @@ __global__ void __launch_bounds__(F::T) xphys_kernel(double2 *__restrict__ pad, 
   for (int q = 0; q < MAXP; ++q) {
     const int m = threadIdx.x + q * F::T;
     if (m < N) {
-      pu[q] = lds[2 * m].x; pv[q] = lds[2 * m].y; pw[q] = lds[2 * m + 1].x; pt[q] = lds[2 * m + 1].y;
+      const double2 a = lds[F::at(m, 0, 2)], c = lds[F::at(m, 1, 2)];
+      pu[q] = a.x; pv[q] = a.y; pw[q] = c.x; pt[q] = c.y;
       const double vel = (fabs(pu[q]) * dxr) + (fabs(pv[q]) * dy) + (fabs(pw[q]) * dzr);
       vmax = fmax(vel, vmax);
     }
@@ -449,14 +478,14 @@ __global__ void __launch_bounds__(F::T) xphys_kernel(double2 *__restrict__ pad, 
         const double t1 = pu[q], t2 = pv[q], t3 = pw[q], t4 = pt[q];
         const double bv2 = t2 * t2;
         if (round == 0) {
-          lds[2 * m] = c2(t1 * t1 - bv2, t1 * t2);
-          lds[2 * m + 1] = c2(t3 * t3 - bv2, t2 * t3);
+          lds[F::at(m, 0, 2)] = c2(t1 * t1 - bv2, t1 * t2);
+          lds[F::at(m, 1, 2)] = c2(t3 * t3 - bv2, t2 * t3);
         } else if (round == 1) {
-          lds[2 * m] = c2(t1 * t3, t1 * t4);
-          lds[2 * m + 1] = c2(t2 * t4, t3 * t4);
+          lds[F::at(m, 0, 2)] = c2(t1 * t3, t1 * t4);
+          lds[F::at(m, 1, 2)] = c2(t2 * t4, t3 * t4);
         } else {
-          lds[2 * m] = c2(bv2, 0.0);
-          lds[2 * m + 1] = c2(0.0, 0.0);
+          lds[F::at(m, 0, 2)] = c2(bv2, 0.0);
+          lds[F::at(m, 1, 2)] = c2(0.0, 0.0);
         }
       }
     }
@@ -467,7 +496,7 @@ __global__ void __launch_bounds__(F::T) xphys_kernel(double2 *__restrict__ pad, 
       const int q = (m == 0) ? 0 : N - m;
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        const double2 Z = lds[2 * m + b], Zc = lds[2 * q + b];
+        const double2 Z = lds[F::at(m, b, 2)], Zc = lds[F::at(q, b, 2)];
         const double2 A = c2((Z.x + Zc.x) * 0.5, (Z.y - Zc.y) * 0.5);
         const double2 Bv = c2((Z.y + Zc.y) * 0.5, (Zc.x - Z.x) * 0.5);
         const int fa = round * 4 + 2 * b;
@@ -487,7 +516,7 @@ __global__ void __launch_bounds__(F::T) xphys_kernel(double2 *__restrict__ pad, 
 // grid: (nxh/B, ny, 8 or 9).  pad field g -> U slot (0,1,2,3,4,6,7,8,5)[g],
 // with truncation and rescale.
 template <class F>
-__global__ void __launch_bounds__(F::T) zfwd_kernel(const double2 *__restrict__ pad, double2 *__restrict__ U,
+__global__ void __launch_bounds__(F::T, F::MINW) zfwd_kernel(const double2 *__restrict__ pad, double2 *__restrict__ U,
                                                            FftPlan P, const double2 *__restrict__ tw, int nxh, int nz,
                                                            int ny, int B_, size_t field, size_t pfield,
                                                            double rescale) {
@@ -500,7 +529,7 @@ __global__ void __launch_bounds__(F::T) zfwd_kernel(const double2 *__restrict__ 
   const double2 *src = pad + (size_t)g * pfield + (size_t)y * N * nxh + kx0;
   for (int e = threadIdx.x; e < N * B; e += F::T) {
     const int z = e / B, b = e % B;
-    lds[e] = src[(size_t)z * nxh + b];
+    lds[F::at(z, b, B)] = src[(size_t)z * nxh + b];
   }
   __syncthreads();
   F::run(lds, B, P, tw, -1);
@@ -508,10 +537,132 @@ __global__ void __launch_bounds__(F::T) zfwd_kernel(const double2 *__restrict__ 
   for (int e = threadIdx.x; e < nz * B; e += F::T) {
     const int j = e / B, b = e % B;
     double2 val;
-    if (j < nh) val = lds[j * B + b];
+    if (j < nh) val = lds[F::at(j, b, B)];
     else if (j == nh) val = c2(0.0, 0.0);
-    else val = lds[(j + nh) * B + b];
+    else val = lds[F::at(j + nh, b, B)];
     dst[(size_t)j * nxh + b] = c2(val.x * rescale, val.y * rescale);
+  }
+}
+
+// ------------------------------------ persistent z transforms (fixed size)
+// One block per CU walks a strided list of tiles (B kx columns of one y and
+// one field).  The next tile's global data is loaded into registers while
+// the current tile is transformed in LDS, so HBM keeps streaming during the
+// FFT; the twiddles are copied to LDS once, so the transform itself issues
+// no global loads (whose in-order completion counter would otherwise make
+// the first twiddle wait for the prefetch as well).  Tiles are numbered
+// kx tile fastest; each XCD takes a contiguous run of blocks, so the two
+// blocks that share a 128-byte row segment run in one L2 at the same time.
+struct ZTile {
+  int x, y, f;
+};
+__device__ __forceinline__ ZTile z_tile(size_t t, int ntx, int ny) {
+  ZTile z;
+  z.x = (int)(t % ntx);
+  const size_t r = t / ntx;
+  z.y = (int)(r % ny);
+  z.f = (int)(r / ny);
+  return z;
+}
+__device__ __forceinline__ size_t xcd_rank() {
+  const size_t G = gridDim.x;
+  return (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+}
+
+// y_to_x for the 4 state fields (slots 0,1,2,6), as zinv_kernel
+template <class F>
+__global__ void __launch_bounds__(F::T, F::MINW) zinv_persist(const double2 *__restrict__ U, double2 *__restrict__ pad,
+                                                              const double2 *__restrict__ twg, int nxh, int nz,
+                                                              int ny, size_t field, size_t pfield) {
+  constexpr int N = F::N, B = F::B, T = F::T, E = (N * B + T - 1) / T;
+  extern __shared__ double2 lds[];  // N*B data, then N twiddles
+  double2 *tw = lds + N * B;
+  for (int i = threadIdx.x; i < N; i += T) tw[i] = twg[i];
+  const int ntx = nxh / B, nh = nz / 2;
+  const size_t ntiles = (size_t)ntx * ny * 4, G = gridDim.x;
+  double2 pre[E];
+  auto load = [&](size_t t) {
+    const ZTile z = z_tile(t, ntx, ny);
+    const int slot = (z.f == 3) ? 6 : z.f;
+    const double2 *src = U + (size_t)slot * field + (size_t)z.y * nz * nxh + (size_t)z.x * B;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int idx = threadIdx.x + e * T;
+      if ((N * B) % T == 0 || idx < N * B) {
+        const int zz = idx / B, b = idx % B;
+        double2 val = c2(0.0, 0.0);
+        if (zz < nh) val = src[(size_t)zz * nxh + b];
+        else if (zz > 2 * nh) val = src[(size_t)(zz - nh) * nxh + b];
+        pre[e] = val;
+      }
+    }
+  };
+  size_t t = xcd_rank();
+  if (t < ntiles) load(t);
+  for (; t < ntiles; t += G) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int idx = threadIdx.x + e * T;
+      if ((N * B) % T == 0 || idx < N * B) lds[F::at(idx / B, idx % B, B)] = pre[e];
+    }
+    __syncthreads();
+    if (t + G < ntiles) load(t + G);
+    F::run(lds, B, FftPlan{}, tw, +1);
+    const ZTile z = z_tile(t, ntx, ny);
+    double2 *dst = pad + (size_t)z.f * pfield + (size_t)z.y * N * nxh + (size_t)z.x * B;
+    for (int e = threadIdx.x; e < N * B; e += T) {
+      const int zz = e / B, b = e % B;
+      dst[(size_t)zz * nxh + b] = lds[F::at(zz, b, B)];
+    }
+    __syncthreads();
+  }
+}
+
+// x_to_y + rescale for nf = 8 (or 9) product fields, as zfwd_kernel
+template <class F>
+__global__ void __launch_bounds__(F::T, F::MINW) zfwd_persist(const double2 *__restrict__ pad, double2 *__restrict__ U,
+                                                              const double2 *__restrict__ twg, int nxh, int nz,
+                                                              int ny, size_t field, size_t pfield, double rescale,
+                                                              int nf) {
+  constexpr int N = F::N, B = F::B, T = F::T, E = (N * B + T - 1) / T;
+  extern __shared__ double2 lds[];
+  double2 *tw = lds + N * B;
+  for (int i = threadIdx.x; i < N; i += T) tw[i] = twg[i];
+  const int ntx = nxh / B, nh = nz / 2;
+  const size_t ntiles = (size_t)ntx * ny * nf, G = gridDim.x;
+  double2 pre[E];
+  auto load = [&](size_t t) {
+    const ZTile z = z_tile(t, ntx, ny);
+    const double2 *src = pad + (size_t)z.f * pfield + (size_t)z.y * N * nxh + (size_t)z.x * B;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int idx = threadIdx.x + e * T;
+      if ((N * B) % T == 0 || idx < N * B) pre[e] = src[(size_t)(idx / B) * nxh + idx % B];
+    }
+  };
+  size_t t = xcd_rank();
+  if (t < ntiles) load(t);
+  for (; t < ntiles; t += G) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int idx = threadIdx.x + e * T;
+      if ((N * B) % T == 0 || idx < N * B) lds[F::at(idx / B, idx % B, B)] = pre[e];
+    }
+    __syncthreads();
+    if (t + G < ntiles) load(t + G);
+    F::run(lds, B, FftPlan{}, tw, -1);
+    const ZTile z = z_tile(t, ntx, ny);
+    const int slot = z.f < 5 ? z.f : (z.f < 8 ? z.f + 1 : 5);
+    double2 *dst = U + (size_t)slot * field + (size_t)z.y * nz * nxh + (size_t)z.x * B;
+    for (int e = threadIdx.x; e < nz * B; e += T) {
+      const int j = e / B, b = e % B;
+      double2 val;
+      if (j < nh) val = lds[F::at(j, b, B)];
+      else if (j == nh) val = c2(0.0, 0.0);
+      else val = lds[F::at(j + nh, b, B)];
+      dst[(size_t)j * nxh + b] = c2(val.x * rescale, val.y * rescale);
+    }
+    __syncthreads();
   }
 }
 
