@@ -272,6 +272,12 @@ __device__ inline Blk xcd_block() {
 // The same Stockham passes with N, B, the radices and the direction known at
 // compile time: index arithmetic folds to shifts and multiplies, the first
 // pass needs no twiddles, and every register array has its exact size.
+// Fixed-size passes form W^(rk) from one table value W^k by binary powering
+// (at most log2 R complex products, a few ulp), instead of R-1 table loads.
+#ifndef PB_FFT_TWPOW
+#define PB_FFT_TWPOW 1
+#endif
+
 // LDS element index of the fixed-size transforms: an XOR swizzle of the
 // low 3 bits of the 16-byte element index by bits 4..6, so that the strided
 // stores of the first Stockham pass (and every other access pattern of the
@@ -314,12 +320,23 @@ __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ 
       L::split(t, NR, b, j);
       const int k = j % Ns;
       if (Ns > 1) {
+#if PB_FFT_TWPOW
+        // one table load; W^(r k) = (W^k)^r by binary powering (depth <= log2 R)
+        double2 w[R];
+        w[1] = tw[k * tstep];
+        if (SIGN > 0) w[1].y = -w[1].y;
+#pragma unroll
+        for (int r = 2; r < R; ++r) w[r] = (r & 1) ? cmul2(w[r - 1], w[1]) : cmul2(w[r / 2], w[r / 2]);
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[q][r] = cmul2(v[q][r], w[r]);
+#else
 #pragma unroll
         for (int r = 1; r < R; ++r) {
           double2 w = tw[r * k * tstep];
           if (SIGN > 0) w.y = -w.y;
           v[q][r] = cmul2(v[q][r], w);
         }
+#endif
       }
       dft<R>(v[q], SIGN);
       const int base = (j / Ns) * Ns * R + k;
