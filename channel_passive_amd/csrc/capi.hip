@@ -82,6 +82,7 @@ struct pb_ctx {
   double *dbuf = nullptr, *dbuf2 = nullptr, *vpart = nullptr, *scal = nullptr;
   double2 *ppe_bc = nullptr;  // statistics time: pressure wall values [2][kz][kx]
   double *dvv = nullptr;      // statistics time: <vv>(y)
+  size_t vpart_n = 0;         // per-wave CFL maxima written by the last x-physical launch
   size_t mean_lds = 0;        // dynamic LDS of mean_kernel (0: one-thread wave00_kernel)
   double *mean_db = nullptr;  // mean_kernel band scratch when it does not fit in LDS
   double *dD0 = nullptr, *dD1 = nullptr, *dD2 = nullptr, *dD0LU = nullptr, *dinteg = nullptr, *ddyr = nullptr,
@@ -268,7 +269,7 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   rc = rc ? rc : dalloc((void **)&c->vd, 2 * c->plane * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->dbuf, (size_t)c->W * c->field * sizeof(double));
   rc = rc ? rc : dalloc((void **)&c->dbuf2, (size_t)c->W * c->field * sizeof(double));
-  rc = rc ? rc : dalloc((void **)&c->vpart, (size_t)c->ny * c->nz32 * sizeof(double));
+  rc = rc ? rc : dalloc((void **)&c->vpart, (size_t)c->ny * c->nz32 * 16 * sizeof(double));  // <= 16 waves/block
   rc = rc ? rc : dalloc((void **)&c->scal, SCAL_N * sizeof(double));
   rc = rc ? rc : dalloc((void **)&c->dD0, bandb);
   rc = rc ? rc : dalloc((void **)&c->dD1, bandb);
@@ -503,6 +504,7 @@ static int launch_xphys_t(pb_ctx *c, int want_vv) {
   StageTimer tm(c, PB_STAGE_XPHYS);
   hipLaunchKernelGGL(pb::xphys_kernel<F>, dim3(c->nz32, c->ny), dim3(F::T), ldsx, c->stream, c->pad, c->px, c->twx,
                      c->nxh, c->nz32, c->pfield, c->ddyr, c->dxr, c->dzr, c->vpart, want_vv ? 3 : 2);
+  c->vpart_n = (size_t)c->ny * c->nz32 * ((F::T + 63) / 64);
   HIPCHK(hipGetLastError());
   return PB_OK;
 }
@@ -544,7 +546,7 @@ static int launch_nonlinear(pb_ctx *c, int update_dt, int want_vv) {
   if (r) return r;
   {
     StageTimer tm(c, PB_STAGE_VELMAX);
-    hipLaunchKernelGGL(pb::velmax_kernel, dim3(1), dim3(256), 0, c->stream, c->vpart, (size_t)c->ny * c->nz32,
+    hipLaunchKernelGGL(pb::velmax_kernel, dim3(1), dim3(pb::VELMAX_THREADS), 0, c->stream, c->vpart, c->vpart_n,
                        c->scal, update_dt, c->cfg.dt_factor, c->cflx);
     HIPCHK(hipGetLastError());
   }

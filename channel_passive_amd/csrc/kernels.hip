@@ -28,7 +28,12 @@ struct FftPlan {
 __device__ __forceinline__ double2 c2(double r, double i) { return make_double2(r, i); }
 __device__ __forceinline__ double2 cadd2(double2 a, double2 b) { return c2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub2(double2 a, double2 b) { return c2(a.x - b.x, a.y - b.y); }
+// The transforms need not reproduce a reference's rounding (FFTW's order is
+// unpinned; the transform stage is held to 1e-13 relative), so their complex
+// products may contract to FMA; everything else in the library is built
+// with contraction off.
 __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
+#pragma clang fp contract(fast)
   return c2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 // multiply by sign*i  (sign = -1 forward, +1 backward)
@@ -55,6 +60,7 @@ __device__ __forceinline__ void dft<4>(double2 *v, int sign) {
 }
 template <>
 __device__ __forceinline__ void dft<3>(double2 *v, int sign) {
+#pragma clang fp contract(fast)
   const double c = -0.5, s = sign * 0.86602540378443864676;
   const double2 t = cadd2(v[1], v[2]), d = csub2(v[1], v[2]);
   const double2 m = c2(v[0].x + c * t.x, v[0].y + c * t.y);
@@ -65,6 +71,7 @@ __device__ __forceinline__ void dft<3>(double2 *v, int sign) {
 }
 template <>
 __device__ __forceinline__ void dft<5>(double2 *v, int sign) {
+#pragma clang fp contract(fast)
   const double c1 = 0.30901699437494742410, c2v = -0.80901699437494742410;
   const double s1 = sign * 0.95105651629515357212, s2 = sign * 0.58778525229247312917;
   const double2 t1 = cadd2(v[1], v[4]), t2 = cadd2(v[2], v[3]);
@@ -81,6 +88,7 @@ __device__ __forceinline__ void dft<5>(double2 *v, int sign) {
 }
 template <>
 __device__ __forceinline__ void dft<8>(double2 *v, int sign) {
+#pragma clang fp contract(fast)
   const double h = 0.70710678118654752440;
   double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
   dft<4>(e, sign);
@@ -163,7 +171,13 @@ __device__ __forceinline__ void dft_pq(double2 *v, int sign) {
 #pragma unroll
     for (int k1 = 1; k1 < P; ++k1) {
       const int m = (n2 * k1) % R;
-      a[n2][k1] = cmul2(a[n2][k1], c2(Wr<R>::c[m], sign * Wr<R>::s[m]));
+      if ((4 * m) % R == 0) {  // quarter turns: exact, no arithmetic
+        const int q = (4 * m / R) & 3;
+        a[n2][k1] = q == 1 ? mul_si(a[n2][k1], sign) : (q == 2 ? c2(-a[n2][k1].x, -a[n2][k1].y)
+                                                                 : mul_si(a[n2][k1], -sign));
+      } else {
+        a[n2][k1] = cmul2(a[n2][k1], c2(Wr<R>::c[m], sign * Wr<R>::s[m]));
+      }
     }
 #pragma unroll
   for (int k1 = 0; k1 < P; ++k1) {
@@ -282,7 +296,10 @@ __device__ inline Blk xcd_block() {
 // low 3 bits of the 16-byte element index by bits 4..6, so that the strided
 // stores of the first Stockham pass (and every other access pattern of the
 // kernels, which are consecutive per lane) spread over all bank groups.
-__device__ __forceinline__ int lds_swz(int i) { return i ^ ((i >> 4) & 7); }
+#ifndef PB_LDS_SWZ
+#define PB_LDS_SWZ 1
+#endif
+__device__ __forceinline__ int lds_swz(int i) { return PB_LDS_SWZ ? i ^ ((i >> 4) & 7) : i; }
 
 // Layout of B lines of N elements in LDS: interleaved (element n of line b
 // at n*B + b, the z transforms, whose global rows hold B consecutive kx) or
@@ -434,7 +451,6 @@ __global__ void __launch_bounds__(F::T, F::MINW) xphys_kernel(double2 *__restric
                                                             double dxr, double dzr, double *__restrict__ vpart,
                                                             int nround) {
   extern __shared__ double2 lds[];  // 2 lines of N, layout F::at
-  __shared__ double red[F::T];
   const int z = blockIdx.x, y = blockIdx.y, N = F::N ? F::N : P.N;
   double2 *line = pad + (size_t)y * nz32 * nxh + (size_t)z * nxh;
   // Hermitian completion of (u + i v) and (w + i theta); FFTW c2r ignores Im of the DC bin
@@ -473,18 +489,12 @@ __global__ void __launch_bounds__(F::T, F::MINW) xphys_kernel(double2 *__restric
       vmax = fmax(vel, vmax);
     }
   }
-  red[threadIdx.x] = vmax;
-  __syncthreads();
-  constexpr int P2 = (F::T & (F::T - 1)) ? (1 << (31 - __builtin_clz(F::T))) : F::T;  // largest 2^j <= T
-  if (P2 < F::T) {
-    if ((int)threadIdx.x < F::T - P2) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + P2]);
-    __syncthreads();
-  }
-  for (int s = P2 / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) vpart[(size_t)y * nz32 + z] = red[0];
+  // CFL maximum: across the wave by lane exchanges, one value per wave
+  // (velmax_kernel takes the maximum of all of them; max is order-free)
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, d, 64));
+  constexpr int NW = (F::T + 63) / 64;
+  if ((threadIdx.x & 63) == 0) vpart[((size_t)y * nz32 + z) * NW + threadIdx.x / 64] = vmax;
   // two pairs of products per round: (P1 + i P2, P3 + i P4), (P5 + i P7, P8 + i P9);
   // a third round (statistics time) transforms v*v alone into pad field 8
   for (int round = 0; round < nround; ++round) {
@@ -686,18 +696,28 @@ __global__ void __launch_bounds__(F::T, F::MINW) zfwd_persist(const double2 *__r
 // ------------------------------------------------- CFL maximum and dt update
 // single block: velmax = max(vpart); on substep 1 (update != 0)
 // dt = dt_factor*dt + (1-dt_factor)*cflx/velmax  (dft_x_realspace.f90:489-498)
-__global__ void velmax_kernel(const double *__restrict__ vpart, size_t n, double *__restrict__ scal, int update,
-                              double dt_factor, double cflx) {
-  __shared__ double red[256];
-  double v = 0.0;
-  for (size_t i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, vpart[i]);
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
+constexpr int VELMAX_THREADS = 1024;
+__global__ void __launch_bounds__(VELMAX_THREADS) velmax_kernel(const double *__restrict__ vpart, size_t n,
+                                                                double *__restrict__ scal, int update,
+                                                                double dt_factor, double cflx) {
+  __shared__ double red[VELMAX_THREADS / 64];
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  const size_t T = VELMAX_THREADS;
+  size_t i = threadIdx.x;
+  for (; i + 3 * T < n; i += 4 * T) {
+    v0 = fmax(v0, vpart[i]);
+    v1 = fmax(v1, vpart[i + T]);
+    v2 = fmax(v2, vpart[i + 2 * T]);
+    v3 = fmax(v3, vpart[i + 3 * T]);
   }
+  for (; i < n; i += T) v0 = fmax(v0, vpart[i]);
+  double v = fmax(fmax(v0, v1), fmax(v2, v3));
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
+  __syncthreads();
   if (threadIdx.x == 0) {
+    for (int w = 1; w < VELMAX_THREADS / 64; ++w) red[0] = fmax(red[0], red[w]);
     scal[1] = red[0];
     if (update) scal[0] = dt_factor * scal[0] + (1.0 - dt_factor) * cflx / red[0];
   }
