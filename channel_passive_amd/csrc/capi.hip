@@ -94,6 +94,7 @@ struct pb_ctx {
   bool have_state = false;
   bool fused = true;  // fused-sweep wall-normal solver (bit-identical to the statement-by-statement port)
   bool ct_fft = true;  // fixed-size transforms where one exists for the grid
+  bool ct_z_ok = false, ct_x_ok = false;
   pb::RK3 rk;
   // per-stage event timing
   bool timing = false;
@@ -238,6 +239,19 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   while (c->Bz * 2 <= 16 && c->nxh % (c->Bz * 2) == 0 && (size_t)c->nz32 * c->Bz * 2 * sizeof(double2) <= lds_cap &&
          c->nz32 * c->Bz * 2 <= 4608)
     c->Bz *= 2;
+  // fixed-size z transforms bring their own tile width and LDS size
+  {
+    using FZ = pb::FftZ1152;
+    const int lz = FZ::N * FZ::B * (int)sizeof(double2);
+    c->ct_z_ok = c->ct_fft && c->nz32 == FZ::N && c->nxh % FZ::B == 0 && lz <= lds_max &&
+                 hipFuncSetAttribute(reinterpret_cast<const void *>(&pb::zinv_kernel<FZ>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lz) == hipSuccess &&
+                 hipFuncSetAttribute(reinterpret_cast<const void *>(&pb::zfwd_kernel<FZ>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lz) == hipSuccess;
+    if (c->ct_z_ok) c->Bz = FZ::B;
+    using FX = pb::FftX1536;
+    c->ct_x_ok = c->ct_fft && c->nx32 == FX::N && FX::N * FX::B * (int)sizeof(double2) <= lds_max;
+  }
   c->plane = (size_t)c->nz * c->nxh;
   c->field = c->plane * c->ny;
   c->pplane = (size_t)c->nz32 * c->nxh;
@@ -532,8 +546,8 @@ static int launch_zfwd_t(pb_ctx *c, int want_vv) {
 
 // fixed-size transforms for the sizes they were built for (PB_FFT_GENERIC=1
 // forces the run-time passes everywhere)
-static bool ct_z(const pb_ctx *c) { return c->ct_fft && c->nz32 == pb::FftZ1152::N && c->Bz == pb::FftZ1152::B; }
-static bool ct_x(const pb_ctx *c) { return c->ct_fft && c->nx32 == pb::FftX1536::N; }
+static bool ct_z(const pb_ctx *c) { return c->ct_z_ok; }
+static bool ct_x(const pb_ctx *c) { return c->ct_x_ok; }
 
 static int launch_nonlinear(pb_ctx *c, int update_dt, int want_vv) {
   int r = PB_OK;

@@ -398,12 +398,12 @@ struct CtFft {
       ct_passes<T_, L, N_, B_, -1, 1, R...>(x, tw);
   }
 };
-// 3/2-padded production sizes.  nz32 = 1152 with 4 kx columns per block:
-// 576 threads hold 8 elements each, radix 18, 8, 8 (one butterfly per
-// thread per pass, 256 of 576 in the first).  nx32 = 1536, the two lines of
-// a pair: 256 threads, radix 16, 16, 6.
+// 3/2-padded production sizes.  nz32 = 1152 with 8 kx columns per block
+// (128-byte row segments; 147 KB of LDS, one block per CU): 768 threads
+// hold 12 elements each, radix 8, 12, 12.  nx32 = 1536, the two lines of a
+// pair: 256 threads, radix 16, 16, 6.
 #ifndef PB_FFTZ1152
-#define PB_FFTZ1152 CtFft<576, 3, 1152, 4, false, 18, 8, 8>
+#define PB_FFTZ1152 CtFft<768, 3, 1152, 8, false, 8, 12, 12>
 #endif
 #ifndef PB_FFTX1536
 #define PB_FFTX1536 CtFft<256, 2, 1536, 2, true, 16, 16, 6>
