@@ -79,7 +79,7 @@ struct pb_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
   // device memory
   double2 *U = nullptr, *RK = nullptr, *pad = nullptr, *buf = nullptr, *vd = nullptr;
-  double *dbuf = nullptr, *dbuf2 = nullptr, *vpart = nullptr, *scal = nullptr;
+  double *dbuf = nullptr, *dbuf2 = nullptr, *rbuf = nullptr, *vpart = nullptr, *scal = nullptr;
   double2 *ppe_bc = nullptr;  // statistics time: pressure wall values [2][kz][kx]
   double *dvv = nullptr;      // statistics time: <vv>(y)
   size_t vpart_n = 0;         // per-wave CFL maxima written by the last x-physical launch
@@ -136,7 +136,7 @@ const char *pb_last_error(void) { return g_err.c_str(); }
 
 static int ctx_free(pb_ctx *c) {
   if (!c) return 0;
-  void *ptrs[] = {c->U, c->RK, c->pad, c->buf, c->vd, c->dbuf, c->dbuf2, c->vpart, c->scal, c->dD0, c->dD1, c->dD2,
+  void *ptrs[] = {c->U, c->RK, c->pad, c->buf, c->vd, c->dbuf, c->dbuf2, c->rbuf, c->vpart, c->scal, c->dD0, c->dD1, c->dD2,
                   c->dD0LU, c->dinteg, c->ddyr, c->drows, c->twz, c->twx, c->slotmap, c->mean_db,
                   c->ppe_bc, c->dvv};
   for (void *p : ptrs)
@@ -283,6 +283,7 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   rc = rc ? rc : dalloc((void **)&c->vd, 2 * c->plane * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->dbuf, (size_t)c->W * c->field * sizeof(double));
   rc = rc ? rc : dalloc((void **)&c->dbuf2, (size_t)c->W * c->field * sizeof(double));
+  rc = rc ? rc : dalloc((void **)&c->rbuf, c->field * sizeof(double));
   rc = rc ? rc : dalloc((void **)&c->vpart, (size_t)c->ny * c->nz32 * 16 * sizeof(double));  // <= 16 waves/block
   rc = rc ? rc : dalloc((void **)&c->scal, SCAL_N * sizeof(double));
   rc = rc ? rc : dalloc((void **)&c->dD0, bandb);
@@ -572,10 +573,14 @@ static int launch_nonlinear(pb_ctx *c, int update_dt, int want_vv) {
 extern "C++" {
 template <int BW>
 static int launch_sweeps(pb_ctx *c, const pb::LineParams &P, const pb::WsArgs &A, int s) {
-  const dim3 grid(blocks_for(c->nsys, 64)), block(64);
-#define SWEEP(id, strm)                                                                                      \
-  hipLaunchKernelGGL((pb::sweep_kernel<BW, id>), grid, block, 0, strm, P, A, c->dbuf2, c->nsys, c->scal); \
-  HIPCHK(hipGetLastError());
+  const dim3 block(64);
+#define SWEEP(id, strm)                                                                                     \
+  {                                                                                                         \
+    const size_t nt = pb::sweep_threads(id, A);                                                             \
+    hipLaunchKernelGGL((pb::sweep_kernel<BW, id>), dim3(blocks_for(nt, 64)), block, 0, strm, P, A, c->dbuf2, \
+                       nt, c->scal);                                                                        \
+    HIPCHK(hipGetLastError());                                                                              \
+  }
   if (s == 1) {
     SWEEP(pb::SW_A1, c->stream);
     SWEEP(pb::SW_B1, c->stream);
@@ -608,6 +613,7 @@ static pb::WsArgs ws_args(pb_ctx *c, int s) {
   A.RK = (pb::cplx *)c->RK;
   A.buf = (pb::cplx *)c->buf;
   A.dbuf = c->dbuf;
+  A.rbuf = c->rbuf;
   A.vd = c->cfg.opp_alpha != 0.0 ? (const pb::cplx *)c->vd : nullptr;
   A.plane = c->plane;
   A.field = c->field;

@@ -808,13 +808,13 @@ __global__ void __launch_bounds__(MEAN_THREADS) mean_kernel(LineParams P, WsArgs
 
 // Fused sweeps over all (kx,kz) != (0,0) systems, one thread per system.
 template <int BW, int SW>
-__global__ void __launch_bounds__(64) sweep_kernel(LineParams P, WsArgs A, double *dbuf2, size_t nsys,
+__global__ void __launch_bounds__(64) sweep_kernel(LineParams P, WsArgs A, double *dbuf2, size_t nthreads,
                                                    const double *scal) {
-  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (sys >= nsys) return;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nthreads) return;
   LineParams Q = P;
   Q.dt = scal[0];
-  run_sweep<BW, SW>(Q, A, dbuf2, sys);
+  run_sweep<BW, SW>(Q, A, dbuf2, idx);
 }
 
 // opposition-control detection planes: vd[0][sys], vd[1][sys]

@@ -516,6 +516,7 @@ struct WsArgs {
   cplx *RK;       // 3 slots
   cplx *buf;      // 2 scratch columns per system
   double *dbuf;   // W*ny doubles per system
+  double *rbuf;   // one real scratch column per system (fused sweeps only)
   const cplx *vd; // detection-plane v (2 planes) or nullptr
   size_t plane;   // nz*nxh
   size_t field;   // ny*plane

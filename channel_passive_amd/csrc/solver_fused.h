@@ -32,11 +32,14 @@
 //   after C : 0 y_phip, 2 y_g, 7 y_theta, R0 y_phih; U factors in dbuf, dbuf2
 //   after D : 0 phi_p, 2 g, 6 theta, R1 phi_h, R2 D2 phi_h; RK updated
 //   after E : 0 y_vp, R3 y_vh; U factor in dbuf
-//   after F : 0 v_p, 7 v_p', R3 v_h', R0 D0 c_h
+//   after F : 0 v_p, 7 v_p', R4 v_h', R0 D0 c_h
 //   after G : 0 u, 1 v, 2 w, 6 theta; RK2 = store_phi
-// R0..R3 are real columns (the reference's (1+i)-scaled homogeneous
-// solutions are (x, x): only x is stored) in the memory of the two scratch
-// columns, which hold D2 LU(D0)^-1 of the stores on s == 1 until B.
+// R0..R4 are real columns (the reference's (1+i)-scaled homogeneous
+// solutions are (x, x): only x is stored); R0..R3 live in the memory of the
+// two scratch columns, which hold D2 LU(D0)^-1 of the stores on s == 1
+// until B, R4 in rbuf.  The U factors and R0..R4 depend on k^2 only and are
+// kept once per (kx, +-kz) pair, at the primary system's index; C and E run
+// one thread per pair, the other sweeps one thread per system.
 // B, C and D are split by right-hand-side group (theta | g | phi,v): the
 // groups read disjoint data, so the split costs no traffic and keeps the
 // register windows of each kernel small; the theta chain can run beside
@@ -84,16 +87,22 @@ struct RV {
   PB_HD double &operator[](int i) const { return p[(size_t)i * s]; }
 };
 
-// Per-system context shared by the sweeps.
+// Per-system context shared by the sweeps.  The systems (kx, kz) and
+// (kx, -kz) have the same k^2, hence bit-identical matrices: their LU
+// factors and the homogeneous solutions (which depend on k^2 only) are
+// formed once, by the pair's primary system prim (kz >= 0), and both
+// systems read them there.
 struct SysCtx {
-  size_t sys;
+  size_t sys, prim;
   int n;
   double kx, kz, k2;
+  PB_HD bool is_prim() const { return sys == prim; }
   PB_HD SV col(const WsArgs &A, int slot) const { return SV{A.U + (size_t)slot * A.field + sys, A.plane}; }
-  // real column k (0..3) in the memory of the two scratch columns; only
-  // the sweeps C_nu to G use it (the complex scratch view is dead by then)
+  // shared real column k of the pair: 0..3 in the memory of the two scratch
+  // columns (the complex scratch view is dead from sweep C on), 4 in rbuf
   PB_HD RV rcol(const WsArgs &A, int k) const {
-    return RV{reinterpret_cast<double *>(A.buf) + (size_t)k * A.field + sys, A.plane};
+    double *base = k < 4 ? reinterpret_cast<double *>(A.buf) + (size_t)k * A.field : A.rbuf;
+    return RV{base + prim, A.plane};
   }
   PB_HD SV rk(const WsArgs &A, int slot) const { return SV{A.RK + (size_t)slot * A.field + sys, A.plane}; }
   PB_HD SV buf(const WsArgs &A, int slot) const { return SV{A.buf + (size_t)slot * A.field + sys, A.plane}; }
@@ -103,6 +112,7 @@ PB_HD bool make_sys(const WsArgs &A, int n, size_t sys, SysCtx &c) {
   const int k = (int)(sys % A.nxh), j = (int)(sys / A.nxh);
   const int kzi = (j <= A.nh) ? j : j + 1 - 2 * A.nh - 1;  // module_ppdfftw.f90:360-374
   c.sys = sys;
+  c.prim = (j <= A.nh) ? sys : (size_t)(A.nz - j) * A.nxh + k;  // kz index nz-j holds -kz
   c.n = n;
   c.kx = k * A.lmfxr;
   c.kz = kzi * A.lmfzr;
@@ -501,18 +511,34 @@ PB_HD void sweep_Bv(const LineParams &P, const WsArgs &A, const SysCtx &c) {
 // solve_v_f.f90:142-148; solve_t.f90:42-71, 158).
 //   C_nu: H_nu; rhs_g (slot 2), rhs_phi (slot 0), phi_h seed -> slot 1
 //   C_t : H_theta; rhs_t (slot 7)
-template <int BW>
-PB_HD void sweep_Cnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
+// Systems handled by one thread of a factorising sweep: the primary
+// (kz >= 0) and, for 0 < kz index < nz/2, its partner at -kz.
+struct PairCtx {
+  SysCtx a, b;
+  bool two;
+};
+PB_HD bool make_pair(const WsArgs &A, int n, size_t t, PairCtx &pc) {
+  const int k = (int)(t % A.nxh), j = (int)(t / A.nxh);  // j = 0 .. nz/2
+  if (!make_sys(A, n, (size_t)j * A.nxh + k, pc.a)) return false;
+  pc.two = j >= 1 && j < A.nh;
+  if (pc.two) make_sys(A, n, (size_t)(A.nz - j) * A.nxh + k, pc.b);
+  else pc.b = pc.a;
+  return true;
+}
+
+template <int BW, bool TWO>
+PB_HD void sweep_Cnu_t(const LineParams &P, const WsArgs &A, const PairCtx &pc) {
+  const SysCtx &c = pc.a;
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const double be = P.dt * P.coef(1, P.s) * P.nu;
   const HelmholtzA H{P.D0, P.D2, 1.0 + be * c.k2, be, n, BW};
-  const OwnU U{A.dbuf + c.sys, A.plane, 2 * BW + 1};
-  SV s0 = c.col(A, 0), s2 = c.col(A, 2);
+  const OwnU U{A.dbuf + c.prim, A.plane, 2 * BW + 1};
+  SV s0 = c.col(A, 0), s2 = c.col(A, 2), t0 = pc.b.col(A, 0), t2 = pc.b.col(A, 2);
   RV h1 = c.rcol(A, 0);
-  CWin<BW> wg, wp;
+  CWin<BW> wg, wp, vg, vp;
   RWin<BW> wh;
-  wg.clear(); wp.clear(); wh.clear();
+  wg.clear(); wp.clear(); vg.clear(); vp.clear(); wh.clear();
   double d00 = 0.0;
   crout_sweep<BW>(g, H, U, [&](int r, const double(&L)[2 * BW + 1]) __attribute__((always_inline)) {
     if (r == 0) d00 = L[0];
@@ -521,24 +547,46 @@ PB_HD void sweep_Cnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
     const double yh = fwd_lrow_r<BW>(g, r, L, wh, h1, r == 0 ? d00 : 0.0);
     s2[r] = yg; s0[r] = yp; h1[r] = yh;
     wg.push(yg); wp.push(yp); wh.push(yh);
+    if (TWO) {
+      const cplx zg = fwd_lrow<BW>(g, r, L, vg, t2, t2[r]);
+      const cplx zp = fwd_lrow<BW>(g, r, L, vp, t0, t0[r]);
+      t2[r] = zg; t0[r] = zp;
+      vg.push(zg); vp.push(zp);
+    }
   });
 }
-
 template <int BW>
-PB_HD void sweep_Ct(const LineParams &P, const WsArgs &A, const SysCtx &c, double *dbuf2) {
+PB_HD void sweep_Cnu(const LineParams &P, const WsArgs &A, const PairCtx &pc) {
+  if (pc.two) sweep_Cnu_t<BW, true>(P, A, pc);
+  else sweep_Cnu_t<BW, false>(P, A, pc);
+}
+
+template <int BW, bool TWO>
+PB_HD void sweep_Ct_t(const LineParams &P, const WsArgs &A, const PairCtx &pc, double *dbuf2) {
+  const SysCtx &c = pc.a;
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const double be = P.dt * P.coef(1, P.s) * P.nu / P.pr;
   const HelmholtzA H{P.D0, P.D2, 1.0 + be * c.k2, be, n, BW};
-  const OwnU U{dbuf2 + c.sys, A.plane, 2 * BW + 1};
-  SV s7 = c.col(A, 7);
-  CWin<BW> wt;
-  wt.clear();
+  const OwnU U{dbuf2 + c.prim, A.plane, 2 * BW + 1};
+  SV s7 = c.col(A, 7), t7 = pc.b.col(A, 7);
+  CWin<BW> wt, vt;
+  wt.clear(); vt.clear();
   crout_sweep<BW>(g, H, U, [&](int r, const double(&L)[2 * BW + 1]) __attribute__((always_inline)) {
     const cplx yt = fwd_lrow<BW>(g, r, L, wt, s7, s7[r]);
     s7[r] = yt;
     wt.push(yt);
+    if (TWO) {
+      const cplx zt = fwd_lrow<BW>(g, r, L, vt, t7, t7[r]);
+      t7[r] = zt;
+      vt.push(zt);
+    }
   });
+}
+template <int BW>
+PB_HD void sweep_Ct(const LineParams &P, const WsArgs &A, const PairCtx &pc, double *dbuf2) {
+  if (pc.two) sweep_Ct_t<BW, true>(P, A, pc, dbuf2);
+  else sweep_Ct_t<BW, false>(P, A, pc, dbuf2);
 }
 
 // D, split by matrix: back substitution and the D0/D2 products
@@ -551,7 +599,7 @@ PB_HD void sweep_Dnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n, s = P.s, S = P.S;
   const double dt = P.dt, nu = P.nu, k2 = c.k2;
-  const OwnU U{A.dbuf + c.sys, A.plane, 2 * BW + 1};
+  const OwnU U{A.dbuf + c.prim, A.plane, 2 * BW + 1};
   double al_g = 0, du_g = 0, al_p = 0, du_p = 0;
   if (s != S) {
     al_g = dt * P.coef(0, s + 1) * nu;
@@ -583,7 +631,8 @@ PB_HD void sweep_Dnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
         r1[m] = gv;
         r2[m] = pv;
       }
-      s2[m] = gv; s0[m] = pv; h1[m] = hv; h3[m] = d2h;
+      s2[m] = gv; s0[m] = pv;
+      if (c.is_prim()) { h1[m] = hv; h3[m] = d2h; }
     });
   }
 }
@@ -592,7 +641,7 @@ template <int BW>
 PB_HD void sweep_Dt(const LineParams &P, const WsArgs &A, const SysCtx &c, double *dbuf2) {
   const BandGeom g{c.n, BW};
   const int n = c.n, s = P.s, S = P.S;
-  const OwnU U{dbuf2 + c.sys, A.plane, 2 * BW + 1};
+  const OwnU U{dbuf2 + c.prim, A.plane, 2 * BW + 1};
   double al = 0, du = 0;
   if (s != S) {
     al = P.dt * P.coef(0, s + 1) * P.nu / P.pr;
@@ -617,34 +666,47 @@ PB_HD void sweep_Dt(const LineParams &P, const WsArgs &A, const SysCtx &c, doubl
 
 // E: LU of the Poisson operator fused with the forward substitution of
 // v_p (wall values -alpha v(y_d) with opposition control, else 0) and v_h
-// (solve_v_f.f90:205-262)
-template <int BW>
-PB_HD void sweep_E(const LineParams &P, const WsArgs &A, const SysCtx &c) {
+// (solve_v_f.f90:205-262), for the pair of systems sharing the operator
+template <int BW, bool TWO>
+PB_HD void sweep_E_t(const LineParams &P, const WsArgs &A, const PairCtx &pc) {
+  const SysCtx &c = pc.a, &d = pc.b;
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const PoissonA Q{P.D0, P.D2, c.k2, n, BW};
-  const OwnU Uq{A.dbuf + c.sys, A.plane, 2 * BW + 1};
+  const OwnU Uq{A.dbuf + c.prim, A.plane, 2 * BW + 1};
   const bool opp = A.opp_alpha != 0.0 && A.vd;
-  SV s0 = c.col(A, 0);
+  SV s0 = c.col(A, 0), t0 = d.col(A, 0);
   RV h1 = c.rcol(A, 1), h5 = c.rcol(A, 3);
-  CWin<BW> wp;
+  CWin<BW> wp, vp;
   RWin<BW> wh;
-  wp.clear(); wh.clear();
+  wp.clear(); vp.clear(); wh.clear();
   crout_sweep<BW>(g, Q, Uq, [&](int r, const double(&L)[2 * BW + 1]) __attribute__((always_inline)) {
-    cplx bp = s0[r];
+    cplx bp = s0[r], bq = TWO ? t0[r] : mk(0.0, 0.0);
     double bh = h1[r];
     if (r == 0 || r == n - 1) {
       // wall rows: the value is read here rather than held in a variable
       // chosen by r, which the compiler would turn into a private-memory
       // address select
-      bp = opp ? cscale(A.vd[(r == 0 ? 0 : A.plane) + c.sys], -A.opp_alpha) : mk(0.0, 0.0);
+      const size_t off = (r == 0 ? 0 : A.plane);
+      bp = opp ? cscale(A.vd[off + c.sys], -A.opp_alpha) : mk(0.0, 0.0);
+      if (TWO) bq = opp ? cscale(A.vd[off + d.sys], -A.opp_alpha) : mk(0.0, 0.0);
       bh = 0.0;
     }
     const cplx yp = fwd_lrow<BW>(g, r, L, wp, s0, bp);
     const double yh = fwd_lrow_r<BW>(g, r, L, wh, h5, bh);
     s0[r] = yp; h5[r] = yh;
     wp.push(yp); wh.push(yh);
+    if (TWO) {
+      const cplx zp = fwd_lrow<BW>(g, r, L, vp, t0, bq);
+      t0[r] = zp;
+      vp.push(zp);
+    }
   });
+}
+template <int BW>
+PB_HD void sweep_E(const LineParams &P, const WsArgs &A, const PairCtx &pc) {
+  if (pc.two) sweep_E_t<BW, true>(P, A, pc);
+  else sweep_E_t<BW, false>(P, A, pc);
 }
 
 // F: back substitution with U of the Poisson operator and the D0/D1
@@ -654,9 +716,9 @@ template <int BW>
 PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
-  const OwnU Uq{A.dbuf + c.sys, A.plane, 2 * BW + 1};
+  const OwnU Uq{A.dbuf + c.prim, A.plane, 2 * BW + 1};
   SV s0 = c.col(A, 0), s7 = c.col(A, 7);
-  RV h5 = c.rcol(A, 3), h8 = c.rcol(A, 0);
+  RV h5 = c.rcol(A, 3), h4 = c.rcol(A, 4), h8 = c.rcol(A, 0);
   CWin<2 * BW + 1> wp;
   RWin<2 * BW + 1> wh;
   wp.clear(); wh.clear();
@@ -668,7 +730,8 @@ PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
       const cplx dvp = mv_row<BW>(g, P.D1, m, r, wp);
       const double dvh = mv_row_r<BW>(g, P.D1, m, r, wh);
       const double d0h = mv_row_r<BW>(g, P.D0, m, r, wh);
-      s0[m] = vp; s7[m] = dvp; h5[m] = dvh; h8[m] = d0h;
+      s0[m] = vp; s7[m] = dvp;
+      if (c.is_prim()) { h4[m] = dvh; h8[m] = d0h; }
     });
   }
 }
@@ -679,7 +742,7 @@ PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const int n = c.n, s = P.s, S = P.S;
   const double dt = P.dt, nu = P.nu, k2 = c.k2;
   SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s7 = c.col(A, 7);
-  RV h1 = c.rcol(A, 1), h3 = c.rcol(A, 2), h5 = c.rcol(A, 3), h8 = c.rcol(A, 0);
+  RV h1 = c.rcol(A, 1), h3 = c.rcol(A, 2), h5 = c.rcol(A, 4), h8 = c.rcol(A, 0);
   SV r2 = c.rk(A, 1);
   // the real columns stand for the complex (x, x) of the reference
   auto cx = [](double x) { return mk(x, x); };
@@ -757,26 +820,39 @@ PB_HD void opp_line(const LineParams &P, const SV &v, const SV &y, int first_lo,
   }
 }
 
-// One sweep of one (kx,kz) != (0,0) system.  A substep runs, for all
-// systems, A1 and B1 (s == 1 only), A, B_g, then the theta chain B_t, C_t,
-// D_t and, in any order relative to it, B_phi, C_nu, D_nu; then E, F, G.
+// One sweep: index is a system (kz-major, kx-minor) or, for the
+// factorising sweeps C_nu, C_theta and E, a pair (kz index 0 .. nz/2).
+// A substep runs, for all systems, A1 and B1 (s == 1 only), A, B_g, then
+// the theta chain B_t, C_t, D_t and, in any order relative to it, B_phi,
+// C_nu, D_nu; then E, F, G.
 enum SweepId { SW_A1, SW_B1, SW_A, SW_BT, SW_CT, SW_DT, SW_BG, SW_BV, SW_CNU, SW_DNU, SW_E, SW_F, SW_G, SW_COUNT };
 
+constexpr bool sweep_paired(int sw) { return sw == SW_CNU || sw == SW_CT || sw == SW_E; }
+// threads of a sweep
+PB_HD size_t sweep_threads(int sw, const WsArgs &A) {
+  return sweep_paired(sw) ? (size_t)(A.nh + 1) * A.nxh : A.plane;
+}
+
 template <int BW, int SW>
-PB_HD void run_sweep(const LineParams &P, const WsArgs &A, double *dbuf2, size_t sys) {
+PB_HD void run_sweep(const LineParams &P, const WsArgs &A, double *dbuf2, size_t idx) {
+  if (sweep_paired(SW)) {
+    PairCtx pc;
+    if (!make_pair(A, P.g.n, idx, pc)) return;
+    if (SW == SW_CT) sweep_Ct<BW>(P, A, pc, dbuf2);
+    if (SW == SW_CNU) sweep_Cnu<BW>(P, A, pc);
+    if (SW == SW_E) sweep_E<BW>(P, A, pc);
+    return;
+  }
   SysCtx c;
-  if (!make_sys(A, P.g.n, sys, c)) return;
+  if (!make_sys(A, P.g.n, idx, c)) return;
   if (SW == SW_A1) sweep_A1<BW>(P, A, c);
   if (SW == SW_B1) sweep_B1<BW>(P, A, c);
   if (SW == SW_A) sweep_A<BW>(P, A, c);
   if (SW == SW_BT) sweep_Bt<BW>(P, A, c);
-  if (SW == SW_CT) sweep_Ct<BW>(P, A, c, dbuf2);
   if (SW == SW_DT) sweep_Dt<BW>(P, A, c, dbuf2);
   if (SW == SW_BG) sweep_Bg<BW>(P, A, c);
   if (SW == SW_BV) sweep_Bv<BW>(P, A, c);
-  if (SW == SW_CNU) sweep_Cnu<BW>(P, A, c);
   if (SW == SW_DNU) sweep_Dnu<BW>(P, A, c);
-  if (SW == SW_E) sweep_E<BW>(P, A, c);
   if (SW == SW_F) sweep_F<BW>(P, A, c);
   if (SW == SW_G) sweep_G(P, A, c);
 }

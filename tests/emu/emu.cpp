@@ -73,7 +73,7 @@ int emu_wavespace(const pb_config *cfg, int s, double dt, double *U, double *RK,
   std::vector<pb::cplx> buf(2 * field);
   std::vector<double> dbuf((size_t)S.W * field);
   pb::WsArgs A;
-  A.U = (pb::cplx *)U; A.RK = (pb::cplx *)RK; A.buf = buf.data(); A.dbuf = dbuf.data();
+  A.U = (pb::cplx *)U; A.RK = (pb::cplx *)RK; A.buf = buf.data(); A.dbuf = dbuf.data(); A.rbuf = nullptr;
   A.vd = (const pb::cplx *)vd; A.plane = plane; A.field = field; A.nxh = nxh; A.nz = nz; A.nh = nz / 2;
   A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha; A.mean_out = mean4;
   for (size_t sys = 0; sys < plane; ++sys) pb::wavespace_line(P, A, sys);
@@ -109,9 +109,9 @@ extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, doubl
   const int nxh = (int)cfg->nx / 2, nz = (int)cfg->nz;
   const size_t plane = (size_t)nz * nxh, field = plane * S.n;
   std::vector<pb::cplx> buf(2 * field);
-  std::vector<double> dbuf((size_t)S.W * field), dbuf2((size_t)S.W * field);
+  std::vector<double> dbuf((size_t)S.W * field), dbuf2((size_t)S.W * field), rbuf(field);
   pb::WsArgs A;
-  A.U = (pb::cplx *)U; A.RK = (pb::cplx *)RK; A.buf = buf.data(); A.dbuf = dbuf.data();
+  A.U = (pb::cplx *)U; A.RK = (pb::cplx *)RK; A.buf = buf.data(); A.dbuf = dbuf.data(); A.rbuf = rbuf.data();
   A.vd = (const pb::cplx *)vd; A.plane = plane; A.field = field; A.nxh = nxh; A.nz = nz; A.nh = nz / 2;
   A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha; A.mean_out = mean4;
   if (S.bw < 3 || S.bw > 5) return -2;
@@ -120,10 +120,10 @@ extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, doubl
   int ids[pb::SW_COUNT];
   const int nid = pb::sweep_sequence(s, ids);
   for (int q = 0; q < nid; ++q)
-    for (size_t sys = 1; sys < plane; ++sys) {
-      if (S.bw == 3) run_id<3>(ids[q], P, A, dbuf2.data(), sys);
-      if (S.bw == 4) run_id<4>(ids[q], P, A, dbuf2.data(), sys);
-      if (S.bw == 5) run_id<5>(ids[q], P, A, dbuf2.data(), sys);
+    for (size_t idx = 0; idx < pb::sweep_threads(ids[q], A); ++idx) {
+      if (S.bw == 3) run_id<3>(ids[q], P, A, dbuf2.data(), idx);
+      if (S.bw == 4) run_id<4>(ids[q], P, A, dbuf2.data(), idx);
+      if (S.bw == 5) run_id<5>(ids[q], P, A, dbuf2.data(), idx);
     }
   return 0;
 }
