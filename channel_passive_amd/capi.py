@@ -53,6 +53,7 @@ SIGNATURES = {
     "pb_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(PbSubstepOut)]),
     "pb_substep_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(PbSubstepOut), _dp,
                                         _dp]),
+    "pb_statistics": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "pb_step": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(PbSubstepOut)]),
     "pb_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "pb_wavespace_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp,
@@ -159,6 +160,19 @@ class Channel:
         _check(load().pb_substep_stats(self.h, s, ctypes.byref(d), ctypes.byref(o), _ptr(p),
                                        vv.ctypes.data_as(_dp)))
         return o.as_dict(), p, vv
+
+    STAT_NAMES = ("U", "W", "Theta", "dUdy", "dWdy", "dTdy", "uu", "vv", "ww", "tt", "uv", "uw", "vw", "ut",
+                  "wt", "vt")
+
+    def statistics(self):
+        """get_statistics_1's profiles of the current state: a dict of
+        ny-vectors plus u_center and t_center."""
+        n = self.cfg.ny
+        out = np.zeros(16 * n + 2)
+        _check(load().pb_statistics(self.h, out.ctypes.data_as(_dp)))
+        d = {k: out[i * n:(i + 1) * n] for i, k in enumerate(self.STAT_NAMES)}
+        d["u_center"], d["t_center"] = out[16 * n], out[16 * n + 1]
+        return d
 
     def step(self, dt: float):
         d = ctypes.c_double(dt)

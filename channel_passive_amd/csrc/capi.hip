@@ -81,6 +81,8 @@ struct pb_ctx {
   double2 *U = nullptr, *RK = nullptr, *pad = nullptr, *buf = nullptr, *vd = nullptr;
   double *dbuf = nullptr, *dbuf2 = nullptr, *rbuf = nullptr, *vpart = nullptr, *scal = nullptr;
   double2 *ppe_bc = nullptr;  // statistics time: pressure wall values [2][kz][kx]
+  double *dcenter = nullptr;  // B_k(0): centre-line values of the basis (get_statistics.f90:566-570)
+  double *dstat = nullptr;    // 16 profiles of ny + 2 centre values
   double *dvv = nullptr;      // statistics time: <vv>(y)
   size_t vpart_n = 0;         // per-wave CFL maxima written by the last x-physical launch
   size_t mean_lds = 0;        // dynamic LDS of mean_kernel (0: one-thread wave00_kernel)
@@ -138,7 +140,7 @@ static int ctx_free(pb_ctx *c) {
   if (!c) return 0;
   void *ptrs[] = {c->U, c->RK, c->pad, c->buf, c->vd, c->dbuf, c->dbuf2, c->rbuf, c->vpart, c->scal, c->dD0, c->dD1, c->dD2,
                   c->dD0LU, c->dinteg, c->ddyr, c->drows, c->twz, c->twx, c->slotmap, c->mean_db,
-                  c->ppe_bc, c->dvv};
+                  c->ppe_bc, c->dvv, c->dcenter, c->dstat};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -298,6 +300,8 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   rc = rc ? rc : dalloc((void **)&c->slotmap, 16 * sizeof(int));
   rc = rc ? rc : dalloc((void **)&c->ppe_bc, 2 * c->plane * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->dvv, c->ny * sizeof(double));
+  rc = rc ? rc : dalloc((void **)&c->dcenter, c->ny * sizeof(double));
+  rc = rc ? rc : dalloc((void **)&c->dstat, ((size_t)16 * c->ny + 2) * sizeof(double));
   // mean line: its columns (and, when they fit, its band) in LDS
   {
     const size_t cols = (size_t)pb::MEAN_COLS * c->ny * sizeof(double2), db = bandb;
@@ -327,6 +331,13 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   UP(c->dinteg, c->col.integ.data(), c->ny * sizeof(double));
   UP(c->ddyr, c->dyr.data(), c->ny * sizeof(double));
   if (!c->opp_rows.empty()) UP(c->drows, c->opp_rows.data(), c->opp_rows.size() * sizeof(double));
+  {
+    std::vector<double> cen(c->ny, 0.0), vals;
+    int first = 0;
+    if (pb::basis_row(c->col, 0.0, first, vals)) { ctx_free(c); return fail(PB_E_ARG, "no basis row at y = 0"); }
+    for (size_t j = 0; j < vals.size(); ++j) cen[first + j] = vals[j];
+    UP(c->dcenter, cen.data(), c->ny * sizeof(double));
+  }
   UP(c->twz, tz.data(), c->nz32 * sizeof(double2));
   UP(c->twx, tx.data(), c->nx32 * sizeof(double2));
 #undef UP
@@ -753,6 +764,21 @@ int pb_substep_stats(pb_ctx *c, int32_t s, double *dt_inout, pb_substep_out *out
   }
   if ((r = launch_wavespace(c, s))) return r;
   return read_outputs(c, s, s, out, dt_inout);
+}
+
+int pb_statistics(pb_ctx *c, double *out) {
+  if (!c || !out) return fail(PB_E_ARG, "null argument");
+  if (!c->have_state) return fail(PB_E_STATE, "pb_statistics before pb_set_state");
+  pb::LineParams P = c->line_params(1, 0.0);
+  hipLaunchKernelGGL(pb::stats_uv_kernel, dim3(blocks_for((size_t)10 * c->ny, 64)), dim3(64), 0, c->stream,
+                     (const pb::cplx *)c->U, c->plane, c->ny, c->nxh, c->dstat);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(pb::stats_mean_kernel, dim3(1), dim3(64), 0, c->stream, P, (const pb::cplx *)c->U,
+                     (pb::cplx *)c->buf, c->plane, c->dcenter, c->cfg.u_gal, c->dstat);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, c->dstat, ((size_t)16 * c->ny + 2) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PB_OK;
 }
 
 int pb_step(pb_ctx *c, double *dt_inout, pb_substep_out *out) {

@@ -849,6 +849,29 @@ __global__ void __launch_bounds__(64) vv_kernel(const cplx *U, size_t plane, int
   if (y >= ny) return;
   vv[y] = mean_vv_row(U + (size_t)1 * plane * ny + (size_t)y * plane, nxh, plane);
 }
+// statistics profiles (get_statistics_1): one thread per (y, product) for the
+// ten Parseval sums, serial over the plane's modes; stat: 16 lines of ny,
+// then u_center, t_center
+__constant__ int kStatPairs[10][2] = {{0, 0}, {1, 1}, {2, 2}, {6, 6}, {0, 1}, {0, 2}, {1, 2}, {0, 6}, {6, 2}, {1, 6}};
+__global__ void __launch_bounds__(64) stats_uv_kernel(const cplx *U, size_t plane, int ny, int nxh, double *stat) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 10 * ny) return;
+  const int q = t / ny, y = t % ny;
+  const size_t field = plane * ny;
+  const cplx *a = U + (size_t)kStatPairs[q][0] * field + (size_t)y * plane;
+  const cplx *b = U + (size_t)kStatPairs[q][1] * field + (size_t)y * plane;
+  stat[(size_t)(6 + q) * ny + y] = mean_uv_row(a, b, nxh, plane);
+}
+// the mean-line profiles and centre values (one thread)
+__global__ void stats_mean_kernel(LineParams P, const cplx *U, cplx *tmp, size_t plane, const double *center,
+                                  double u_gal, double *stat) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const size_t field = plane * P.g.n;
+  const SV m[3] = {SV{const_cast<cplx *>(U), plane}, SV{const_cast<cplx *>(U) + 2 * field, plane},
+                   SV{const_cast<cplx *>(U) + 6 * field, plane}};
+  mean_line_stats(P, m, SV{tmp, plane}, SV{tmp + field, plane}, center, u_gal, stat, stat + (size_t)16 * P.g.n);
+}
+
 // wall values of the pressure Poisson equation from v (slot 1), per system
 __global__ void __launch_bounds__(64) ppe_bc_kernel(LineParams P, const cplx *U, cplx *tmp, cplx *bc, size_t plane,
                                                     size_t nsys, double nu) {

@@ -485,18 +485,42 @@ PB_HD void solve_ppe(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3
   band_mv(P.g, P.D0, buf1, u6);
 }
 
-// <vv>(y) of get_uv (get_statistics.f90:982-1017) for one y: the modes of
-// the plane row in kz-major, kx-minor order, kx = 0 once, kx > 0 twice.
-PB_HD double mean_vv_row(const cplx *row, int nxh, size_t plane) {
+// <ab>(y) of get_uv (get_statistics.f90:982-1017) for one y: the modes of
+// the plane rows in kz-major, kx-minor order, kx = 0 once, kx > 0 twice.
+PB_HD double mean_uv_row(const cplx *a, const cplx *b, int nxh, size_t plane) {
   double acc = 0.0;
   for (size_t l = 0; l < plane; ++l) {
-    const cplx v = row[l];
+    const cplx u = a[l], v = b[l];
     if (l % nxh == 0)
-      acc = acc + v.re * v.re + v.im * v.im;
+      acc = acc + u.re * v.re + u.im * v.im;
     else
-      acc = acc + 2.0 * (v.re * v.re + v.im * v.im);
+      acc = acc + 2.0 * (u.re * v.re + u.im * v.im);
   }
   return acc;
+}
+PB_HD double mean_vv_row(const cplx *row, int nxh, size_t plane) { return mean_uv_row(row, row, nxh, plane); }
+
+// Mean-line profiles of get_statistics_1 (get_statistics.f90:54-86 and the
+// gradient routines 546-652): out lines (ny each) 0 U (+u_gal), 1 W,
+// 2 Theta, 3 dU/dy, 4 dW/dy, 5 dTheta/dy; centre[0] = u_center (u_gal +
+// sum Re(c_k) B_k(0)), centre[1] = t_center.  m[q] are the (0,0) columns of
+// u, w, theta; t1, t2 scratch.
+PB_HD void mean_line_stats(const LineParams &P, const SV m[3], const SV &t1, const SV &t2, const double *center,
+                           double u_gal, double *out, double *centre) {
+  const int n = P.g.n;
+  for (int q = 0; q < 3; ++q)
+    for (int y = 0; y < n; ++y) out[(size_t)q * n + y] = m[q][y].re + (q == 0 ? u_gal : 0.0);
+  for (int q = 0; q < 3; ++q) {
+    for (int y = 0; y < n; ++y) t1[y] = m[q][y];
+    band_bs(P.g, P.D0LU, t1);
+    band_mv(P.g, P.D1, t1, t2);
+    for (int y = 0; y < n; ++y) out[(size_t)(3 + q) * n + y] = t2[y].re;
+    if (q != 1) {
+      double a = (q == 0) ? u_gal : 0.0;
+      for (int k = 0; k < n; ++k) a = a + t1[k].re * center[k];
+      centre[q == 0 ? 0 : 1] = a;
+    }
+  }
 }
 
 // Detection-plane value of a collocation column: c = D0^{-1} v (in buf),

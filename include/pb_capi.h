@@ -128,6 +128,14 @@ int pb_substep(pb_ctx *ctx, int32_t s, double *dt_inout, pb_substep_out *out);
  * exactly as with pb_substep.  For s != 1 it is pb_substep. */
 int pb_substep_stats(pb_ctx *ctx, int32_t s, double *dt_inout, pb_substep_out *out, double *p_out,
                      double *vv_out);
+/* Profiles of get_statistics_1 (get_statistics.f90:54-155) from the state
+ * held between steps: out receives 16 lines of ny doubles, in this order:
+ * U (+u_gal), W, Theta (get_u, 1024-1048); dU/dy, dW/dy, dTheta/dy
+ * (D1 LU(D0)^-1 of the mean column, 546-652); <uu>, <vv>, <ww>, <tt>,
+ * <uv>, <uw>, <vw>, <ut>, <wt>, <vt> (get_uv, 982-1017: all modes, kx = 0
+ * once, kx > 0 twice); then u_center and t_center (2 doubles). */
+#define PB_NSTAT_LINES 16
+int pb_statistics(pb_ctx *ctx, double *out);
 /* rk_step substeps; out may be NULL or point to rk_step records. */
 int pb_step(pb_ctx *ctx, double *dt_inout, pb_substep_out *out);
 /* Wait for the context's stream. */

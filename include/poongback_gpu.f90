@@ -85,6 +85,12 @@ module poongback_gpu
        real(c_double), intent(out) :: vv(*)
      end function pb_substep_stats
 
+     integer(c_int) function pb_statistics(ctx, out) bind(C, name="pb_statistics")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: out(*)
+     end function pb_statistics
+
      integer(c_int) function pb_sync(ctx) bind(C, name="pb_sync")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

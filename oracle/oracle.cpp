@@ -1349,6 +1349,56 @@ int orc_substep(void *h, double *Sd, double *RKd, int s, double *dt_inout, doubl
   return 0;
 }
 
+// Profiles of get_statistics_1 (get_statistics.f90:54-155) from the state
+// S(ny,nxh,nz,4) = u, v, w, theta.  out: 16 lines of ny doubles
+//   0 U (get_u, 1024-1048, + u_gal)  1 W  2 Theta
+//   3 dU/dy  4 dW/dy  5 dTheta/dy (D1 LU(D0)^-1 of the mean column, 546-652)
+//   6 uu 7 vv 8 ww 9 tt 10 uv 11 uw 12 vw 13 ut 14 wt 15 vt (get_uv, 982-1017)
+// then u_center, t_center (centre-line coefficients, 566-570, 604-607).
+static const int kUvPairs[10][2] = {{0, 0}, {1, 1}, {2, 2}, {3, 3}, {0, 1}, {0, 2}, {1, 2}, {0, 3}, {3, 2}, {1, 3}};
+int orc_statistics(void *h, const double *Sd, double *out) {
+  Ctx *c = static_cast<Ctx *>(h);
+  const int n = c->ny, bw = c->bw;
+  const size_t nl = (size_t)c->nxh * c->nz;
+  const cd *S = reinterpret_cast<const cd *>(Sd);
+  auto line = [&](int q) { return out + (size_t)q * n; };
+  const double ugal = c->cfg.u_gal;
+  const int fld[3] = {0, 2, 3};
+  for (int q = 0; q < 3; ++q) {
+    const cd *m = S + ix(*c, fld[q], 0, 0, 0);
+    for (int y = 0; y < n; ++y) line(q)[y] = m[y].re + (q == 0 ? ugal : 0.0);
+  }
+  double centre[2] = {0.0, 0.0};
+  for (int q = 0; q < 3; ++q) {
+    std::vector<cd> c1(S + ix(*c, fld[q], 0, 0, 0), S + ix(*c, fld[q], 0, 0, 0) + n), c2(n);
+    labs_bs(c->d0lu.data(), n, bw, c1.data());
+    labs_mv(c->d1.data(), n, bw, c1.data(), c2.data());
+    for (int y = 0; y < n; ++y) line(3 + q)[y] = c2[y].re;
+    if (q != 1) {
+      double a = (q == 0) ? ugal : 0.0;
+      for (int k = 0; k < n; ++k) a = a + c1[k].re * c->center[k];
+      centre[q == 0 ? 0 : 1] = a;
+    }
+  }
+  for (int q = 0; q < 10; ++q) {
+    double *st = line(6 + q);
+    for (int y = 0; y < n; ++y) st[y] = 0.0;
+    for (size_t l = 0; l < nl; ++l) {
+      const int k = (int)(l % c->nxh), j = (int)(l / c->nxh);
+      const cd *a = S + ix(*c, kUvPairs[q][0], j, k, 0), *b = S + ix(*c, kUvPairs[q][1], j, k, 0);
+      for (int y = 0; y < n; ++y) {
+        if (k == 0)
+          st[y] = st[y] + a[y].re * b[y].re + a[y].im * b[y].im;
+        else
+          st[y] = st[y] + 2.0 * (a[y].re * b[y].re + a[y].im * b[y].im);
+      }
+    }
+  }
+  out[(size_t)16 * n] = centre[0];
+  out[(size_t)16 * n + 1] = centre[1];
+  return 0;
+}
+
 // Statistics-time pressure (get_statistics_1 -> get_uv(v, v) and the PPE
 // wall values, then get_pressure.f90 after the products).
 //   Sd: state S(ny,nxh,nz,4) at the start of the substep (v = slot 1)

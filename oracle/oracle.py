@@ -44,6 +44,7 @@ def lib():
                                     ctypes.c_int, ctypes.c_int, _dp]
         L.orc_substep.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp]
         L.orc_pressure.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_statistics.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_labs_setup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_double, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
         L.orc_bspline_row.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -176,6 +177,19 @@ class Oracle:
         vv = np.zeros(self.cfg.ny)
         lib().orc_pressure(self.h, S.ctypes.data_as(_dp), P.ctypes.data_as(_dp), _p(vv))
         return vv
+
+    STAT_NAMES = ("U", "W", "Theta", "dUdy", "dWdy", "dTdy", "uu", "vv", "ww", "tt", "uv", "uw", "vw", "ut",
+                  "wt", "vt")
+
+    def statistics(self, S: np.ndarray) -> dict:
+        """get_statistics_1's profiles of the state S (4 slots): a dict of
+        ny-vectors plus u_center and t_center."""
+        n = self.cfg.ny
+        out = np.zeros(16 * n + 2)
+        lib().orc_statistics(self.h, np.ascontiguousarray(S).ctypes.data_as(_dp), _p(out))
+        d = {k: out[i * n:(i + 1) * n] for i, k in enumerate(self.STAT_NAMES)}
+        d["u_center"], d["t_center"] = out[16 * n], out[16 * n + 1]
+        return d
 
     def substep(self, S: np.ndarray, RK: np.ndarray, s: int, dt: float):
         out = np.zeros(6)

@@ -190,3 +190,28 @@ extern "C" int emu_pressure(const pb_config *cfg, double *V, double *U, double *
   }
   return 0;
 }
+
+// get_statistics_1 profiles with the product's code, device layout U
+// (9 slots; u, v, w, theta in slots 0, 1, 2, 6); out: 16*ny + 2 doubles.
+extern "C" int emu_statistics(const pb_config *cfg, double *U, double *out) {
+  Setup S;
+  if (make_setup(cfg, S)) return -1;
+  pb::LineParams P = params(cfg, S, 1, 0.0);
+  const int nxh = (int)cfg->nx / 2, n = S.n;
+  const size_t plane = (size_t)cfg->nz * nxh, field = plane * n;
+  pb::cplx *Uc = (pb::cplx *)U;
+  const int pairs[10][2] = {{0, 0}, {1, 1}, {2, 2}, {6, 6}, {0, 1}, {0, 2}, {1, 2}, {0, 6}, {6, 2}, {1, 6}};
+  for (int q = 0; q < 10; ++q)
+    for (int y = 0; y < n; ++y)
+      out[(size_t)(6 + q) * n + y] = pb::mean_uv_row(Uc + pairs[q][0] * field + (size_t)y * plane,
+                                                      Uc + pairs[q][1] * field + (size_t)y * plane, nxh, plane);
+  std::vector<double> cen(n, 0.0), vals;
+  int first = 0;
+  if (pb::basis_row(S.col, 0.0, first, vals)) return -3;
+  for (size_t j = 0; j < vals.size(); ++j) cen[first + j] = vals[j];
+  std::vector<pb::cplx> t(2 * field);
+  const pb::SV m[3] = {pb::SV{Uc, plane}, pb::SV{Uc + 2 * field, plane}, pb::SV{Uc + 6 * field, plane}};
+  pb::mean_line_stats(P, m, pb::SV{t.data(), plane}, pb::SV{t.data() + field, plane}, cen.data(), cfg->u_gal, out,
+                      out + (size_t)16 * n);
+  return 0;
+}

@@ -111,3 +111,25 @@ def test_pressure_bitwise_cpu(oracle_lib, emu_lib, name):
     assert np.array_equal(vv_e, vv_o)
     pe = np.transpose(Ud[5], (1, 2, 0))
     assert np.array_equal(pe, P[5]), np.abs(pe - P[5]).max()
+
+
+@pytest.mark.parametrize("name", ["tiny", "re180_64"])
+def test_statistics_bitwise_cpu(oracle_lib, emu_lib, name):
+    """The product's statistics code (solver.h mean_uv_row, mean_line_stats)
+    in the device layout equals the oracle bit for bit; the centre values
+    (from the product's own basis row at y = 0) to 1e-15."""
+    cfg = dataclasses.replace(preset(name), u_gal=0.1)
+    orc = oracle_lib.Oracle(cfg)
+    S = make_state(cfg, seed=41)
+    ref = orc.statistics(S)
+    U = np.zeros((9, cfg.ny, cfg.nz, cfg.nxh), np.complex128)
+    for i, f in enumerate((0, 1, 2, 6)):
+        U[f] = np.transpose(S[i], (2, 0, 1))
+    out = np.zeros(16 * cfg.ny + 2)
+    c = cfg.to_c()
+    assert emu_lib.emu_statistics(ctypes.byref(c), U.ctypes.data_as(dp), out.ctypes.data_as(dp)) == 0
+    n = cfg.ny
+    for i, k in enumerate(orc.STAT_NAMES):
+        assert np.array_equal(out[i * n:(i + 1) * n], ref[k]), k
+    assert abs(out[16 * n] - ref["u_center"]) <= 1e-15 * abs(ref["u_center"])
+    assert abs(out[16 * n + 1] - ref["t_center"]) <= 1e-15 * max(abs(ref["t_center"]), 1.0)

@@ -274,3 +274,21 @@ def test_full_size_invariants(lib, oracle_lib):
     cU = O.band_solve(orc.d0lu, orc.bw, Sg[0, 0, 0].copy())
     assert abs(0.5 * np.dot(orc.integ, cU.real) - cfg.bulk_u_value()) <= 1e-12 * cfg.bulk_u_value()
     print("[full] done", flush=True)
+
+
+@pytest.mark.parametrize("name", ["re180_64", "re1000_thin"])
+def test_statistics(lib, oracle_lib, name):
+    """get_statistics_1 profiles of the device state: bitwise against the
+    oracle (same serial Parseval order); centre values to 1e-15."""
+    cfg = dataclasses.replace(CFGS[name], u_gal=0.05)
+    orc = oracle_lib.Oracle(cfg)
+    S, RK = developed(cfg, orc)
+    ref = orc.statistics(S)
+    ch = lib.Channel(cfg)
+    ch.set_state(S, RK)
+    st = ch.statistics()
+    ch.close()
+    for k in orc.STAT_NAMES:
+        assert np.array_equal(st[k], ref[k]), (k, np.abs(st[k] - ref[k]).max())
+    assert abs(st["u_center"] - ref["u_center"]) <= 1e-15 * abs(ref["u_center"])
+    assert abs(st["t_center"] - ref["t_center"]) <= 1e-15 * max(abs(ref["t_center"]), 1.0)
