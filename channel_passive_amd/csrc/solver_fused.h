@@ -129,39 +129,98 @@ PB_HD auto win_or(const W &w, int q, const Col &y, int p) -> decltype(w.w[0]) {
   return q < BW ? w.w[q < BW ? q : 0] : y[p];
 }
 
+// Row loops.  Interior rows bw .. n-bw-1 have first(r) = r-bw and
+// last(r) = r+bw, so their packed slots are compile-time constants; the
+// loops call row(r, RowKind<true>) for them and row(r, RowKind<false>) for
+// the boundary rows.  A row's band entries are read into registers before
+// they are used (interior rows unconditionally), so the loads of a row issue
+// together instead of each waiting behind its own band condition.
+template <bool B>
+struct RowKind {
+  static constexpr bool mid = B;
+};
+template <int BW, class F>
+PB_HD void down_rows(int n, F &&row) {
+  const int hi = n - BW, lo = BW < hi ? BW : hi;
+  for (int r = 0; r < lo; ++r) row(r, RowKind<false>{});
+  for (int r = lo; r < hi; ++r) row(r, RowKind<true>{});
+  for (int r = hi > lo ? hi : lo; r < n; ++r) row(r, RowKind<false>{});
+}
+template <int BW, class F>
+PB_HD void up_rows(int n, F &&row) {
+  const int hi = n - BW, lo = BW < hi ? BW : hi;
+  for (int r = n - 1; r >= hi && r >= lo; --r) row(r, RowKind<false>{});
+  for (int r = hi - 1; r >= lo; --r) row(r, RowKind<true>{});
+  for (int r = lo - 1; r >= 0; --r) row(r, RowKind<false>{});
+}
+
+// lr[q] = L(r, r-1-q) for the columns p >= first(r) of row r (others unused),
+// lr[2bw] = L(r, r)
+template <int BW, bool MID, class M>
+PB_HD void load_lrow(const BandGeom &g, const M &L, int r, double (&lr)[2 * BW + 1]) {
+  if (MID) {
+#pragma unroll
+    for (int q = 0; q < BW; ++q) lr[q] = L(r, BW - 1 - q);
+#pragma unroll
+    for (int q = BW; q < 2 * BW; ++q) lr[q] = 0.0;
+  } else {
+    const int f = g.first(r);
+#pragma unroll
+    for (int q = 0; q < 2 * BW; ++q) {
+      const int p = r - 1 - q;
+      lr[q] = p >= f ? L(r, g.slot(r, p)) : 0.0;
+    }
+  }
+  lr[2 * BW] = L(r, BW);
+}
 // forward substitution of one row, farthest first
-template <int BW, class M>
-PB_HD cplx fwd_row(const BandGeom &g, const M &L, int r, const CWin<BW> &w, const SV &y, cplx b) {
+template <int BW, bool MID>
+PB_HD cplx fwd_row(const BandGeom &g, int r, const double (&lr)[2 * BW + 1], const CWin<BW> &w, const SV &y, cplx b) {
   const int f = g.first(r);
   cplx v = b;
 #pragma unroll
   for (int q = 2 * BW - 1; q >= 0; --q) {
     const int p = r - 1 - q;
-    if (p >= f) v = csub(v, cscale(win_or<BW>(w, q, y, p), L(r, g.slot(r, p))));
+    if (MID ? q < BW : p >= f) v = csub(v, cscale(win_or<BW>(w, q, y, p), lr[q]));
   }
-  return cdivr(v, L(r, BW));
+  return cdivr(v, lr[2 * BW]);
 }
 
+// ur[q] = U(r, r+1+q) for the columns c <= last(r) of row r (others unused)
+template <int BW, bool MID, class M>
+PB_HD void load_urow(const BandGeom &g, const M &U, int r, double (&ur)[2 * BW]) {
+  if (MID) {
+#pragma unroll
+    for (int q = 0; q < BW; ++q) ur[q] = U(r, BW + 1 + q);
+#pragma unroll
+    for (int q = BW; q < 2 * BW; ++q) ur[q] = 0.0;
+  } else {
+    const int l = g.last(r);
+#pragma unroll
+    for (int q = 0; q < 2 * BW; ++q) {
+      const int c = r + 1 + q;
+      ur[q] = c <= l ? U(r, g.slot(r, c)) : 0.0;
+    }
+  }
+}
 // back substitution of one row: w.w[q] = x(r+1+q); nearest first
-template <int BW, class M>
-PB_HD cplx bwd_row(const BandGeom &g, const M &U, int r, const CWin<2 * BW + 1> &w, cplx y) {
+template <int BW, bool MID>
+PB_HD cplx bwd_row(const BandGeom &g, int r, const double (&ur)[2 * BW], const CWin<2 * BW + 1> &w, cplx y) {
   const int l = g.last(r);
   cplx v = y;
 #pragma unroll
   for (int q = 0; q < 2 * BW; ++q) {
-    const int c = r + 1 + q;
-    if (c <= l) v = csub(v, cscale(w.w[q], U(r, g.slot(r, c))));
+    if (MID ? q < BW : r + 1 + q <= l) v = csub(v, cscale(w.w[q], ur[q]));
   }
   return v;
 }
-template <int BW, class M>
-PB_HD double bwd_row_r(const BandGeom &g, const M &U, int r, const RWin<2 * BW + 1> &w, double y) {
+template <int BW, bool MID>
+PB_HD double bwd_row_r(const BandGeom &g, int r, const double (&ur)[2 * BW], const RWin<2 * BW + 1> &w, double y) {
   const int l = g.last(r);
   double v = y;
 #pragma unroll
   for (int q = 0; q < 2 * BW; ++q) {
-    const int c = r + 1 + q;
-    if (c <= l) v = v - w.w[q] * U(r, g.slot(r, c));
+    if (MID ? q < BW : r + 1 + q <= l) v = v - w.w[q] * ur[q];
   }
   return v;
 }
@@ -215,8 +274,9 @@ struct HelmholtzA {  // (1 + beta k2) D0 - beta D2 with diagonal-only wall rows
   int n, bw;
   PB_HD double operator()(int r, int slot) const {
     if (r == 0 || r == n - 1) return slot == bw ? dummy * D0(r, bw) - beta * D2(r, bw) : 0.0;
-    return dummy * D0(r, slot) - beta * D2(r, slot);
+    return interior(r, slot);
   }
+  PB_HD double interior(int r, int slot) const { return dummy * D0(r, slot) - beta * D2(r, slot); }
 };
 struct PoissonA {  // D2 - k2 D0 with D0-diagonal wall rows
   SharedBand D0, D2;
@@ -224,8 +284,9 @@ struct PoissonA {  // D2 - k2 D0 with D0-diagonal wall rows
   int n, bw;
   PB_HD double operator()(int r, int slot) const {
     if (r == 0 || r == n - 1) return slot == bw ? D0(r, bw) : 0.0;
-    return -k2 * D0(r, slot) + D2(r, slot);
+    return interior(r, slot);
   }
+  PB_HD double interior(int r, int slot) const { return -k2 * D0(r, slot) + D2(r, slot); }
 };
 
 // One Crout row (fixed_bandec*): Lrow[q] = L(r, r-q); U(r, c) stored into Uo.
@@ -275,7 +336,7 @@ PB_HD void crout_row_mid(const M &A, const OwnU &Uo, int r, double (&Lrow)[2 * B
   for (int qd = 2 * BW; qd > BW; --qd) Lrow[qd] = 0.0;
 #pragma unroll
   for (int qd = BW; qd >= 0; --qd) {
-    double v = A(r, BW - qd);
+    double v = A.interior(r, BW - qd);
 #pragma unroll
     for (int qp = qd + 1; qp <= BW; ++qp) v = v - Lrow[qp] * uw[qp - 1][qp - qd - 1];
     Lrow[qd] = v;
@@ -284,7 +345,7 @@ PB_HD void crout_row_mid(const M &A, const OwnU &Uo, int r, double (&Lrow)[2 * B
   double un[BW];
 #pragma unroll
   for (int qc = 1; qc <= BW; ++qc) {
-    double v = A(r, BW + qc);
+    double v = A.interior(r, BW + qc);
 #pragma unroll
     for (int qp = 1; qp + qc <= BW; ++qp) v = v - Lrow[qp] * uw[qp - 1][qp + qc - 1];
     un[qc - 1] = inv * v;
@@ -298,9 +359,10 @@ PB_HD void crout_row_mid(const M &A, const OwnU &Uo, int r, double (&Lrow)[2 * B
   for (int d = 0; d < BW; ++d) uw[0][d] = un[d];
 }
 
-// Crout LU of a whole system, row by row, calling row(r, Lrow) after each
-// row: the first 2bw and the last bw rows (whose packed entries reach
-// further) read U back from Uo, the rows between keep it in registers.
+// Crout LU of a whole system, row by row, calling row(r, Lrow, kind) after
+// each row: the first 2bw and the last bw rows (whose packed entries reach
+// further) read U back from Uo, the rows between (kind RowKind<true>: no wall
+// row, first(r) = r-bw) keep it in registers.
 template <int BW, class M, class F>
 PB_HD void crout_sweep(const BandGeom &g, const M &A, const OwnU &Uo, F &&row) {
   const int n = g.n;
@@ -309,7 +371,7 @@ PB_HD void crout_sweep(const BandGeom &g, const M &A, const OwnU &Uo, F &&row) {
   double L[2 * BW + 1];
   for (int r = 0; r < r1; ++r) {
     crout_row<BW>(g, A, Uo, r, L);
-    row(r, L);
+    row(r, L, RowKind<false>{});
   }
   if (mid) {
     double uw[BW][BW];
@@ -319,35 +381,36 @@ PB_HD void crout_sweep(const BandGeom &g, const M &A, const OwnU &Uo, F &&row) {
       for (int d = 1; d <= BW; ++d) uw[qp - 1][d - 1] = Uo(r1 - qp, BW + d);
     for (int r = r1; r < r2; ++r) {
       crout_row_mid<BW>(A, Uo, r, L, uw);
-      row(r, L);
+      row(r, L, RowKind<true>{});
     }
   }
   for (int r = r2; r < n; ++r) {
     crout_row<BW>(g, A, Uo, r, L);
-    row(r, L);
+    row(r, L, RowKind<false>{});
   }
 }
 
 // forward substitution with the current row's L (Lrow[q] = L(r, r-q))
-template <int BW>
+// (interior rows, MID: the terms q <= bw, all from the window)
+template <int BW, bool MID>
 PB_HD cplx fwd_lrow(const BandGeom &g, int r, const double (&Lrow)[2 * BW + 1], const CWin<BW> &w, const SV &y,
                     cplx b) {
   const int f = g.first(r);
   cplx v = b;
 #pragma unroll
   for (int q = 2 * BW; q >= 1; --q) {
-    if (r - q >= f) v = csub(v, cscale(win_or<BW>(w, q - 1, y, r - q), Lrow[q]));
+    if (MID ? q <= BW : r - q >= f) v = csub(v, cscale(win_or<BW>(w, q - 1, y, r - q), Lrow[q]));
   }
   return cdivr(v, Lrow[0]);
 }
-template <int BW>
+template <int BW, bool MID>
 PB_HD double fwd_lrow_r(const BandGeom &g, int r, const double (&Lrow)[2 * BW + 1], const RWin<BW> &w, const RV &y,
                         double b) {
   const int f = g.first(r);
   double v = b;
 #pragma unroll
   for (int q = 2 * BW; q >= 1; --q) {
-    if (r - q >= f) v = v - win_or<BW>(w, q - 1, y, r - q) * Lrow[q];
+    if (MID ? q <= BW : r - q >= f) v = v - win_or<BW>(w, q - 1, y, r - q) * Lrow[q];
   }
   return v / Lrow[0];
 }
@@ -361,13 +424,16 @@ PB_HD void sweep_A1(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   SV o1 = c.buf(A, 0), o2 = c.buf(A, 1), o3 = c.col(A, 5);
   CWin<BW> w1, w2, w3;
   w1.clear(); w2.clear(); w3.clear();
-  for (int r = 0; r < c.n; ++r) {
-    const cplx y1 = fwd_row<BW>(g, P.D0LU, r, w1, o1, r1[r]);
-    const cplx y2 = fwd_row<BW>(g, P.D0LU, r, w2, o2, r2[r]);
-    const cplx y3 = fwd_row<BW>(g, P.D0LU, r, w3, o3, r3[r]);
+  down_rows<BW>(c.n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double lr[2 * BW + 1];
+    load_lrow<BW, MID>(g, P.D0LU, r, lr);
+    const cplx y1 = fwd_row<BW, MID>(g, r, lr, w1, o1, r1[r]);
+    const cplx y2 = fwd_row<BW, MID>(g, r, lr, w2, o2, r2[r]);
+    const cplx y3 = fwd_row<BW, MID>(g, r, lr, w3, o3, r3[r]);
     o1[r] = y1; o2[r] = y2; o3[r] = y3;
     w1.push(y1); w2.push(y2); w3.push(y3);
-  }
+  });
 }
 
 template <int BW>
@@ -376,17 +442,20 @@ PB_HD void sweep_B1(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   SV o1 = c.buf(A, 0), o2 = c.buf(A, 1), o3 = c.col(A, 5);
   CWin<2 * BW + 1> w1, w2, w3;
   w1.clear(); w2.clear(); w3.clear();
-  for (int r = c.n - 1; r >= 0; --r) {
-    const cplx x1 = bwd_row<BW>(g, P.D0LU, r, w1, o1[r]);
-    const cplx x2 = bwd_row<BW>(g, P.D0LU, r, w2, o2[r]);
-    const cplx x3 = bwd_row<BW>(g, P.D0LU, r, w3, o3[r]);
+  up_rows<BW>(c.n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double ur[2 * BW];
+    load_urow<BW, MID>(g, P.D0LU, r, ur);
+    const cplx x1 = bwd_row<BW, MID>(g, r, ur, w1, o1[r]);
+    const cplx x2 = bwd_row<BW, MID>(g, r, ur, w2, o2[r]);
+    const cplx x3 = bwd_row<BW, MID>(g, r, ur, w3, o3[r]);
     w1.push(x1); w2.push(x2); w3.push(x3);
     for_completed_rows<BW>(c.n, r, [&](int m) {
       o1[m] = mv_row<BW>(g, P.D2, m, r, w1);
       o2[m] = mv_row<BW>(g, P.D2, m, r, w2);
       o3[m] = mv_row<BW>(g, P.D2, m, r, w3);
     });
-  }
+  });
 }
 
 // A: hvhg point-wise terms (hvhg.f90:169-204, 256-258) and forward
@@ -401,20 +470,23 @@ PB_HD void sweep_A(const LineParams &P, const WsArgs &A, const SysCtx &c) {
      s7 = c.col(A, 7), s8 = c.col(A, 8);
   CWin<BW> wB, wE, wF, w8;
   wB.clear(); wE.clear(); wF.clear(); w8.clear();
-  for (int r = 0; r < c.n; ++r) {
+  down_rows<BW>(c.n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
     const cplx p1 = s0[r], p2 = s1[r], p3 = s2[r], p4 = s3[r], p5 = s4[r], p7 = s6[r], p8 = s7[r], p9 = s8[r];
+    double lr[2 * BW + 1];
+    load_lrow<BW, MID>(g, P.D0LU, r, lr);
     const cplx u1a = cadd(cscale(csub(p1, p3), kxkz), cscale(p5, kz2 - kx2));
     const cplx u3a = csub(csub(cscale(p1, -kx2), cscale(p5, 2.0 * kxkz)), cscale(p3, kz2));
     const cplx u3b = csub(cmul(ikx, p4), cmul(ikz, p2));
     const cplx u4b = cadd(cmul(ikz, p4), cmul(ikx, p2));
     const cplx u7a = cadd(cmul(ikx, p7), cmul(ikz, p9));
-    const cplx yB = fwd_row<BW>(g, P.D0LU, r, wB, s0, u3a);
-    const cplx yE = fwd_row<BW>(g, P.D0LU, r, wE, s2, u3b);
-    const cplx yF = fwd_row<BW>(g, P.D0LU, r, wF, s4, u4b);
-    const cplx y8 = fwd_row<BW>(g, P.D0LU, r, w8, s7, p8);
+    const cplx yB = fwd_row<BW, MID>(g, r, lr, wB, s0, u3a);
+    const cplx yE = fwd_row<BW, MID>(g, r, lr, wE, s2, u3b);
+    const cplx yF = fwd_row<BW, MID>(g, r, lr, wF, s4, u4b);
+    const cplx y8 = fwd_row<BW, MID>(g, r, lr, w8, s7, p8);
     s0[r] = yB; s2[r] = yE; s4[r] = yF; s7[r] = y8; s6[r] = u1a; s8[r] = u7a;
     wB.push(yB); wE.push(yE); wF.push(yF); w8.push(y8);
-  }
+  });
 }
 
 // B, split by right-hand-side group (disjoint inputs, so the split moves no
@@ -457,13 +529,16 @@ PB_HD void sweep_Bg(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   SV s2 = c.col(A, 2), s6 = c.col(A, 6), r1 = c.rk(A, 0), d1 = c.buf(A, 0);
   CWin<2 * BW + 1> cE;
   cE.clear();
-  for (int r = n - 1; r >= 0; --r) {
-    cE.push(bwd_row<BW>(g, P.D0LU, r, cE, s2[r]));
+  up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double ur[2 * BW];
+    load_urow<BW, MID>(g, P.D0LU, r, ur);
+    cE.push(bwd_row<BW, MID>(g, r, ur, cE, s2[r]));
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx hg = cadd(s6[m], mv_row<BW>(g, P.D1, m, r, cE));
       s2[m] = rhs_row(P, q, m, n, r1, d1, hg);
     });
-  }
+  });
 }
 
 template <int BW>
@@ -474,13 +549,16 @@ PB_HD void sweep_Bt(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   SV s7 = c.col(A, 7), s8 = c.col(A, 8), r3 = c.rk(A, 2), d3 = c.col(A, 5);
   CWin<2 * BW + 1> c8;
   c8.clear();
-  for (int r = n - 1; r >= 0; --r) {
-    c8.push(bwd_row<BW>(g, P.D0LU, r, c8, s7[r]));
+  up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double ur[2 * BW];
+    load_urow<BW, MID>(g, P.D0LU, r, ur);
+    c8.push(bwd_row<BW, MID>(g, r, ur, c8, s7[r]));
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx ht = cneg(cadd(s8[m], mv_row<BW>(g, P.D1, m, r, c8)));
       s7[m] = rhs_row(P, q, m, n, r3, d3, ht);
     });
-  }
+  });
 }
 
 template <int BW>
@@ -495,15 +573,18 @@ PB_HD void sweep_Bv(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   SV s0 = c.col(A, 0), s1 = c.col(A, 1), s3 = c.col(A, 3), s4 = c.col(A, 4), r2 = c.rk(A, 1), d2 = c.buf(A, 1);
   CWin<2 * BW + 1> cB, cF;
   cB.clear(); cF.clear();
-  for (int r = n - 1; r >= 0; --r) {
-    cB.push(bwd_row<BW>(g, P.D0LU, r, cB, s0[r]));
-    cF.push(bwd_row<BW>(g, P.D0LU, r, cF, s4[r]));
+  up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double ur[2 * BW];
+    load_urow<BW, MID>(g, P.D0LU, r, ur);
+    cB.push(bwd_row<BW, MID>(g, r, ur, cB, s0[r]));
+    cF.push(bwd_row<BW, MID>(g, r, ur, cF, s4[r]));
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx u5b = cadd(cadd(mv_row<BW>(g, P.D1, m, r, cB), cmul(a, s1[m])), cmul(b, s3[m]));
       const cplx hv = cadd(u5b, mv_row<BW>(g, P.D2, m, r, cF));
       s0[m] = rhs_row(P, q, m, n, r2, d2, hv);
     });
-  }
+  });
 }
 
 // C, split by matrix: Crout LU row by row fused with the forward
@@ -540,16 +621,18 @@ PB_HD void sweep_Cnu_t(const LineParams &P, const WsArgs &A, const PairCtx &pc) 
   RWin<BW> wh;
   wg.clear(); wp.clear(); vg.clear(); vp.clear(); wh.clear();
   double d00 = 0.0;
-  crout_sweep<BW>(g, H, U, [&](int r, const double(&L)[2 * BW + 1]) __attribute__((always_inline)) {
-    if (r == 0) d00 = L[0];
-    const cplx yg = fwd_lrow<BW>(g, r, L, wg, s2, s2[r]);
-    const cplx yp = fwd_lrow<BW>(g, r, L, wp, s0, s0[r]);
-    const double yh = fwd_lrow_r<BW>(g, r, L, wh, h1, r == 0 ? d00 : 0.0);
+  crout_sweep<BW>(g, H, U, [&](int r, const double(&L)[2 * BW + 1], auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    const bool r0 = !MID && r == 0;
+    if (r0) d00 = L[0];
+    const cplx yg = fwd_lrow<BW, MID>(g, r, L, wg, s2, s2[r]);
+    const cplx yp = fwd_lrow<BW, MID>(g, r, L, wp, s0, s0[r]);
+    const double yh = fwd_lrow_r<BW, MID>(g, r, L, wh, h1, r0 ? d00 : 0.0);
     s2[r] = yg; s0[r] = yp; h1[r] = yh;
     wg.push(yg); wp.push(yp); wh.push(yh);
     if (TWO) {
-      const cplx zg = fwd_lrow<BW>(g, r, L, vg, t2, t2[r]);
-      const cplx zp = fwd_lrow<BW>(g, r, L, vp, t0, t0[r]);
+      const cplx zg = fwd_lrow<BW, MID>(g, r, L, vg, t2, t2[r]);
+      const cplx zp = fwd_lrow<BW, MID>(g, r, L, vp, t0, t0[r]);
       t2[r] = zg; t0[r] = zp;
       vg.push(zg); vp.push(zp);
     }
@@ -572,12 +655,13 @@ PB_HD void sweep_Ct_t(const LineParams &P, const WsArgs &A, const PairCtx &pc, d
   SV s7 = c.col(A, 7), t7 = pc.b.col(A, 7);
   CWin<BW> wt, vt;
   wt.clear(); vt.clear();
-  crout_sweep<BW>(g, H, U, [&](int r, const double(&L)[2 * BW + 1]) __attribute__((always_inline)) {
-    const cplx yt = fwd_lrow<BW>(g, r, L, wt, s7, s7[r]);
+  crout_sweep<BW>(g, H, U, [&](int r, const double(&L)[2 * BW + 1], auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    const cplx yt = fwd_lrow<BW, MID>(g, r, L, wt, s7, s7[r]);
     s7[r] = yt;
     wt.push(yt);
     if (TWO) {
-      const cplx zt = fwd_lrow<BW>(g, r, L, vt, t7, t7[r]);
+      const cplx zt = fwd_lrow<BW, MID>(g, r, L, vt, t7, t7[r]);
       t7[r] = zt;
       vt.push(zt);
     }
@@ -613,10 +697,13 @@ PB_HD void sweep_Dnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   CWin<2 * BW + 1> wg, wp;
   RWin<2 * BW + 1> wh;
   wg.clear(); wp.clear(); wh.clear();
-  for (int r = n - 1; r >= 0; --r) {
-    wg.push(bwd_row<BW>(g, U, r, wg, s2[r]));
-    wp.push(bwd_row<BW>(g, U, r, wp, s0[r]));
-    wh.push(bwd_row_r<BW>(g, U, r, wh, y1[r]));
+  up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double ur[2 * BW];
+    load_urow<BW, MID>(g, U, r, ur);
+    wg.push(bwd_row<BW, MID>(g, r, ur, wg, s2[r]));
+    wp.push(bwd_row<BW, MID>(g, r, ur, wp, s0[r]));
+    wh.push(bwd_row_r<BW, MID>(g, r, ur, wh, y1[r]));
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx gv = mv_row<BW>(g, P.D0, m, r, wg);
       const cplx d2g = mv_row<BW>(g, P.D2, m, r, wg);
@@ -634,7 +721,7 @@ PB_HD void sweep_Dnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
       s2[m] = gv; s0[m] = pv;
       if (c.is_prim()) { h1[m] = hv; h3[m] = d2h; }
     });
-  }
+  });
 }
 
 template <int BW>
@@ -650,8 +737,11 @@ PB_HD void sweep_Dt(const LineParams &P, const WsArgs &A, const SysCtx &c, doubl
   SV s6 = c.col(A, 6), s7 = c.col(A, 7), r3 = c.rk(A, 2);
   CWin<2 * BW + 1> wt;
   wt.clear();
-  for (int r = n - 1; r >= 0; --r) {
-    wt.push(bwd_row<BW>(g, U, r, wt, s7[r]));
+  up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double ur[2 * BW];
+    load_urow<BW, MID>(g, U, r, ur);
+    wt.push(bwd_row<BW, MID>(g, r, ur, wt, s7[r]));
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx tv = mv_row<BW>(g, P.D0, m, r, wt);
       if (s != S) {
@@ -661,7 +751,7 @@ PB_HD void sweep_Dt(const LineParams &P, const WsArgs &A, const SysCtx &c, doubl
       }
       s6[m] = tv;
     });
-  }
+  });
 }
 
 // E: LU of the Poisson operator fused with the forward substitution of
@@ -680,10 +770,11 @@ PB_HD void sweep_E_t(const LineParams &P, const WsArgs &A, const PairCtx &pc) {
   CWin<BW> wp, vp;
   RWin<BW> wh;
   wp.clear(); vp.clear(); wh.clear();
-  crout_sweep<BW>(g, Q, Uq, [&](int r, const double(&L)[2 * BW + 1]) __attribute__((always_inline)) {
+  crout_sweep<BW>(g, Q, Uq, [&](int r, const double(&L)[2 * BW + 1], auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
     cplx bp = s0[r], bq = TWO ? t0[r] : mk(0.0, 0.0);
     double bh = h1[r];
-    if (r == 0 || r == n - 1) {
+    if (!MID && (r == 0 || r == n - 1)) {
       // wall rows: the value is read here rather than held in a variable
       // chosen by r, which the compiler would turn into a private-memory
       // address select
@@ -692,12 +783,12 @@ PB_HD void sweep_E_t(const LineParams &P, const WsArgs &A, const PairCtx &pc) {
       if (TWO) bq = opp ? cscale(A.vd[off + d.sys], -A.opp_alpha) : mk(0.0, 0.0);
       bh = 0.0;
     }
-    const cplx yp = fwd_lrow<BW>(g, r, L, wp, s0, bp);
-    const double yh = fwd_lrow_r<BW>(g, r, L, wh, h5, bh);
+    const cplx yp = fwd_lrow<BW, MID>(g, r, L, wp, s0, bp);
+    const double yh = fwd_lrow_r<BW, MID>(g, r, L, wh, h5, bh);
     s0[r] = yp; h5[r] = yh;
     wp.push(yp); wh.push(yh);
     if (TWO) {
-      const cplx zp = fwd_lrow<BW>(g, r, L, vp, t0, bq);
+      const cplx zp = fwd_lrow<BW, MID>(g, r, L, vp, t0, bq);
       t0[r] = zp;
       vp.push(zp);
     }
@@ -722,9 +813,12 @@ PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   CWin<2 * BW + 1> wp;
   RWin<2 * BW + 1> wh;
   wp.clear(); wh.clear();
-  for (int r = n - 1; r >= 0; --r) {
-    wp.push(bwd_row<BW>(g, Uq, r, wp, s0[r]));
-    wh.push(bwd_row_r<BW>(g, Uq, r, wh, h5[r]));
+  up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double ur[2 * BW];
+    load_urow<BW, MID>(g, Uq, r, ur);
+    wp.push(bwd_row<BW, MID>(g, r, ur, wp, s0[r]));
+    wh.push(bwd_row_r<BW, MID>(g, r, ur, wh, h5[r]));
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx vp = mv_row<BW>(g, P.D0, m, r, wp);
       const cplx dvp = mv_row<BW>(g, P.D1, m, r, wp);
@@ -733,7 +827,7 @@ PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
       s0[m] = vp; s7[m] = dvp;
       if (c.is_prim()) { h4[m] = dvh; h8[m] = d0h; }
     });
-  }
+  });
 }
 
 // G: influence coefficients and the final combination on mirror pairs
@@ -806,18 +900,26 @@ PB_HD void opp_line(const LineParams &P, const SV &v, const SV &y, int first_lo,
   const int n = g.n;
   CWin<BW> wf;
   wf.clear();
-  for (int r = 0; r < n; ++r) {
-    const cplx yr = fwd_row<BW>(g, P.D0LU, r, wf, y, v[r]);
+  down_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double lr[2 * BW + 1];
+    load_lrow<BW, MID>(g, P.D0LU, r, lr);
+    const cplx yr = fwd_row<BW, MID>(g, r, lr, wf, y, v[r]);
     y[r] = yr;
     wf.push(yr);
-  }
+  });
   CWin<2 * BW + 1> wb;
   wb.clear();
-  for (int r = n - 1; r >= first_lo; --r) {
-    wb.push(bwd_row<BW>(g, P.D0LU, r, wb, y[r]));
+  // rows below first_lo are not needed: they do nothing
+  up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    if (r < first_lo) return;
+    double ur[2 * BW];
+    load_urow<BW, MID>(g, P.D0LU, r, ur);
+    wb.push(bwd_row<BW, MID>(g, r, ur, wb, y[r]));
     if (r == first_hi) *hi = opp_interp<BW>(wb, rows + k, k);
     if (r == first_lo) *lo = opp_interp<BW>(wb, rows, k);
-  }
+  });
 }
 
 // One sweep: index is a system (kz-major, kx-minor) or, for the
