@@ -806,11 +806,39 @@ __global__ void __launch_bounds__(MEAN_THREADS) mean_kernel(LineParams P, WsArgs
     for (int i = threadIdx.x; i < n; i += MEAN_THREADS) gU(1)[(size_t)i * A.plane] = mk(0.0, 0.0);
 }
 
+// Block order of the per-system sweeps that read what is stored once per
+// (kx, +-kz) pair (U factors, homogeneous solutions): the blocks of kz rows
+// j and nz-j with the same kx range are numbered back to back on one XCD
+// (hardware blocks b and b+8 run on the same XCD), so the partner's reads
+// of the shared rows mostly hit that XCD's L2.  Other shapes keep the
+// plain order.
+__device__ __forceinline__ size_t pair_block(const WsArgs &A) {
+  const size_t G = gridDim.x, b = blockIdx.x;
+  const int cpr = A.nxh / 64;  // blocks per kz row
+  if (blockDim.x != 64 || A.nxh % 64 != 0 || G % 8 != 0 || G != (size_t)A.nz * cpr || A.nh < 1) return b;
+  const size_t L = (b % 8) * (G / 8) + b / 8;  // each XCD a contiguous range
+  const size_t npb = (size_t)(A.nh - 1) * cpr * 2;
+  int row, c;
+  if (L < npb) {
+    const size_t q = L / 2;
+    const int j = 1 + (int)(q / cpr);
+    c = (int)(q % cpr);
+    row = (L % 2) ? A.nz - j : j;
+  } else {  // kz rows 0 and nz/2 have no partner
+    const size_t l2 = L - npb;
+    row = (l2 < (size_t)cpr) ? 0 : A.nh;
+    c = (int)(l2 % cpr);
+  }
+  return (size_t)row * cpr + c;
+}
+
 // Fused sweeps over all (kx,kz) != (0,0) systems, one thread per system.
 template <int BW, int SW>
 __global__ void __launch_bounds__(64) sweep_kernel(LineParams P, WsArgs A, double *dbuf2, size_t nthreads,
                                                    const double *scal) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr bool shared_reads = SW == SW_DNU || SW == SW_DT || SW == SW_F || SW == SW_G;
+  const size_t blk = shared_reads ? pair_block(A) : (size_t)blockIdx.x;
+  const size_t idx = blk * blockDim.x + threadIdx.x;
   if (idx >= nthreads) return;
   LineParams Q = P;
   Q.dt = scal[0];
