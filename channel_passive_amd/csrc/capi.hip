@@ -97,6 +97,10 @@ struct pb_ctx {
   bool fused = true;  // fused-sweep wall-normal solver (bit-identical to the statement-by-statement port)
   bool ct_fft = true;  // fixed-size transforms where one exists for the grid
   bool ct_z_ok = false, ct_x_ok = false;
+  bool zs_on = true;   // streamed (double-buffered, one block per CU) z transforms where they apply
+  bool zs_ok = false;
+  bool zs_inv = true, zs_fwd = true;
+  int ncu = 0;         // compute units: grid of the streamed z transforms
   pb::RK3 rk;
   // per-stage event timing
   bool timing = false;
@@ -182,6 +186,11 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   c->rk = pb::rk3_coefficients();
   if (const char *e = getenv("PB_WAVESPACE_LITERAL")) c->fused = (e[0] == '0');
   if (const char *e = getenv("PB_FFT_GENERIC")) c->ct_fft = (e[0] == '0');
+  if (const char *e = getenv("PB_Z_STREAM")) {
+    c->zs_on = (e[0] != '0');
+    c->zs_inv = (e[0] != 'f');
+    c->zs_fwd = (e[0] != 'i');
+  }
   int r = pb::build_collocation(c->order, c->ny, cfg->knot_map, cfg->stretch, 2.0 * cfg->lmfy, c->col);
   if (r) { ctx_free(c); return fail(PB_E_ARG, "B-spline set-up failed (%d): check ny, labs_k, knots", r); }
   c->bw = pb::eval_bandwidth(c->col);
@@ -251,6 +260,11 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
                  hipFuncSetAttribute(reinterpret_cast<const void *>(&pb::zfwd_kernel<FZ>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, lz) == hipSuccess;
     if (c->ct_z_ok) c->Bz = FZ::B;
+    // the streamed z transforms: same passes and twiddles on 4-column tiles,
+    // two tile buffers in static LDS
+    const int lzs = (2 * pb::zs::NE + pb::zs::NTW) * (int)sizeof(double2);
+    c->zs_ok = c->zs_on && c->ct_z_ok && c->nxh % pb::zs::B == 0 && lzs <= lds_max &&
+               hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c->ncu > 0;
     using FX = pb::FftX1536;
     c->ct_x_ok = c->ct_fft && c->nx32 == FX::N && FX::N * FX::B * (int)sizeof(double2) <= lds_max;
   }
@@ -561,10 +575,26 @@ static int launch_zfwd_t(pb_ctx *c, int want_vv) {
 static bool ct_z(const pb_ctx *c) { return c->ct_z_ok; }
 static bool ct_x(const pb_ctx *c) { return c->ct_x_ok; }
 
+static int launch_zinv_stream(pb_ctx *c) {
+  StageTimer tm(c, PB_STAGE_ZINV);
+  hipLaunchKernelGGL(pb::zinv_stream_kernel, dim3(c->ncu), dim3(pb::zs::T), 0, c->stream, c->U, c->pad, c->twz, c->nxh,
+                     c->nz, c->ny, c->field, c->pfield);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+static int launch_zfwd_stream(pb_ctx *c, int want_vv) {
+  StageTimer tm(c, PB_STAGE_ZFWD);
+  hipLaunchKernelGGL(pb::zfwd_stream_kernel, dim3(c->ncu), dim3(pb::zs::T), 0, c->stream, c->pad, c->U, c->twz,
+                     c->nxh, c->nz, c->ny, c->field, c->pfield, c->rescale, want_vv ? 9 : 8);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+
 static int launch_nonlinear(pb_ctx *c, int update_dt, int want_vv) {
   int r = PB_OK;
   const int ez = pick_ept((size_t)c->nz32 * c->Bz), ex = pick_ept((size_t)c->nx32 * 2);
-  if (ct_z(c)) r = launch_zinv_t<pb::FftZ1152>(c);
+  if (c->zs_ok && c->zs_inv) r = launch_zinv_stream(c);
+  else if (ct_z(c)) r = launch_zinv_t<pb::FftZ1152>(c);
   else PB_EPT_DISPATCH(ez, r = launch_zinv_t<F>(c));
   if (r) return r;
   if (ct_x(c)) r = launch_xphys_t<pb::FftX1536>(c, want_vv);
@@ -576,7 +606,8 @@ static int launch_nonlinear(pb_ctx *c, int update_dt, int want_vv) {
                        c->scal, update_dt, c->cfg.dt_factor, c->cflx);
     HIPCHK(hipGetLastError());
   }
-  if (ct_z(c)) r = launch_zfwd_t<pb::FftZ1152>(c, want_vv);
+  if (c->zs_ok && c->zs_fwd) r = launch_zfwd_stream(c, want_vv);
+  else if (ct_z(c)) r = launch_zfwd_t<pb::FftZ1152>(c, want_vv);
   else PB_EPT_DISPATCH(ez, r = launch_zfwd_t<F>(c, want_vv));
   return r;
 }

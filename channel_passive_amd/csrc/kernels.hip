@@ -313,7 +313,24 @@ struct Lines {
   }
 };
 
-template <int T, class L, int N, int B, int SIGN, int Ns, int R>
+// Barrier between the passes.  LDS_ONLY waits for this wave's LDS operations
+// only (lgkmcnt), not for its outstanding global loads: a kernel that copies
+// its next tile global -> LDS while transforming the current one must not
+// wait for that copy at every pass.  The empty asm statements keep the
+// compiler from moving LDS accesses across the barrier.
+template <bool LDS_ONLY>
+__device__ __forceinline__ void pass_sync() {
+  if (LDS_ONLY) {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt, expcnt not waited for
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  } else {
+    __syncthreads();
+  }
+}
+
+template <int T, class L, int N, int B, int SIGN, int Ns, int R, bool LDS_ONLY = false>
 __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ tw) {
   constexpr int NR = N / R, NBF = NR * B, ROUNDS = (NBF + T - 1) / T;
   constexpr int tstep = N / (Ns * R);
@@ -328,7 +345,7 @@ __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ 
       for (int r = 0; r < R; ++r) v[q][r] = x[L::at(j + r * NR, b)];
     }
   }
-  __syncthreads();
+  pass_sync<LDS_ONLY>();
 #pragma unroll
   for (int q = 0; q < ROUNDS; ++q) {
     const int t = threadIdx.x + q * T;
@@ -361,12 +378,17 @@ __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ 
       for (int r = 0; r < R; ++r) x[L::at(base + r * Ns, b)] = v[q][r];
     }
   }
-  __syncthreads();
+  pass_sync<LDS_ONLY>();
 }
 template <int T, class L, int N, int B, int SIGN, int Ns, int R, int... Rest>
 __device__ __forceinline__ void ct_passes(double2 *x, const double2 *__restrict__ tw) {
   ct_pass<T, L, N, B, SIGN, Ns, R>(x, tw);
   if constexpr (sizeof...(Rest) > 0) ct_passes<T, L, N, B, SIGN, Ns * R, Rest...>(x, tw);
+}
+template <int T, class L, int N, int B, int SIGN, int Ns, int R, int... Rest>
+__device__ __forceinline__ void ct_passes_lds(double2 *x, const double2 *__restrict__ tw) {
+  ct_pass<T, L, N, B, SIGN, Ns, R, true>(x, tw);
+  if constexpr (sizeof...(Rest) > 0) ct_passes_lds<T, L, N, B, SIGN, Ns * R, Rest...>(x, tw);
 }
 
 // Transform policies for the three kernels: N = B = 0 means sizes at run
@@ -571,15 +593,28 @@ __global__ void __launch_bounds__(F::T, F::MINW) zfwd_kernel(const double2 *__re
   }
 }
 
-// ------------------------------------ persistent z transforms (fixed size)
-// One block per CU walks a strided list of tiles (B kx columns of one y and
-// one field).  The next tile's global data is loaded into registers while
-// the current tile is transformed in LDS, so HBM keeps streaming during the
-// FFT; the twiddles are copied to LDS once, so the transform itself issues
-// no global loads (whose in-order completion counter would otherwise make
-// the first twiddle wait for the prefetch as well).  Tiles are numbered
-// kx tile fastest; each XCD takes a contiguous run of blocks, so the two
-// blocks that share a 128-byte row segment run in one L2 at the same time.
+// ---------------------------------------- streamed z transforms (N = 1152)
+// One block per CU walks a strided list of tiles (4 kx columns of one y and
+// one field) and keeps two tile buffers in LDS.  The next tile is copied
+// global -> LDS directly (global_load_lds_dwordx4, no registers) while the
+// current one is transformed, so HBM keeps streaming during the FFT; the
+// passes wait for LDS operations only.  The buffers are separate arrays and
+// the loop body is written out once per buffer, so the compiler can tell
+// that the transform's LDS accesses do not touch the buffer being filled and
+// does not hold them until the copy has landed.  Tiles are numbered kx tile
+// fastest; each XCD takes a contiguous run of blocks.
+namespace zs {
+constexpr int N = 1152, B = 4, NE = N * B, T = 768, NTW = N / 8;  // twiddles W^k, k < N/8
+using L = Lines<N, B, false>;
+template <int SIGN>
+__device__ __forceinline__ void fft(double2 *x, const double2 *tw) {
+  ct_passes_lds<T, L, N, B, SIGN, 1, 8, 12, 12>(x, tw);
+}
+}  // namespace zs
+__shared__ double2 zs_buf0[zs::NE];
+__shared__ double2 zs_buf1[zs::NE];
+__shared__ double2 zs_tw[zs::NTW];
+
 struct ZTile {
   int x, y, f;
 };
@@ -595,101 +630,118 @@ __device__ __forceinline__ size_t xcd_rank() {
   const size_t G = gridDim.x;
   return (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
 }
+// everything this wave has issued, LDS copies included, has landed
+__device__ __forceinline__ void all_sync() {
+  __builtin_amdgcn_s_waitcnt(0);
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+// Copy of element e = swz(p) of a tile into LDS slot p (the swizzle is an
+// involution, so slot p = L::at(e / B, e % B)); the wave's 64 lanes fill 64
+// consecutive slots.  row(z) gives the global row of tile row z, or < 0 for
+// a row of zeros (written by the threads; the copy skips those lanes).
+// The zeros are written before any copy is issued: an LDS store issued
+// after a copy into the same buffer would have to wait for that copy.
+template <bool ZEROS, class Row>
+__device__ __forceinline__ void zs_issue(const double2 *src, int nxh, double2 *buf, Row row) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (ZEROS) {
+    for (int p = threadIdx.x; p < zs::NE; p += zs::T)
+      if (row(lds_swz(p) / zs::B) < 0) buf[p] = c2(0.0, 0.0);
+  }
+  for (int i = wave; i < zs::NE / 64; i += zs::T / 64) {
+    const int p = i * 64 + lane, e = lds_swz(p);
+    const int gr = row(e / zs::B);
+    if (!ZEROS || gr >= 0)
+      __builtin_amdgcn_global_load_lds((const void *)(src + (size_t)gr * nxh + e % zs::B),
+                                       (__attribute__((address_space(3))) void *)(buf + i * 64), 16, 0, 0);
+  }
+}
 
-// y_to_x for the 4 state fields (slots 0,1,2,6), as zinv_kernel
-template <class F>
-__global__ void __launch_bounds__(F::T, F::MINW) zinv_persist(const double2 *__restrict__ U, double2 *__restrict__ pad,
-                                                              const double2 *__restrict__ twg, int nxh, int nz,
-                                                              int ny, size_t field, size_t pfield) {
-  constexpr int N = F::N, B = F::B, T = F::T, E = (N * B + T - 1) / T;
-  extern __shared__ double2 lds[];  // N*B data, then N twiddles
-  double2 *tw = lds + N * B;
-  for (int i = threadIdx.x; i < N; i += T) tw[i] = twg[i];
-  const int ntx = nxh / B, nh = nz / 2;
+// y_to_x for the 4 state fields (slots 0,1,2,6), as zinv_kernel; grid = one
+// block per CU, tiles ntx*ny*4
+__device__ __forceinline__ void zinv_stream_tile(double2 *x, double2 *__restrict__ pad, size_t t, int ntx, int ny,
+                                                 int nxh, size_t pfield) {
+  zs::fft<+1>(x, zs_tw);
+  const ZTile z = z_tile(t, ntx, ny);
+  double2 *dst = pad + (size_t)z.f * pfield + (size_t)z.y * zs::N * nxh + (size_t)z.x * zs::B;
+  for (int e = threadIdx.x; e < zs::NE; e += zs::T) {
+    const int zz = e / zs::B, b = e % zs::B;
+    dst[(size_t)zz * nxh + b] = x[zs::L::at(zz, b)];
+  }
+  pass_sync<true>();  // every wave has read x before it is refilled
+}
+__global__ void __launch_bounds__(zs::T, 3) zinv_stream_kernel(const double2 *__restrict__ U, double2 *__restrict__ pad,
+                                                               const double2 *__restrict__ twg, int nxh, int nz,
+                                                               int ny, size_t field, size_t pfield) {
+  for (int i = threadIdx.x; i < zs::NTW; i += zs::T) zs_tw[i] = twg[i];
+  const int ntx = nxh / zs::B, nh = nz / 2;
   const size_t ntiles = (size_t)ntx * ny * 4, G = gridDim.x;
-  double2 pre[E];
-  auto load = [&](size_t t) {
+  // padded row zz <- state row zz (zz < nz/2) or zz - nz/2 (zz > nz); zeros between
+  auto issue = [&](size_t t, double2 *buf) {
     const ZTile z = z_tile(t, ntx, ny);
     const int slot = (z.f == 3) ? 6 : z.f;
-    const double2 *src = U + (size_t)slot * field + (size_t)z.y * nz * nxh + (size_t)z.x * B;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int idx = threadIdx.x + e * T;
-      if ((N * B) % T == 0 || idx < N * B) {
-        const int zz = idx / B, b = idx % B;
-        double2 val = c2(0.0, 0.0);
-        if (zz < nh) val = src[(size_t)zz * nxh + b];
-        else if (zz > 2 * nh) val = src[(size_t)(zz - nh) * nxh + b];
-        pre[e] = val;
-      }
-    }
+    const double2 *src = U + (size_t)slot * field + (size_t)z.y * nz * nxh + (size_t)z.x * zs::B;
+    zs_issue<true>(src, nxh, buf, [&](int zz) { return zz < nh ? zz : (zz > 2 * nh ? zz - nh : -1); });
   };
   size_t t = xcd_rank();
-  if (t < ntiles) load(t);
-  for (; t < ntiles; t += G) {
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int idx = threadIdx.x + e * T;
-      if ((N * B) % T == 0 || idx < N * B) lds[F::at(idx / B, idx % B, B)] = pre[e];
-    }
-    __syncthreads();
-    if (t + G < ntiles) load(t + G);
-    F::run(lds, B, FftPlan{}, tw, +1);
-    const ZTile z = z_tile(t, ntx, ny);
-    double2 *dst = pad + (size_t)z.f * pfield + (size_t)z.y * N * nxh + (size_t)z.x * B;
-    for (int e = threadIdx.x; e < N * B; e += T) {
-      const int zz = e / B, b = e % B;
-      dst[(size_t)zz * nxh + b] = lds[F::at(zz, b, B)];
-    }
-    __syncthreads();
+  if (t < ntiles) issue(t, zs_buf0);
+  while (t < ntiles) {
+    all_sync();  // tile t is in buffer 0
+    if (t + G < ntiles) issue(t + G, zs_buf1);
+    zinv_stream_tile(zs_buf0, pad, t, ntx, ny, nxh, pfield);
+    t += G;
+    if (t >= ntiles) break;
+    all_sync();  // tile t is in buffer 1
+    if (t + G < ntiles) issue(t + G, zs_buf0);
+    zinv_stream_tile(zs_buf1, pad, t, ntx, ny, nxh, pfield);
+    t += G;
   }
 }
 
 // x_to_y + rescale for nf = 8 (or 9) product fields, as zfwd_kernel
-template <class F>
-__global__ void __launch_bounds__(F::T, F::MINW) zfwd_persist(const double2 *__restrict__ pad, double2 *__restrict__ U,
-                                                              const double2 *__restrict__ twg, int nxh, int nz,
-                                                              int ny, size_t field, size_t pfield, double rescale,
-                                                              int nf) {
-  constexpr int N = F::N, B = F::B, T = F::T, E = (N * B + T - 1) / T;
-  extern __shared__ double2 lds[];
-  double2 *tw = lds + N * B;
-  for (int i = threadIdx.x; i < N; i += T) tw[i] = twg[i];
-  const int ntx = nxh / B, nh = nz / 2;
+__device__ __forceinline__ void zfwd_stream_tile(double2 *x, double2 *__restrict__ U, size_t t, int ntx, int ny,
+                                                 int nxh, int nz, size_t field, double rescale) {
+  zs::fft<-1>(x, zs_tw);
+  const ZTile z = z_tile(t, ntx, ny);
+  const int nh = nz / 2;
+  const int slot = z.f < 5 ? z.f : (z.f < 8 ? z.f + 1 : 5);
+  double2 *dst = U + (size_t)slot * field + (size_t)z.y * nz * nxh + (size_t)z.x * zs::B;
+  for (int e = threadIdx.x; e < nz * zs::B; e += zs::T) {
+    const int j = e / zs::B, b = e % zs::B;
+    double2 val;
+    if (j < nh) val = x[zs::L::at(j, b)];
+    else if (j == nh) val = c2(0.0, 0.0);
+    else val = x[zs::L::at(j + nh, b)];
+    dst[(size_t)j * nxh + b] = c2(val.x * rescale, val.y * rescale);
+  }
+  pass_sync<true>();
+}
+__global__ void __launch_bounds__(zs::T, 3) zfwd_stream_kernel(const double2 *__restrict__ pad, double2 *__restrict__ U,
+                                                               const double2 *__restrict__ twg, int nxh, int nz,
+                                                               int ny, size_t field, size_t pfield, double rescale,
+                                                               int nf) {
+  for (int i = threadIdx.x; i < zs::NTW; i += zs::T) zs_tw[i] = twg[i];
+  const int ntx = nxh / zs::B;
   const size_t ntiles = (size_t)ntx * ny * nf, G = gridDim.x;
-  double2 pre[E];
-  auto load = [&](size_t t) {
+  auto issue = [&](size_t t, double2 *buf) {
     const ZTile z = z_tile(t, ntx, ny);
-    const double2 *src = pad + (size_t)z.f * pfield + (size_t)z.y * N * nxh + (size_t)z.x * B;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int idx = threadIdx.x + e * T;
-      if ((N * B) % T == 0 || idx < N * B) pre[e] = src[(size_t)(idx / B) * nxh + idx % B];
-    }
+    const double2 *src = pad + (size_t)z.f * pfield + (size_t)z.y * zs::N * nxh + (size_t)z.x * zs::B;
+    zs_issue<false>(src, nxh, buf, [](int zz) { return zz; });
   };
   size_t t = xcd_rank();
-  if (t < ntiles) load(t);
-  for (; t < ntiles; t += G) {
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int idx = threadIdx.x + e * T;
-      if ((N * B) % T == 0 || idx < N * B) lds[F::at(idx / B, idx % B, B)] = pre[e];
-    }
-    __syncthreads();
-    if (t + G < ntiles) load(t + G);
-    F::run(lds, B, FftPlan{}, tw, -1);
-    const ZTile z = z_tile(t, ntx, ny);
-    const int slot = z.f < 5 ? z.f : (z.f < 8 ? z.f + 1 : 5);
-    double2 *dst = U + (size_t)slot * field + (size_t)z.y * nz * nxh + (size_t)z.x * B;
-    for (int e = threadIdx.x; e < nz * B; e += T) {
-      const int j = e / B, b = e % B;
-      double2 val;
-      if (j < nh) val = lds[F::at(j, b, B)];
-      else if (j == nh) val = c2(0.0, 0.0);
-      else val = lds[F::at(j + nh, b, B)];
-      dst[(size_t)j * nxh + b] = c2(val.x * rescale, val.y * rescale);
-    }
-    __syncthreads();
+  if (t < ntiles) issue(t, zs_buf0);
+  while (t < ntiles) {
+    all_sync();
+    if (t + G < ntiles) issue(t + G, zs_buf1);
+    zfwd_stream_tile(zs_buf0, U, t, ntx, ny, nxh, nz, field, rescale);
+    t += G;
+    if (t >= ntiles) break;
+    all_sync();
+    if (t + G < ntiles) issue(t + G, zs_buf0);
+    zfwd_stream_tile(zs_buf1, U, t, ntx, ny, nxh, nz, field, rescale);
+    t += G;
   }
 }
 

@@ -8,6 +8,7 @@ per field of the specification.  At the full Re_tau=1000 size, where the
 oracle does not fit a test, size-independent invariants are checked.
 """
 import dataclasses
+import os
 
 import numpy as np
 import pytest
@@ -132,6 +133,27 @@ def test_nonlinear_full_spectrum(lib, oracle_lib, name):
     assert not np.any(P[:9, cfg.nz // 2])
     assert abs(vm - vm_ref) <= 1e-13 * vm_ref
     ch.close()
+
+
+def test_z_stream_bitwise(lib):
+    """The streamed z transforms (two LDS tile buffers per CU, 4-column tiles)
+    run the same passes with the same twiddles as the one-tile-per-block
+    kernels (8-column tiles): identical products, also with the v*v field."""
+    cfg = CFGS["re1000_flat"]
+    S = full_spectrum(cfg, seed=7)
+    out = {}
+    for mode in ("1", "0"):
+        os.environ["PB_Z_STREAM"] = mode
+        try:
+            ch = lib.Channel(cfg)
+        finally:
+            del os.environ["PB_Z_STREAM"]
+        out[mode] = ch.nonlinear_host(S, want_vv=True)
+        ch.close()
+    (P1, vm1), (P0, vm0) = out["1"], out["0"]
+    for slot in range(9):
+        assert np.array_equal(P1[slot], P0[slot]), slot
+    assert vm1 == vm0
 
 
 @pytest.mark.parametrize("name", ["re180_64", "re180_192", "re1000_thin"])
