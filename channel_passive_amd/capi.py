@@ -54,6 +54,7 @@ SIGNATURES = {
     "pb_substep_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(PbSubstepOut), _dp,
                                         _dp]),
     "pb_statistics": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "pb_spectra": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "pb_step": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(PbSubstepOut)]),
     "pb_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "pb_wavespace_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp,
@@ -173,6 +174,19 @@ class Channel:
         d = {k: out[i * n:(i + 1) * n] for i, k in enumerate(self.STAT_NAMES)}
         d["u_center"], d["t_center"] = out[16 * n], out[16 * n + 1]
         return d
+
+    # write_energy_spectra.f90's datasets in its order (pb_spectra)
+    SPECTRA_NAMES = ("uu", "vv", "ww", "tt", "uv", "vw", "uw", "ut", "vt", "wt", "oxox", "oyoy", "ozoz",
+                     "oxoy", "oxoz", "oyoz", "voz", "woy")
+
+    def spectra(self):
+        """1-D energy (co)spectra of the current state: (ex, ez) with
+        ex[q, kx, y] summed over kz and ez[q, kz, y] summed over kx."""
+        c = self.cfg
+        ex = np.zeros((18, c.nxh, c.ny))
+        ez = np.zeros((18, c.nz, c.ny))
+        _check(load().pb_spectra(self.h, ex.ctypes.data_as(_dp), ez.ctypes.data_as(_dp)))
+        return ex, ez
 
     def step(self, dt: float):
         d = ctypes.c_double(dt)

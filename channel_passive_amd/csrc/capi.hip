@@ -812,6 +812,41 @@ int pb_statistics(pb_ctx *c, double *out) {
   return PB_OK;
 }
 
+int pb_spectra(pb_ctx *c, double *ex, double *ez) {
+  if (!c || (!ex && !ez)) return fail(PB_E_ARG, "null argument");
+  if (!c->have_state) return fail(PB_E_STATE, "pb_spectra before pb_set_state");
+  if (c->bw < 3 || c->bw > 5) return fail(PB_E_ARG, "pb_spectra: bandwidth %d outside 3..5", c->bw);
+  const pb::LineParams P = c->line_params(1, 0.0);
+  pb::cplx *U = (pb::cplx *)c->U, *buf = (pb::cplx *)c->buf;
+  const dim3 b64(64), grid(blocks_for(c->plane, 64));
+  switch (c->bw) {
+    case 3: hipLaunchKernelGGL(pb::deriv_kernel<3>, grid, b64, 0, c->stream, P, U, buf, c->field, c->plane); break;
+    case 4: hipLaunchKernelGGL(pb::deriv_kernel<4>, grid, b64, 0, c->stream, P, U, buf, c->field, c->plane); break;
+    case 5: hipLaunchKernelGGL(pb::deriv_kernel<5>, grid, b64, 0, c->stream, P, U, buf, c->field, c->plane); break;
+  }
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(pb::vort_kernel, dim3(blocks_for(c->field, 256)), dim3(256), 0, c->stream, U,
+                     (const pb::cplx *)buf, c->field, c->plane, c->nxh, c->nh, c->lmfxr, c->lmfzr, c->field);
+  HIPCHK(hipGetLastError());
+  // outputs staged in the padded-field buffer (free between substeps)
+  double *dex = reinterpret_cast<double *>(c->pad), *dez = dex + (size_t)PB_NSPECTRA * c->nxh * c->ny;
+  const size_t nx_out = (size_t)PB_NSPECTRA * c->nxh * c->ny, nz_out = (size_t)PB_NSPECTRA * c->nz * c->ny;
+  if (ex) {
+    hipLaunchKernelGGL(pb::spectra_x_kernel, dim3(blocks_for(nx_out, 256)), dim3(256), 0, c->stream,
+                       (const pb::cplx *)U, c->field, c->plane, c->ny, c->nxh, c->nz, dex);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ex, dex, nx_out * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  if (ez) {
+    hipLaunchKernelGGL(pb::spectra_z_kernel, dim3(blocks_for(nz_out, 256)), dim3(256), 0, c->stream,
+                       (const pb::cplx *)U, c->field, c->plane, c->ny, c->nxh, c->nz, dez);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ez, dez, nz_out * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PB_OK;
+}
+
 int pb_step(pb_ctx *c, double *dt_inout, pb_substep_out *out) {
   if (!c || !dt_inout) return fail(PB_E_ARG, "null argument");
   if (!c->have_state) return fail(PB_E_STATE, "pb_step before pb_set_state");

@@ -952,6 +952,71 @@ __global__ void stats_mean_kernel(LineParams P, const cplx *U, cplx *tmp, size_t
   mean_line_stats(P, m, SV{tmp, plane}, SV{tmp + field, plane}, center, u_gal, stat, stat + (size_t)16 * P.g.n);
 }
 
+// ------------------------------------------ 1-D energy spectra (write_spectra)
+// du/dy, dw/dy of every system, (0,0) included, into the scratch columns
+template <int BW>
+__global__ void __launch_bounds__(64) deriv_kernel(LineParams P, cplx *U, cplx *buf, size_t field, size_t plane) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= plane) return;
+  deriv_line<BW>(P, SV{U + sys, plane}, SV{U + 2 * field + sys, plane}, SV{buf + sys, plane},
+                 SV{buf + field + sys, plane});
+}
+// vorticity into slots 3, 4, 5 (get_vorticity_field, get_statistics.f90:912-948):
+// ox = dw/dy - (0,kz) v, oy = (0,kz) u - (0,kx) w, oz = (0,kx) v - du/dy with (0,k) = dcmplx(0d0,k)
+__global__ void vort_kernel(cplx *U, const cplx *buf, size_t field, size_t plane, int nxh, int nh, double lmfxr,
+                            double lmfzr, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t sys = i % plane;
+  const int k = (int)(sys % nxh), j = (int)(sys / nxh);
+  const int kzi = (j <= nh) ? j : j + 1 - 2 * nh - 1;  // module_ppdfftw.f90:360-374
+  const cplx ikx = mk(0.0, k * lmfxr), ikz = mk(0.0, kzi * lmfzr);
+  const cplx u = U[i], v = U[field + i], w = U[2 * field + i], du = buf[i], dw = buf[field + i];
+  U[3 * field + i] = csub(dw, cmul(ikz, v));
+  U[4 * field + i] = csub(cmul(ikz, u), cmul(ikx, w));
+  U[5 * field + i] = csub(cmul(ikx, v), du);
+}
+// field ids: 0 u, 1 v, 2 w, 3 theta, 4 ox, 5 oy, 6 oz -> device slots
+__constant__ int kSpecSlot[7] = {0, 1, 2, 6, 3, 4, 5};
+// pairs in write_energy_spectra.f90's dataset order (86-239); its kz list
+// pairs "Eozoz_kz" with (ox, oz) (218-219)
+__constant__ int kSpecX[18][2] = {{0, 0}, {1, 1}, {2, 2}, {3, 3}, {0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3},
+                                  {3, 2}, {4, 4}, {5, 5}, {6, 6}, {4, 5}, {4, 6}, {5, 6}, {1, 6}, {2, 5}};
+__constant__ int kSpecZ[18][2] = {{0, 0}, {1, 1}, {2, 2}, {3, 3}, {0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3},
+                                  {2, 3}, {4, 4}, {5, 5}, {4, 6}, {4, 5}, {4, 6}, {5, 6}, {1, 6}, {2, 5}};
+__device__ __forceinline__ double spec_term(cplx a, cplx b) { return a.re * b.re + a.im * b.im; }
+// E_x(q, kx, y): one thread per output, serial sum over the kz index in
+// storage order (get_E_uu_x, write_energy_spectra.f90:326-359)
+__global__ void spectra_x_kernel(const cplx *U, size_t field, size_t plane, int ny, int nxh, int nz, double *ex) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)18 * ny * nxh) return;
+  const int q = (int)(t / ((size_t)ny * nxh)), rem = (int)(t % ((size_t)ny * nxh));
+  const int y = rem / nxh, kx = rem % nxh;
+  const cplx *a = U + (size_t)kSpecSlot[kSpecX[q][0]] * field + (size_t)y * plane + kx;
+  const cplx *b = U + (size_t)kSpecSlot[kSpecX[q][1]] * field + (size_t)y * plane + kx;
+  double st = 0.0;
+  for (int j = 0; j < nz; ++j) {
+    const double tm = spec_term(a[(size_t)j * nxh], b[(size_t)j * nxh]);
+    st = (kx == 0) ? st + tm : st + 2.0 * tm;
+  }
+  ex[((size_t)q * nxh + kx) * ny + y] = st;
+}
+// E_z(q, kz, y): serial sum over kx ascending (get_E_uu_z, 366-399)
+__global__ void spectra_z_kernel(const cplx *U, size_t field, size_t plane, int ny, int nxh, int nz, double *ez) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)18 * ny * nz) return;
+  const int q = (int)(t / ((size_t)ny * nz)), rem = (int)(t % ((size_t)ny * nz));
+  const int y = rem / nz, j = rem % nz;
+  const cplx *a = U + (size_t)kSpecSlot[kSpecZ[q][0]] * field + (size_t)y * plane + (size_t)j * nxh;
+  const cplx *b = U + (size_t)kSpecSlot[kSpecZ[q][1]] * field + (size_t)y * plane + (size_t)j * nxh;
+  double st = 0.0;
+  for (int k = 0; k < nxh; ++k) {
+    const double tm = spec_term(a[k], b[k]);
+    st = (k == 0) ? st + tm : st + 2.0 * tm;
+  }
+  ez[((size_t)q * nz + j) * ny + y] = st;
+}
+
 // wall values of the pressure Poisson equation from v (slot 1), per system
 __global__ void __launch_bounds__(64) ppe_bc_kernel(LineParams P, const cplx *U, cplx *tmp, cplx *bc, size_t plane,
                                                     size_t nsys, double nu) {

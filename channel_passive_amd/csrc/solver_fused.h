@@ -881,6 +881,40 @@ PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   }
 }
 
+// y derivative of two columns of one system, d = D1 LU(D0)^-1 x, into the
+// columns o1, o2 (which first hold LU(D0)'s forward substitution): the
+// arithmetic of labs_bs then labs_mat_mul (get_y_dir_deriv_of_field,
+// get_statistics.f90:658-681)
+template <int BW>
+PB_HD void deriv_line(const LineParams &P, const SV &x1, const SV &x2, const SV &o1, const SV &o2) {
+  const BandGeom g{P.g.n, BW};
+  const int n = g.n;
+  CWin<BW> f1, f2;
+  f1.clear(); f2.clear();
+  down_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double lr[2 * BW + 1];
+    load_lrow<BW, MID>(g, P.D0LU, r, lr);
+    const cplx y1 = fwd_row<BW, MID>(g, r, lr, f1, o1, x1[r]);
+    const cplx y2 = fwd_row<BW, MID>(g, r, lr, f2, o2, x2[r]);
+    o1[r] = y1; o2[r] = y2;
+    f1.push(y1); f2.push(y2);
+  });
+  CWin<2 * BW + 1> w1, w2;
+  w1.clear(); w2.clear();
+  up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double ur[2 * BW];
+    load_urow<BW, MID>(g, P.D0LU, r, ur);
+    w1.push(bwd_row<BW, MID>(g, r, ur, w1, o1[r]));
+    w2.push(bwd_row<BW, MID>(g, r, ur, w2, o2[r]));
+    for_completed_rows<BW>(n, r, [&](int m) {
+      o1[m] = mv_row<BW>(g, P.D1, m, r, w1);
+      o2[m] = mv_row<BW>(g, P.D1, m, r, w2);
+    });
+  });
+}
+
 // Opposition-control detection planes of one system: c = LU(D0)^-1 v by a
 // down and an up sweep with the arithmetic of band_bs (y stored in the
 // scratch column), then v(y_d) = sum_j B_j(y_d) c_(first+j), ascending j,

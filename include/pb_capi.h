@@ -136,6 +136,20 @@ int pb_substep_stats(pb_ctx *ctx, int32_t s, double *dt_inout, pb_substep_out *o
  * once, kx > 0 twice); then u_center and t_center (2 doubles). */
 #define PB_NSTAT_LINES 16
 int pb_statistics(pb_ctx *ctx, double *out);
+/* 1-D energy (co)spectra of the state held between steps
+ * (write_energy_spectra.f90:9-300, replacing its get_E_uu_x/get_E_uu_z
+ * datasets; the HDF5 output stays with the caller).  du/dy and dw/dy are
+ * D1 LU(D0)^-1 of every line (get_y_dir_deriv_of_field,
+ * get_statistics.f90:658-681), the vorticity follows get_vorticity_field
+ * (912-948).  ex receives PB_NSPECTRA planes (nx/2, ny) of sums over kz
+ * (get_E_uu_x, write_energy_spectra.f90:326-359), ez PB_NSPECTRA planes
+ * (nz, ny) of sums over kx (get_E_uu_z, 366-399), y fastest, kx = 0 once and
+ * kx > 0 twice, in the reference's dataset order: uu vv ww tt uv vw uw ut vt
+ * wt oxox oyoy ozoz oxoy oxoz oyoz voz woy (its kz "ozoz" is the (ox, oz)
+ * co-spectrum, as in write_energy_spectra.f90:218-219).  Either output may be
+ * NULL.  Uses product slots 3-5 and the scratch columns; the state is kept. */
+#define PB_NSPECTRA 18
+int pb_spectra(pb_ctx *ctx, double *ex, double *ez);
 /* rk_step substeps; out may be NULL or point to rk_step records. */
 int pb_step(pb_ctx *ctx, double *dt_inout, pb_substep_out *out);
 /* Wait for the context's stream. */

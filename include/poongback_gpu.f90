@@ -91,6 +91,12 @@ module poongback_gpu
        real(c_double), intent(out) :: out(*)
      end function pb_statistics
 
+     integer(c_int) function pb_spectra(ctx, ex, ez) bind(C, name="pb_spectra")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: ex(*), ez(*)
+     end function pb_spectra
+
      integer(c_int) function pb_sync(ctx) bind(C, name="pb_sync")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -1399,6 +1399,79 @@ int orc_statistics(void *h, const double *Sd, double *out) {
   return 0;
 }
 
+// 1-D energy (co)spectra of a state S(ny,nxh,nz,4) (write_energy_spectra.f90:9-300 without
+// the HDF5 output).
+//   du/dy, dw/dy: labs_bs(D0_LU) then labs_mat_mul(D1) on every line, kx = kz = 0 included
+//     (get_y_dir_deriv_of_field, get_statistics.f90:658-681)
+//   vorticity (get_vorticity_field, get_statistics.f90:912-948), with (0,k) = dcmplx(0d0,k):
+//     ox = dw/dy - (0,kz) v,  oy = (0,kz) u - (0,kx) w,  oz = (0,kx) v - du/dy
+//   E_ab_x(y, kx) = sum over the kz index in storage order of w (Re a Re b + Im a Im b),
+//     w = 1 at kx = 0, else 2 (get_E_uu_x, write_energy_spectra.f90:326-359);
+//   E_ab_z(y, kz) = the same sum over kx in ascending order (get_E_uu_z, 366-399).
+// Fields 0 u, 1 v, 2 w, 3 theta, 4 ox, 5 oy, 6 oz; pairs in the reference's dataset order
+// (write_energy_spectra.f90:86-239).  Its kz list computes "Eozoz_kz" from (ox, oz)
+// (write_energy_spectra.f90:218-219), and so does this list.
+//   ex: 18 x nxh x ny, ez: 18 x nz x ny (y fastest); either may be null
+static const int kSpecX[18][2] = {{0, 0}, {1, 1}, {2, 2}, {3, 3}, {0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3},
+                                  {3, 2}, {4, 4}, {5, 5}, {6, 6}, {4, 5}, {4, 6}, {5, 6}, {1, 6}, {2, 5}};
+static const int kSpecZ[18][2] = {{0, 0}, {1, 1}, {2, 2}, {3, 3}, {0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3},
+                                  {2, 3}, {4, 4}, {5, 5}, {4, 6}, {4, 5}, {4, 6}, {5, 6}, {1, 6}, {2, 5}};
+int orc_spectra(void *h, const double *Sd, double *ex, double *ez) {
+  Ctx *c = static_cast<Ctx *>(h);
+  const int n = c->ny, bw = c->bw, nxh = c->nxh, nz = c->nz;
+  const size_t line = (size_t)n, nl = (size_t)nxh * nz, fsz = nl * line;
+  const cd *S = reinterpret_cast<const cd *>(Sd);
+  std::vector<cd> F((size_t)7 * fsz);
+  for (int f = 0; f < 4; ++f) std::copy(S + (size_t)f * fsz, S + (size_t)(f + 1) * fsz, F.begin() + (size_t)f * fsz);
+  std::vector<cd> buf(n), du(n), dw(n);
+  for (int j = 0; j < nz; ++j)
+    for (int k = 0; k < nxh; ++k) {
+      const size_t o = ((size_t)j * nxh + k) * line;
+      const cd *u = &F[o], *v = &F[fsz + o], *w = &F[2 * fsz + o];
+      std::copy(u, u + n, buf.begin());
+      labs_bs(c->d0lu.data(), n, bw, buf.data());
+      labs_mv(c->d1.data(), n, bw, buf.data(), du.data());
+      std::copy(w, w + n, buf.begin());
+      labs_bs(c->d0lu.data(), n, bw, buf.data());
+      labs_mv(c->d1.data(), n, bw, buf.data(), dw.data());
+      const cd ikz = C(0.0, c->kz[j] * c->lmfzr), ikx = C(0.0, c->kx[k] * c->lmfxr);
+      cd *ox = &F[4 * fsz + o], *oy = &F[5 * fsz + o], *oz = &F[6 * fsz + o];
+      for (int y = 0; y < n; ++y) {
+        ox[y] = sub(dw[y], mul(ikz, v[y]));
+        oy[y] = sub(mul(ikz, u[y]), mul(ikx, w[y]));
+        oz[y] = sub(mul(ikx, v[y]), du[y]);
+      }
+    }
+  auto term = [](cd a, cd b) { return a.re * b.re + a.im * b.im; };
+  for (int q = 0; q < 18; ++q) {
+    if (ex) {
+      const cd *A = &F[(size_t)kSpecX[q][0] * fsz], *B = &F[(size_t)kSpecX[q][1] * fsz];
+      double *o = ex + (size_t)q * nxh * n;
+      for (size_t i = 0; i < (size_t)nxh * n; ++i) o[i] = 0.0;
+      for (size_t l = 0; l < nl; ++l) {
+        const int k = (int)(l % nxh);
+        const cd *a = A + l * line, *b = B + l * line;
+        double *st = o + (size_t)k * n;
+        for (int y = 0; y < n; ++y)
+          st[y] = (k == 0) ? st[y] + term(a[y], b[y]) : st[y] + 2.0 * term(a[y], b[y]);
+      }
+    }
+    if (ez) {
+      const cd *A = &F[(size_t)kSpecZ[q][0] * fsz], *B = &F[(size_t)kSpecZ[q][1] * fsz];
+      double *o = ez + (size_t)q * nz * n;
+      for (size_t i = 0; i < (size_t)nz * n; ++i) o[i] = 0.0;
+      for (size_t l = 0; l < nl; ++l) {
+        const int k = (int)(l % nxh), j = (int)(l / nxh);
+        const cd *a = A + l * line, *b = B + l * line;
+        double *st = o + (size_t)j * n;
+        for (int y = 0; y < n; ++y)
+          st[y] = (k == 0) ? st[y] + term(a[y], b[y]) : st[y] + 2.0 * term(a[y], b[y]);
+      }
+    }
+  }
+  return 0;
+}
+
 // Statistics-time pressure (get_statistics_1 -> get_uv(v, v) and the PPE
 // wall values, then get_pressure.f90 after the products).
 //   Sd: state S(ny,nxh,nz,4) at the start of the substep (v = slot 1)

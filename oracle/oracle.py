@@ -45,6 +45,7 @@ def lib():
         L.orc_substep.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp]
         L.orc_pressure.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.orc_statistics.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_spectra.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.orc_labs_setup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_double, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
         L.orc_bspline_row.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -190,6 +191,20 @@ class Oracle:
         d = {k: out[i * n:(i + 1) * n] for i, k in enumerate(self.STAT_NAMES)}
         d["u_center"], d["t_center"] = out[16 * n], out[16 * n + 1]
         return d
+
+    # write_energy_spectra.f90's datasets, in its order (fields: u v w t, vorticity ox oy oz)
+    SPECTRA_NAMES = ("uu", "vv", "ww", "tt", "uv", "vw", "uw", "ut", "vt", "wt", "oxox", "oyoy", "ozoz",
+                     "oxoy", "oxoz", "oyoz", "voz", "woy")
+
+    def spectra(self, S: np.ndarray):
+        """1-D energy (co)spectra of the state S (4 slots): (ex, ez) with
+        ex[q, kx, y] summed over kz and ez[q, kz, y] summed over kx, q in
+        SPECTRA_NAMES order ("ozoz" of ez is the reference's (ox, oz))."""
+        c = self.cfg
+        ex = np.zeros((18, c.nxh, c.ny))
+        ez = np.zeros((18, c.nz, c.ny))
+        lib().orc_spectra(self.h, np.ascontiguousarray(S).ctypes.data_as(_dp), _p(ex), _p(ez))
+        return ex, ez
 
     def substep(self, S: np.ndarray, RK: np.ndarray, s: int, dt: float):
         out = np.zeros(6)

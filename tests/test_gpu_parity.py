@@ -314,3 +314,24 @@ def test_statistics(lib, oracle_lib, name):
         assert np.array_equal(st[k], ref[k]), (k, np.abs(st[k] - ref[k]).max())
     assert abs(st["u_center"] - ref["u_center"]) <= 1e-15 * abs(ref["u_center"])
     assert abs(st["t_center"] - ref["t_center"]) <= 1e-15 * max(abs(ref["t_center"]), 1.0)
+
+
+@pytest.mark.parametrize("name", ["re180_64", "re1000_thin", "tiny_sin"])
+def test_spectra(lib, oracle_lib, name):
+    """1-D energy (co)spectra of the device state (write_energy_spectra):
+    derivatives, vorticity and the serial sums bitwise against the oracle;
+    the state itself is left as it was."""
+    cfg = CFGS[name]
+    orc = oracle_lib.Oracle(cfg)
+    S, RK = developed(cfg, orc)
+    ex_ref, ez_ref = orc.spectra(S)
+    ch = lib.Channel(cfg)
+    ch.set_state(S, RK)
+    ex, ez = ch.spectra()
+    S2, RK2 = ch.get_state()
+    ch.close()
+    assert lib.Channel.SPECTRA_NAMES == orc.SPECTRA_NAMES
+    for q, k in enumerate(orc.SPECTRA_NAMES):
+        assert np.array_equal(ex[q], ex_ref[q]), (k, "x", np.abs(ex[q] - ex_ref[q]).max())
+        assert np.array_equal(ez[q], ez_ref[q]), (k, "z", np.abs(ez[q] - ez_ref[q]).max())
+    assert np.array_equal(S2, S) and np.array_equal(RK2, RK)
