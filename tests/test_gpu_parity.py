@@ -262,6 +262,17 @@ def test_full_size_invariants(lib, oracle_lib):
     print("[full] step", flush=True)
     out = ch.step(cfg.dt)
     assert all(np.isfinite(list(o.values())).all() for o in out)
+    # spectra at full size: each velocity/scalar spectrum sums over its
+    # wavenumber to the Parseval profile of pb_statistics (same terms)
+    print("[full] statistics and spectra", flush=True)
+    st = ch.statistics()
+    ex, ez = ch.spectra()
+    scale = max(np.abs(st[k]).max() for k in ("uu", "vv", "ww", "tt"))
+    for q, name in enumerate(("uu", "vv", "ww", "tt", "uv", "vw", "uw", "ut", "vt", "wt")):
+        assert np.allclose(ex[q].sum(axis=0), st[name], rtol=1e-11, atol=1e-13 * scale), name
+        assert np.allclose(ez[q].sum(axis=0), st[name], rtol=1e-11, atol=1e-13 * scale), name
+    assert np.isfinite(ex).all() and np.isfinite(ez).all() and np.array_equal(ez[12], ez[14])
+    del ex, ez
     print("[full] get_state", out[-1], flush=True)
     Sg, RKg = ch.get_state()
     ch.close()
