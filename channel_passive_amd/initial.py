@@ -136,3 +136,27 @@ def make_state(cfg: ChannelConfig, seed: int = 0xC0FFEE, amp: float = 0.1, kmax:
             if ix == 0:
                 S[slot, (cfg.nz - iz) % cfg.nz, 0, :] = np.conj(val)
     return S
+
+
+def reference_state(cfg: ChannelConfig, y: np.ndarray) -> np.ndarray:
+    """The reference's cold-start field (set_initial_value_in_y_pencil,
+    initial_step.f90:63-121) on the collocation points y: the laminar
+    profile 1.5 (1 - y^2) - u_gal on the mean mode, a v mode at kx index 1
+    and a (u, v, w) mode pair at kz = +-1 on kx = 0 (conjugates of each
+    other); theta is a copy of u on every mode.  State S(4, nz, nx/2, ny)."""
+    S = np.zeros((4, cfg.nz, cfg.nxh, cfg.ny), dtype=np.complex128)
+    y = np.asarray(y, dtype=np.float64)
+    a = 1.0 - y ** 2
+    b = ((1.0 - y) * (1.0 + y)) ** 2
+    S[0, 0, 0] = a * 1.5 - cfg.u_gal
+    if cfg.nxh > 1:
+        S[1, 0, 1] = b * 2e-2 + 1j * (b * 1e-2)
+    if cfg.nz > 1:
+        S[0, 1, 0] = a * 2e-2 + 1j * (a * 1e-2)
+        S[1, 1, 0] = -a * 1e-2 + 1j * (a * 1e-2)
+        S[2, 1, 0] = -a * 1e-2 + 1j * (-a * 2e-2)
+        S[0, cfg.nz - 1, 0] = a * 2e-2 + 1j * (-a * 1e-2)
+        S[1, cfg.nz - 1, 0] = -a * 1e-2 + 1j * (-a * 1e-2)
+        S[2, cfg.nz - 1, 0] = -a * 1e-2 + 1j * (a * 2e-2)
+    S[3] = S[0]
+    return S

@@ -47,3 +47,28 @@ def test_divergence_free_and_symmetric(oracle_lib):
     for j in range(1, nh):
         assert np.array_equal(S[:, j, 0], np.conj(S[:, cfg.nz - j, 0]))
     assert not np.any(S[:, nh])
+
+
+def test_reference_cold_start(oracle_lib):
+    """initial_step.f90:63-121: the kz = +-1 modes on kx = 0 are conjugate,
+    theta copies u, the mean mode carries the flow rate 1 - u_gal
+    (0.5 int (1.5 (1 - y^2) - u_gal) dy), v vanishes with its slope at the
+    walls on the kx = 1 mode, and nothing else is set."""
+    import dataclasses
+    from channel_passive_amd.initial import reference_state
+    from oracle import oracle as O
+    cfg = dataclasses.replace(preset("re180_64"), u_gal=0.2)
+    orc = oracle_lib.Oracle(cfg)
+    S = reference_state(cfg, orc.y)
+    assert np.array_equal(S[:, cfg.nz - 1, 0], np.conj(S[:, 1, 0]))
+    assert np.array_equal(S[3], S[0])
+    c = O.band_solve(orc.d0lu, orc.bw, S[0, 0, 0].copy())
+    assert abs(0.5 * np.dot(orc.integ, c.real) - (1.0 - cfg.u_gal)) <= 1e-13
+    v1 = S[1, 0, 1]
+    dv1 = O.band_matvec(orc.d1, orc.bw, O.band_solve(orc.d0lu, orc.bw, v1.copy()))
+    assert abs(v1[0]) < 1e-30 and abs(v1[-1]) < 1e-30
+    assert abs(dv1[0]) <= 1e-12 * np.abs(dv1).max() and abs(dv1[-1]) <= 1e-12 * np.abs(dv1).max()
+    mask = np.ones((cfg.nz, cfg.nxh), bool)
+    for j, k in ((0, 0), (0, 1), (1, 0), (cfg.nz - 1, 0)):
+        mask[j, k] = False
+    assert not np.any(S[:, mask])
