@@ -33,7 +33,13 @@ def configs():
     thin385 = dataclasses.replace(preset("re1000"), nx=16, nz=12, name="re1000_thin")  # ny = 385
     tiny_sin = dataclasses.replace(preset("tiny"), knot_map=0, stretch=0.9, labs_k=6, name="tiny_sin")
     flat = dataclasses.replace(preset("re1000"), ny=33, name="re1000_flat")  # production x/z transform sizes
-    return {"re180_64": small, "re180_192": opp, "re1000_thin": thin385, "tiny_sin": tiny_sin, "re1000_flat": flat}
+    # band widths 3 and 5 (the others are 4): uniform knots, order 7, ny 34, opposition control;
+    # order 10 on 65 points
+    bw3 = dataclasses.replace(preset("tiny"), knot_map=0, stretch=0.0, labs_k=6, ny=34, opp_alpha=0.5,
+                              opp_yplus=20.0, name="tiny_bw3")
+    bw5 = dataclasses.replace(preset("tiny"), ny=65, labs_k=9, name="tiny_bw5")
+    return {"re180_64": small, "re180_192": opp, "re1000_thin": thin385, "tiny_sin": tiny_sin, "re1000_flat": flat,
+            "tiny_bw3": bw3, "tiny_bw5": bw5}
 
 
 CFGS = configs()
@@ -53,7 +59,7 @@ def developed(cfg, orc, nsteps=1, seed=11):
     return S, RK
 
 
-@pytest.mark.parametrize("name", ["re180_64", "re1000_thin", "tiny_sin"])
+@pytest.mark.parametrize("name", ["re180_64", "re1000_thin", "tiny_sin", "tiny_bw3", "tiny_bw5"])
 def test_labs_bitwise(lib, oracle_lib, name):
     cfg = CFGS[name]
     orc = oracle_lib.Oracle(cfg)
@@ -183,7 +189,7 @@ def test_pressure_stats(lib, oracle_lib, name):
     assert np.array_equal(S1, S2) and np.array_equal(RK1, RK2) and out == out2
 
 
-@pytest.mark.parametrize("name", ["re180_64", "re180_192", "re1000_thin", "tiny_sin"])
+@pytest.mark.parametrize("name", ["re180_64", "re180_192", "re1000_thin", "tiny_sin", "tiny_bw3", "tiny_bw5"])
 @pytest.mark.parametrize("s", [1, 2, 3])
 def test_wavespace_bitwise(lib, oracle_lib, name, s):
     cfg = CFGS[name]
@@ -206,7 +212,7 @@ def test_wavespace_bitwise(lib, oracle_lib, name, s):
     ch.close()
 
 
-@pytest.mark.parametrize("name", ["re180_64", "re180_192", "re1000_thin"])
+@pytest.mark.parametrize("name", ["re180_64", "re180_192", "re1000_thin", "tiny_bw3", "tiny_bw5"])
 def test_step_parity(lib, oracle_lib, name):
     cfg = CFGS[name]
     orc = oracle_lib.Oracle(cfg)
@@ -352,7 +358,7 @@ def test_statistics(lib, oracle_lib, name):
     assert abs(st["t_center"] - ref["t_center"]) <= 1e-15 * max(abs(ref["t_center"]), 1.0)
 
 
-@pytest.mark.parametrize("name", ["re180_64", "re1000_thin", "tiny_sin"])
+@pytest.mark.parametrize("name", ["re180_64", "re1000_thin", "tiny_sin", "tiny_bw3", "tiny_bw5"])
 def test_spectra(lib, oracle_lib, name):
     """1-D energy (co)spectra of the device state (write_energy_spectra):
     derivatives, vorticity and the serial sums bitwise against the oracle;
