@@ -266,7 +266,7 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
     c->zs_ok = c->zs_on && c->ct_z_ok && c->nxh % pb::zs::B == 0 && lzs <= lds_max &&
                hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c->ncu > 0;
     using FX = pb::FftX1536;
-    c->ct_x_ok = c->ct_fft && c->nx32 == FX::N && FX::N * FX::B * (int)sizeof(double2) <= lds_max;
+    c->ct_x_ok = c->ct_fft && c->nx32 == FX::N && FX::L::SIZE * (int)sizeof(double2) <= lds_max;
   }
   c->plane = (size_t)c->nz * c->nxh;
   c->field = c->plane * c->ny;
@@ -540,7 +540,7 @@ static int launch_zinv_t(pb_ctx *c) {
 }
 template <class F>
 static int launch_xphys_t(pb_ctx *c, int want_vv) {
-  const size_t ldsx = (size_t)c->nx32 * 2 * sizeof(double2);
+  const size_t ldsx = F::lds_elems(c->nx32, 2) * sizeof(double2);
   StageTimer tm(c, PB_STAGE_XPHYS);
   hipLaunchKernelGGL(pb::xphys_kernel<F>, dim3(c->nz32, c->ny), dim3(F::T), ldsx, c->stream, c->pad, c->px, c->twx,
                      c->nxh, c->nz32, c->pfield, c->ddyr, c->dxr, c->dzr, c->vpart, want_vv ? 3 : 2);

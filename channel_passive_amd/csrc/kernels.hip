@@ -302,11 +302,21 @@ __device__ inline Blk xcd_block() {
 __device__ __forceinline__ int lds_swz(int i) { return PB_LDS_SWZ ? i ^ ((i >> 4) & 7) : i; }
 
 // Layout of B lines of N elements in LDS: interleaved (element n of line b
-// at n*B + b, the z transforms, whose global rows hold B consecutive kx) or
-// separate (b*N + n, the pair of x lines).
+// at n*B + b, the z transforms, whose global rows hold B consecutive kx;
+// XOR-swizzled, a permutation, as the streamed copies into LDS need) or
+// separate (b*N + n, the pair of x lines; one pad element after every 16,
+// which breaks the power-of-two strides as well and keeps the index of
+// element n0 + r*S, S a multiple of 16, linear in r: the passes address
+// their R elements with constant offsets from one base).
 template <int N, int B, bool SEP>
 struct Lines {
-  __device__ static int at(int n, int b) { return lds_swz(SEP ? b * N + n : n * B + b); }
+  static constexpr bool PAD = SEP;
+  static constexpr int SIZE = PAD ? N * B + (N * B) / 16 : N * B;
+  static_assert(!PAD || N % 16 == 0, "padded lines start on a 16-element boundary");
+  __device__ static int at(int n, int b) {
+    const int i = SEP ? b * N + n : n * B + b;
+    return PAD ? i + (i >> 4) : lds_swz(i);
+  }
   // butterfly t of a pass with N/R butterflies per line -> (line, index)
   __device__ static void split(int t, int NR, int &b, int &j) {
     if (SEP) { b = t / NR; j = t % NR; } else { b = t % B; j = t / B; }
@@ -330,6 +340,14 @@ __device__ __forceinline__ void pass_sync() {
   }
 }
 
+// LDS index of element n0 + r*S of line b (ALIGNED: n0 % 16 == 0 and r*S < 16)
+template <class L, int S, bool ALIGNED>
+__device__ __forceinline__ int lds_at(int n0, int b, int r) {
+  if constexpr (L::PAD && S % 16 == 0) return L::at(n0, b) + r * (S + S / 16);
+  else if constexpr (L::PAD && ALIGNED) return L::at(n0, b) + r * S;
+  else return L::at(n0 + r * S, b);
+}
+
 template <int T, class L, int N, int B, int SIGN, int Ns, int R, bool LDS_ONLY = false>
 __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ tw) {
   constexpr int NR = N / R, NBF = NR * B, ROUNDS = (NBF + T - 1) / T;
@@ -342,7 +360,7 @@ __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ 
       int b, j;
       L::split(t, NR, b, j);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = x[L::at(j + r * NR, b)];
+      for (int r = 0; r < R; ++r) v[q][r] = x[lds_at<L, NR, false>(j, b, r)];
     }
   }
   pass_sync<LDS_ONLY>();
@@ -375,7 +393,8 @@ __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ 
       dft<R>(v[q], SIGN);
       const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[L::at(base + r * Ns, b)] = v[q][r];
+      // first pass of a radix-16 plan: base = 16 j
+      for (int r = 0; r < R; ++r) x[lds_at<L, Ns, Ns == 1 && R == 16>(base, b, r)] = v[q][r];
     }
   }
   pass_sync<LDS_ONLY>();
@@ -398,6 +417,7 @@ __device__ __forceinline__ void ct_passes_lds(double2 *x, const double2 *__restr
 template <int EPT>
 struct RtFft {
   static constexpr int T = FFT_THREADS, N = 0, B = 0, ELEMS = EPT, MINW = 1;
+  static size_t lds_elems(int n, int b) { return (size_t)n * b; }
   __device__ static int at(int n, int b, int B_) { return n * B_ + b; }
   __device__ static void run(double2 *x, int B_, const FftPlan &P, const double2 *__restrict__ tw, int sign) {
     lds_fft<EPT>(x, B_, P, tw, sign);
@@ -410,6 +430,7 @@ struct CtFft {
   // register space wasted)
   static constexpr int T = T_, N = N_, B = B_, ELEMS = (N_ * B_ + T_ - 1) / T_, MINW = MINW_;
   using L = Lines<N_, B_, SEP>;
+  static size_t lds_elems(int, int) { return L::SIZE; }
   static_assert((R * ...) == N_, "radices must multiply to N");
   static_assert(T_ % 32 == 0 && T_ <= 1024, "block size");
   __device__ static int at(int n, int b, int) { return L::at(n, b); }
