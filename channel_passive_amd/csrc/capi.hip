@@ -623,17 +623,16 @@ static int launch_sweeps(pb_ctx *c, const pb::LineParams &P, const pb::WsArgs &A
                        nt, c->scal);                                                                        \
     HIPCHK(hipGetLastError());                                                                              \
   }
-  if (s == 1) {
-    SWEEP(pb::SW_A1, c->stream);
-    SWEEP(pb::SW_B1, c->stream);
-  }
+  const bool s1 = s == 1;
+  if (s1) SWEEP(pb::SW_A1, c->stream);
   SWEEP(pb::SW_A, c->stream);
-  SWEEP(pb::SW_BG, c->stream);  // reads u1a in slot 6 before D_t writes theta there
+  // reads u1a in slot 6 before D_t writes theta there
+  if (s1) SWEEP(pb::SW_BG1, c->stream) else SWEEP(pb::SW_BG, c->stream);
   // theta chain (slots 5, 6, 7, 8, RK3, dbuf2) beside B_v, C_nu, D_nu (slots
   // 0-4, RK1, RK2, dbuf); joined before E and F overwrite slots 5, 7, 8
   HIPCHK(hipEventRecord(c->ev_fork2, c->stream));
   HIPCHK(hipStreamWaitEvent(c->side2, c->ev_fork2, 0));
-  SWEEP(pb::SW_BT, c->side2);
+  if (s1) SWEEP(pb::SW_BT1, c->side2) else SWEEP(pb::SW_BT, c->side2);
   SWEEP(pb::SW_CT, c->side2);
   SWEEP(pb::SW_DT, c->side2);
   HIPCHK(hipEventRecord(c->ev_join2, c->side2));

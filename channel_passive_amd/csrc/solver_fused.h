@@ -36,16 +36,18 @@
 //   after G : 0 u, 1 v, 2 w, 6 theta; RK2 = store_phi
 // R0..R4 are real columns (the reference's (1+i)-scaled homogeneous
 // solutions are (x, x): only x is stored); R0..R3 live in the memory of the
-// two scratch columns, which hold D2 LU(D0)^-1 of the stores on s == 1
-// until B, R4 in rbuf.  The U factors and R0..R4 depend on k^2 only and are
+// two scratch columns, which hold A1's forward substitutions of the stores
+// on s == 1 until B, R4 in rbuf.  The U factors and R0..R4 depend on k^2 only and are
 // kept once per (kx, +-kz) pair, at the primary system's index; C and E run
 // one thread per pair, the other sweeps one thread per system.
 // B, C and D are split by right-hand-side group (theta | g | phi,v): the
 // groups read disjoint data, so the split costs no traffic and keeps the
 // register windows of each kernel small; the theta chain can run beside
 // the others.
-// s == 1 adds a down/up pair (A1/B1) that forms D2 LU(D0)^-1 of the three
-// stores in scratch columns buf0, buf1 and slot 5.
+// s == 1 adds a down sweep (A1: forward substitution of the three stores
+// with LU(D0) into buf0, buf1 and slot 5); the B sweeps of s == 1 finish
+// D2 LU(D0)^-1 of the stores in their up sweeps (B_g1 for g and phi,
+// B_t1 for theta).
 #pragma once
 
 #include "solver.h"
@@ -436,28 +438,6 @@ PB_HD void sweep_A1(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   });
 }
 
-template <int BW>
-PB_HD void sweep_B1(const LineParams &P, const WsArgs &A, const SysCtx &c) {
-  const BandGeom g{c.n, BW};
-  SV o1 = c.buf(A, 0), o2 = c.buf(A, 1), o3 = c.col(A, 5);
-  CWin<2 * BW + 1> w1, w2, w3;
-  w1.clear(); w2.clear(); w3.clear();
-  up_rows<BW>(c.n, [&](int r, auto kind) __attribute__((always_inline)) {
-    constexpr bool MID = decltype(kind)::mid;
-    double ur[2 * BW];
-    load_urow<BW, MID>(g, P.D0LU, r, ur);
-    const cplx x1 = bwd_row<BW, MID>(g, r, ur, w1, o1[r]);
-    const cplx x2 = bwd_row<BW, MID>(g, r, ur, w2, o2[r]);
-    const cplx x3 = bwd_row<BW, MID>(g, r, ur, w3, o3[r]);
-    w1.push(x1); w2.push(x2); w3.push(x3);
-    for_completed_rows<BW>(c.n, r, [&](int m) {
-      o1[m] = mv_row<BW>(g, P.D2, m, r, w1);
-      o2[m] = mv_row<BW>(g, P.D2, m, r, w2);
-      o3[m] = mv_row<BW>(g, P.D2, m, r, w3);
-    });
-  });
-}
-
 // A: hvhg point-wise terms (hvhg.f90:169-204, 256-258) and forward
 // substitution of u3a, u3b, u4b, P8 with LU(D0)
 template <int BW>
@@ -509,11 +489,12 @@ PB_HD RhsCoef rhs_coef(const LineParams &P, double k2, bool scalar) {
   r.zeta = (s != P.S) ? P.dt * P.coef(3, s + 1) : 0.0;
   return r;
 }
-// right-hand side of one row and the zeta*h store
-PB_HD cplx rhs_row(const LineParams &P, const RhsCoef &q, int m, int n, SV &store, const SV &d2s, cplx h) {
+// right-hand side of one row and the zeta*h store; d2s = (D2 LU(D0)^-1 store)(m)
+// on substep 1
+PB_HD cplx rhs_row(const LineParams &P, const RhsCoef &q, int m, int n, SV &store, cplx d2s, cplx h) {
   cplx rhs;
   if (P.s == 1)
-    rhs = cadd(cadd(cscale(store[m], q.du), cscale(h, q.gamma)), cscale(d2s[m], q.al));
+    rhs = cadd(cadd(cscale(store[m], q.du), cscale(h, q.gamma)), cscale(d2s, q.al));
   else
     rhs = cadd(store[m], cscale(h, q.gamma));
   if (m == 0 || m == n - 1) rhs = mk(0.0, 0.0);
@@ -521,42 +502,54 @@ PB_HD cplx rhs_row(const LineParams &P, const RhsCoef &q, int m, int n, SV &stor
   return rhs;
 }
 
-template <int BW>
+// S1 (substep 1): the same up sweep also back-substitutes A1's forward
+// substitutions of the g and phi stores (y1, y2) and forms D2 LU(D0)^-1 of
+// them row by row: the g part is consumed at once, the phi part replaces y2
+// in place for B_phi (a row m is emitted only after y2[m] has been read).
+template <int BW, bool S1>
 PB_HD void sweep_Bg(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const RhsCoef q = rhs_coef(P, c.k2, false);
-  SV s2 = c.col(A, 2), s6 = c.col(A, 6), r1 = c.rk(A, 0), d1 = c.buf(A, 0);
-  CWin<2 * BW + 1> cE;
-  cE.clear();
+  SV s2 = c.col(A, 2), s6 = c.col(A, 6), r1 = c.rk(A, 0), y1 = c.buf(A, 0), y2 = c.buf(A, 1);
+  CWin<2 * BW + 1> cE, c1, c2;
+  cE.clear(); c1.clear(); c2.clear();
   up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
     constexpr bool MID = decltype(kind)::mid;
     double ur[2 * BW];
     load_urow<BW, MID>(g, P.D0LU, r, ur);
     cE.push(bwd_row<BW, MID>(g, r, ur, cE, s2[r]));
+    if (S1) {
+      c1.push(bwd_row<BW, MID>(g, r, ur, c1, y1[r]));
+      c2.push(bwd_row<BW, MID>(g, r, ur, c2, y2[r]));
+    }
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx hg = cadd(s6[m], mv_row<BW>(g, P.D1, m, r, cE));
-      s2[m] = rhs_row(P, q, m, n, r1, d1, hg);
+      const cplx d2s = S1 ? mv_row<BW>(g, P.D2, m, r, c1) : mk(0.0, 0.0);
+      if (S1) y2[m] = mv_row<BW>(g, P.D2, m, r, c2);
+      s2[m] = rhs_row(P, q, m, n, r1, d2s, hg);
     });
   });
 }
 
-template <int BW>
+template <int BW, bool S1>
 PB_HD void sweep_Bt(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const RhsCoef q = rhs_coef(P, c.k2, true);
-  SV s7 = c.col(A, 7), s8 = c.col(A, 8), r3 = c.rk(A, 2), d3 = c.col(A, 5);
-  CWin<2 * BW + 1> c8;
-  c8.clear();
+  SV s7 = c.col(A, 7), s8 = c.col(A, 8), r3 = c.rk(A, 2), y3 = c.col(A, 5);
+  CWin<2 * BW + 1> c8, c3;
+  c8.clear(); c3.clear();
   up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
     constexpr bool MID = decltype(kind)::mid;
     double ur[2 * BW];
     load_urow<BW, MID>(g, P.D0LU, r, ur);
     c8.push(bwd_row<BW, MID>(g, r, ur, c8, s7[r]));
+    if (S1) c3.push(bwd_row<BW, MID>(g, r, ur, c3, y3[r]));
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx ht = cneg(cadd(s8[m], mv_row<BW>(g, P.D1, m, r, c8)));
-      s7[m] = rhs_row(P, q, m, n, r3, d3, ht);
+      const cplx d2s = S1 ? mv_row<BW>(g, P.D2, m, r, c3) : mk(0.0, 0.0);
+      s7[m] = rhs_row(P, q, m, n, r3, d2s, ht);
     });
   });
 }
@@ -582,7 +575,8 @@ PB_HD void sweep_Bv(const LineParams &P, const WsArgs &A, const SysCtx &c) {
     for_completed_rows<BW>(n, r, [&](int m) {
       const cplx u5b = cadd(cadd(mv_row<BW>(g, P.D1, m, r, cB), cmul(a, s1[m])), cmul(b, s3[m]));
       const cplx hv = cadd(u5b, mv_row<BW>(g, P.D2, m, r, cF));
-      s0[m] = rhs_row(P, q, m, n, r2, d2, hv);
+      // s == 1: D2 LU(D0)^-1 of the phi store, formed by B_g1
+      s0[m] = rhs_row(P, q, m, n, r2, P.s == 1 ? d2[m] : mk(0.0, 0.0), hv);
     });
   });
 }
@@ -958,10 +952,14 @@ PB_HD void opp_line(const LineParams &P, const SV &v, const SV &y, int first_lo,
 
 // One sweep: index is a system (kz-major, kx-minor) or, for the
 // factorising sweeps C_nu, C_theta and E, a pair (kz index 0 .. nz/2).
-// A substep runs, for all systems, A1 and B1 (s == 1 only), A, B_g, then
-// the theta chain B_t, C_t, D_t and, in any order relative to it, B_phi,
-// C_nu, D_nu; then E, F, G.
-enum SweepId { SW_A1, SW_B1, SW_A, SW_BT, SW_CT, SW_DT, SW_BG, SW_BV, SW_CNU, SW_DNU, SW_E, SW_F, SW_G, SW_COUNT };
+// A substep runs, for all systems, A1 (s == 1 only), A, B_g, then the theta
+// chain B_t, C_t, D_t and, in any order relative to it, B_phi, C_nu, D_nu;
+// then E, F, G (on s == 1 B_g1 and B_t1 stand for B_g and B_t).
+enum SweepId {
+  SW_A1, SW_A, SW_BT, SW_CT, SW_DT, SW_BG, SW_BV, SW_CNU, SW_DNU, SW_E, SW_F, SW_G,
+  SW_BT1, SW_BG1,  // substep 1: B with D2 LU(D0)^-1 of the stores
+  SW_COUNT
+};
 
 constexpr bool sweep_paired(int sw) { return sw == SW_CNU || sw == SW_CT || sw == SW_E; }
 // threads of a sweep
@@ -982,11 +980,12 @@ PB_HD void run_sweep(const LineParams &P, const WsArgs &A, double *dbuf2, size_t
   SysCtx c;
   if (!make_sys(A, P.g.n, idx, c)) return;
   if (SW == SW_A1) sweep_A1<BW>(P, A, c);
-  if (SW == SW_B1) sweep_B1<BW>(P, A, c);
   if (SW == SW_A) sweep_A<BW>(P, A, c);
-  if (SW == SW_BT) sweep_Bt<BW>(P, A, c);
+  if (SW == SW_BT) sweep_Bt<BW, false>(P, A, c);
+  if (SW == SW_BT1) sweep_Bt<BW, true>(P, A, c);
   if (SW == SW_DT) sweep_Dt<BW>(P, A, c, dbuf2);
-  if (SW == SW_BG) sweep_Bg<BW>(P, A, c);
+  if (SW == SW_BG) sweep_Bg<BW, false>(P, A, c);
+  if (SW == SW_BG1) sweep_Bg<BW, true>(P, A, c);
   if (SW == SW_BV) sweep_Bv<BW>(P, A, c);
   if (SW == SW_DNU) sweep_Dnu<BW>(P, A, c);
   if (SW == SW_F) sweep_F<BW>(P, A, c);
@@ -996,8 +995,10 @@ PB_HD void run_sweep(const LineParams &P, const WsArgs &A, double *dbuf2, size_t
 // the sweeps of substep s in a serial order valid for the stream schedule
 inline int sweep_sequence(int s, int *ids) {
   int k = 0;
-  if (s == 1) { ids[k++] = SW_A1; ids[k++] = SW_B1; }
-  const int rest[] = {SW_A, SW_BG, SW_BT, SW_CT, SW_DT, SW_BV, SW_CNU, SW_DNU, SW_E, SW_F, SW_G};
+  const bool s1 = s == 1;
+  if (s1) ids[k++] = SW_A1;
+  const int rest[] = {SW_A, s1 ? SW_BG1 : SW_BG, s1 ? SW_BT1 : SW_BT, SW_CT, SW_DT, SW_BV,
+                      SW_CNU, SW_DNU, SW_E, SW_F, SW_G};
   for (int q : rest) ids[k++] = q;
   return k;
 }

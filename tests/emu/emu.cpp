@@ -86,7 +86,8 @@ template <int BW>
 static void run_id(int id, const pb::LineParams &P, const pb::WsArgs &A, double *dbuf2, size_t sys) {
   switch (id) {
     case pb::SW_A1: pb::run_sweep<BW, pb::SW_A1>(P, A, dbuf2, sys); break;
-    case pb::SW_B1: pb::run_sweep<BW, pb::SW_B1>(P, A, dbuf2, sys); break;
+    case pb::SW_BG1: pb::run_sweep<BW, pb::SW_BG1>(P, A, dbuf2, sys); break;
+    case pb::SW_BT1: pb::run_sweep<BW, pb::SW_BT1>(P, A, dbuf2, sys); break;
     case pb::SW_A: pb::run_sweep<BW, pb::SW_A>(P, A, dbuf2, sys); break;
     case pb::SW_BT: pb::run_sweep<BW, pb::SW_BT>(P, A, dbuf2, sys); break;
     case pb::SW_CT: pb::run_sweep<BW, pb::SW_CT>(P, A, dbuf2, sys); break;
