@@ -240,6 +240,41 @@ int basis_row(const Collocation &c, double y, int &first, std::vector<double> &v
   return 0;
 }
 
+int detection_row(const Collocation &c, const std::vector<double> &d0lu, int bw, double y, std::vector<double> &r) {
+  int first = 0;
+  std::vector<double> vals;
+  if (basis_row(c, y, first, vals)) return -1;
+  const int n = c.n, W = 2 * bw + 1;
+  auto fst = [&](int row) { return row < bw ? 0 : (row >= n - bw ? n - 1 - 2 * bw : row - bw); };
+  auto lst = [&](int row) { return row < bw ? 2 * bw : (row >= n - bw ? n - 1 : row + bw); };
+  auto at = [&](int row, int col) {
+    int slot;
+    if (row < bw && col > row + bw) slot = 2 * bw - col;
+    else if (row >= n - bw && col < row - bw) slot = n - 1 - col;
+    else slot = bw + col - row;
+    return d0lu[(size_t)row * W + slot];
+  };
+  std::vector<double> b(n, 0.0);
+  for (size_t j = 0; j < vals.size(); ++j) b[first + j] = vals[j];
+  // U^T w = b (unit lower), rows of U that reach column col, ascending
+  std::vector<double> w(n);
+  for (int col = 0; col < n; ++col) {
+    double v = b[col];
+    for (int row = std::max(0, col - 2 * bw); row < col; ++row)
+      if (col <= lst(row)) v = v - at(row, col) * w[row];
+    w[col] = v;
+  }
+  // L^T r = w (upper with the diagonal), descending
+  r.assign(n, 0.0);
+  for (int p = n - 1; p >= 0; --p) {
+    double v = w[p];
+    for (int q = p + 1; q <= std::min(n - 1, p + 2 * bw); ++q)
+      if (fst(q) <= p) v = v - at(q, p) * r[q];
+    r[p] = v / at(p, p);
+  }
+  return 0;
+}
+
 // LABS_eval_bw (module_labs.f90:245-293): smallest k for which every entry
 // of D0, D1, D2 outside the packed band pattern is exactly zero.
 int eval_bandwidth(const Collocation &c) {

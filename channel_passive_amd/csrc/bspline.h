@@ -18,5 +18,9 @@ int build_collocation(int order, int n, int knot_map, double stretch, double ly,
 int basis_row(const Collocation &c, double y, int &first, std::vector<double> &vals);
 int eval_bandwidth(const Collocation &c);
 void to_band(const std::vector<double> &full, int n, int bw, std::vector<double> &band);
+// r with r . v = B(y) . LU(D0)^-1 v for every column v: the transposed
+// solve (LU)^T r = b of the packed Crout factor (L with its diagonal, unit
+// U), b = the basis values at y in their columns
+int detection_row(const Collocation &c, const std::vector<double> &d0lu, int bw, double y, std::vector<double> &r);
 
 }  // namespace pb

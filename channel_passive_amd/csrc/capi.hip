@@ -69,8 +69,7 @@ struct pb_ctx {
   double Lx, Lz, dxr, dzr, cflx, rescale, lmfxr, lmfzr;
   pb::Collocation col;
   std::vector<double> d0, d1, d2, d0lu, dyr;
-  int opp_first[2] = {0, 0};
-  std::vector<double> opp_rows;
+  std::vector<double> opp_rows;  // the two detection rows, ny doubles each
   pb::FftPlan pz, px;
   int Bz;
   hipStream_t stream = nullptr;
@@ -214,10 +213,11 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
     c->dyr[i] = 1.0 / (cfg->dy_factor * d);
   }
   if (cfg->opp_alpha != 0.0) {
+    // detection rows r with v(y_d) = r . v (= B(y_d) . LU(D0)^-1 v)
     std::vector<double> lo, hi;
     if (cfg->opp_ydist <= 0.0 || cfg->opp_ydist >= 1.0 ||
-        pb::basis_row(c->col, -1.0 + cfg->opp_ydist, c->opp_first[0], lo) ||
-        pb::basis_row(c->col, 1.0 - cfg->opp_ydist, c->opp_first[1], hi)) {
+        pb::detection_row(c->col, c->d0lu, c->bw, -1.0 + cfg->opp_ydist, lo) ||
+        pb::detection_row(c->col, c->d0lu, c->bw, 1.0 - cfg->opp_ydist, hi)) {
       ctx_free(c);
       return fail(PB_E_ARG, "opposition-control plane %g outside the channel", cfg->opp_ydist);
     }
@@ -499,23 +499,10 @@ int pb_init_stores(pb_ctx *c) {
 }
 
 // ---- stage launchers ------------------------------------------------------
-static int launch_oppvd(pb_ctx *c, const double2 *vfield_base_minus_one) {
-  pb::LineParams P = c->line_params(1, 0.0);
+static int launch_oppvd(pb_ctx *c, const double2 *state) {
   StageTimer tm(c, PB_STAGE_OPPVD);
-  const dim3 grid(blocks_for(c->nsys, 64)), blk(64);
-  const pb::cplx *U = (const pb::cplx *)vfield_base_minus_one;
-  pb::cplx *buf = (pb::cplx *)c->buf, *vd = (pb::cplx *)c->vd;
-  const int lo = c->opp_first[0], hi = c->opp_first[1], k = c->order;
-  // windowed sweeps when the interpolation stencil fits the up-sweep window
-  const bool win = c->fused && k <= 2 * c->bw + 1 && lo <= hi;
-  if (win && c->bw == 3)
-    hipLaunchKernelGGL(pb::opp_kernel<3>, grid, blk, 0, c->stream, P, U, buf, vd, c->plane, c->nsys, lo, hi, c->drows, k);
-  else if (win && c->bw == 4)
-    hipLaunchKernelGGL(pb::opp_kernel<4>, grid, blk, 0, c->stream, P, U, buf, vd, c->plane, c->nsys, lo, hi, c->drows, k);
-  else if (win && c->bw == 5)
-    hipLaunchKernelGGL(pb::opp_kernel<5>, grid, blk, 0, c->stream, P, U, buf, vd, c->plane, c->nsys, lo, hi, c->drows, k);
-  else
-    hipLaunchKernelGGL(pb::oppvd_kernel, grid, blk, 0, c->stream, P, U, buf, vd, c->plane, c->nsys, lo, hi, c->drows, k);
+  hipLaunchKernelGGL(pb::opp_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream,
+                     (const pb::cplx *)state, (pb::cplx *)c->vd, c->plane, c->nsys, c->ny, c->drows);
   HIPCHK(hipGetLastError());
   return PB_OK;
 }

@@ -918,27 +918,15 @@ __global__ void __launch_bounds__(64) sweep_kernel(LineParams P, WsArgs A, doubl
   run_sweep<BW, SW>(Q, A, dbuf2, idx);
 }
 
-// opposition-control detection planes: vd[0][sys], vd[1][sys]
-__global__ void __launch_bounds__(64) oppvd_kernel(LineParams P, const cplx *U, cplx *buf, cplx *vd, size_t plane,
-                                                   size_t nsys, int first_lo, int first_hi, const double *rows, int k) {
+// opposition-control detection planes of the state's v (slot 1):
+// vd[0][sys] = r_lo . v, vd[1][sys] = r_hi . v (rows = r_lo, r_hi)
+__global__ void __launch_bounds__(64) opp_kernel(const cplx *U, cplx *vd, size_t plane, size_t nsys, int n,
+                                                 const double *rows) {
   const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sys >= nsys) return;
-  const SV v{const_cast<cplx *>(U) + (size_t)1 * plane * P.g.n + sys, plane};
-  const SV b{buf + sys, plane};
-  vd[sys] = interp_plane(P, v, b, first_lo, rows, k);
-  vd[plane + sys] = interp_plane(P, v, b, first_hi, rows + k, k);
-}
-
-// the same with the windowed sweeps: reads v once, y written and read once
-template <int BW>
-__global__ void __launch_bounds__(64) opp_kernel(LineParams P, const cplx *U, cplx *buf, cplx *vd, size_t plane,
-                                                 size_t nsys, int first_lo, int first_hi, const double *rows, int k) {
-  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (sys >= nsys) return;
-  const SV v{const_cast<cplx *>(U) + (size_t)1 * plane * P.g.n + sys, plane};
-  const SV y{buf + sys, plane};
+  const SV v{const_cast<cplx *>(U) + (size_t)1 * plane * n + sys, plane};
   cplx lo, hi;
-  opp_line<BW>(P, v, y, first_lo, first_hi, rows, k, &lo, &hi);
+  opp_dots(v, rows, n, &lo, &hi);
   vd[sys] = lo;
   vd[plane + sys] = hi;
 }

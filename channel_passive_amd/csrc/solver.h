@@ -523,14 +523,20 @@ PB_HD void mean_line_stats(const LineParams &P, const SV m[3], const SV &t1, con
   }
 }
 
-// Detection-plane value of a collocation column: c = D0^{-1} v (in buf),
-// v(y_d) = sum_j B_j(y_d) c_j, ascending j.
-PB_HD cplx interp_plane(const LineParams &P, const SV &v, const SV &buf, int first, const double *vals, int k) {
-  for (int i = 0; i < P.g.n; ++i) buf[i] = v[i];
-  band_bs(P.g, P.D0LU, buf);
-  cplx acc = cscale(buf[first], vals[0]);
-  for (int j = 1; j < k; ++j) acc = cadd(acc, cscale(buf[first + j], vals[j]));
-  return acc;
+// Detection-plane values of a collocation column, both planes: v(y_d) =
+// r . v, ascending rows, with r = LU(D0)^-T B(y_d) formed on the host
+// (rows = r_lo then r_hi, n each): the same value as B(y_d) . D0^-1 v, read
+// in one pass over v.
+PB_HD void opp_dots(const SV &v, const double *rows, int n, cplx *lo, cplx *hi) {
+  const cplx x0 = v[0];
+  cplx a = cscale(x0, ld_uniform(rows)), b = cscale(x0, ld_uniform(rows + n));
+  for (int i = 1; i < n; ++i) {
+    const cplx x = v[i];
+    a = cadd(a, cscale(x, ld_uniform(rows + i)));
+    b = cadd(b, cscale(x, ld_uniform(rows + n + i)));
+  }
+  *lo = a;
+  *hi = b;
 }
 
 // Everything one system needs: field base pointers in the [slot][y][kz][kx]

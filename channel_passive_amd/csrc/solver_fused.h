@@ -909,47 +909,6 @@ PB_HD void deriv_line(const LineParams &P, const SV &x1, const SV &x2, const SV 
   });
 }
 
-// Opposition-control detection planes of one system: c = LU(D0)^-1 v by a
-// down and an up sweep with the arithmetic of band_bs (y stored in the
-// scratch column), then v(y_d) = sum_j B_j(y_d) c_(first+j), ascending j,
-// from the up sweep's window when it reaches row first (needs k <= 2bw+1).
-template <int BW>
-PB_HD cplx opp_interp(const CWin<2 * BW + 1> &w, const double *vals, int k) {
-  cplx acc = cscale(w.w[0], ld_uniform(vals));
-#pragma unroll
-  for (int j = 1; j <= 2 * BW; ++j)
-    if (j < k) acc = cadd(acc, cscale(w.w[j], ld_uniform(vals + j)));
-  return acc;
-}
-template <int BW>
-PB_HD void opp_line(const LineParams &P, const SV &v, const SV &y, int first_lo, int first_hi, const double *rows,
-                    int k, cplx *lo, cplx *hi) {
-  const BandGeom g{P.g.n, BW};
-  const int n = g.n;
-  CWin<BW> wf;
-  wf.clear();
-  down_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
-    constexpr bool MID = decltype(kind)::mid;
-    double lr[2 * BW + 1];
-    load_lrow<BW, MID>(g, P.D0LU, r, lr);
-    const cplx yr = fwd_row<BW, MID>(g, r, lr, wf, y, v[r]);
-    y[r] = yr;
-    wf.push(yr);
-  });
-  CWin<2 * BW + 1> wb;
-  wb.clear();
-  // rows below first_lo are not needed: they do nothing
-  up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
-    constexpr bool MID = decltype(kind)::mid;
-    if (r < first_lo) return;
-    double ur[2 * BW];
-    load_urow<BW, MID>(g, P.D0LU, r, ur);
-    wb.push(bwd_row<BW, MID>(g, r, ur, wb, y[r]));
-    if (r == first_hi) *hi = opp_interp<BW>(wb, rows + k, k);
-    if (r == first_lo) *lo = opp_interp<BW>(wb, rows, k);
-  });
-}
-
 // One sweep: index is a system (kz-major, kx-minor) or, for the
 // factorising sweeps C_nu, C_theta and E, a pair (kz index 0 .. nz/2).
 // A substep runs, for all systems, A1 (s == 1 only), A, B_g, then the theta

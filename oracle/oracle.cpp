@@ -527,8 +527,7 @@ struct Ctx {
   std::vector<int> kx, kz;
   double rk[4][3];
   // opposition control: B-spline rows at the two detection planes
-  int opp_lo_first = 0, opp_hi_first = 0;
-  std::vector<double> opp_lo_vals, opp_hi_vals;
+  std::vector<double> opp_lo_row, opp_hi_row;  // detection rows r: v(y_d) = r . v
   FFTPlan px, pz;
 };
 
@@ -882,11 +881,40 @@ static void wave00(Line &L, cd *u1, cd *u2, cd *u3, cd *u4, cd *u5, cd *u7, cd *
 
 // B-spline interpolation of a collocation-value column at a detection plane:
 // c = LABS_bs(D0_LU, v); v(y_d) = sum_j B_j(y_d) c_j (ascending j).
-static cd interp_at(const Ctx &c, const cd *v, int first, const std::vector<double> &vals) {
-  std::vector<cd> tmp(v, v + c.ny);
-  labs_bs(c.d0lu.data(), c.ny, c.bw, tmp.data());
-  cd acc = mulr(tmp[first], vals[0]);
-  for (size_t j = 1; j < vals.size(); ++j) acc = add(acc, mulr(tmp[first + j], vals[j]));
+// Opposition-control detection row (the project's definition; the
+// reference's hooks are commented out, solve_v_f.f90:253-257):
+// v(y_d) = B(y_d) . D0^-1 v = r . v with r = (LU(D0))^-T b, b = the basis
+// values at y_d in their columns.  D0 = L U (Crout: L with its diagonal,
+// unit U), so U^T w = b forward (rows of U that reach the column, ascending),
+// then L^T r = w backward (rows whose band starts at or before the column,
+// ascending), each sum subtracted term by term.
+static int detection_row(const Ctx &c, double yd, std::vector<double> &r) {
+  std::vector<double> vals;
+  const int first = bspl_row_at(c.kgsl, c.ny, c.brk, yd, vals);
+  if (first < 0) return -1;
+  const int n = c.ny, bw = c.bw;
+  Band B{n, bw, 2 * bw + 1};
+  auto at = [&](int row, int col) { return c.d0lu[(size_t)row * B.w + B.slot(row, col)]; };
+  std::vector<double> b(n, 0.0), w(n);
+  for (size_t j = 0; j < vals.size(); ++j) b[first + j] = vals[j];
+  for (int col = 0; col < n; ++col) {
+    double v = b[col];
+    for (int row = std::max(0, col - 2 * bw); row < col; ++row)
+      if (col <= B.last(row)) v = v - at(row, col) * w[row];
+    w[col] = v;
+  }
+  r.assign(n, 0.0);
+  for (int p = n - 1; p >= 0; --p) {
+    double v = w[p];
+    for (int q = p + 1; q <= std::min(n - 1, p + 2 * bw); ++q)
+      if (B.first(q) <= p) v = v - at(q, p) * r[q];
+    r[p] = v / at(p, p);
+  }
+  return 0;
+}
+static cd detect_dot(const Ctx &c, const cd *v, const std::vector<double> &r) {
+  cd acc = mulr(v[0], r[0]);
+  for (int i = 1; i < c.ny; ++i) acc = add(acc, mulr(v[i], r[i]));
   return acc;
 }
 
@@ -1035,9 +1063,8 @@ void *orc_create(const pb_config *cfg) {
     c->dyr[i] = 1.0 / (cfg->dy_factor * d);
   }
   if (cfg->opp_alpha != 0.0) {
-    c->opp_lo_first = bspl_row_at(c->kgsl, c->ny, c->brk, -1.0 + cfg->opp_ydist, c->opp_lo_vals);
-    c->opp_hi_first = bspl_row_at(c->kgsl, c->ny, c->brk, 1.0 - cfg->opp_ydist, c->opp_hi_vals);
-    if (c->opp_lo_first < 0 || c->opp_hi_first < 0) { delete c; return nullptr; }
+    if (detection_row(*c, -1.0 + cfg->opp_ydist, c->opp_lo_row) ||
+        detection_row(*c, 1.0 - cfg->opp_ydist, c->opp_hi_row)) { delete c; return nullptr; }
   }
   c->px = make_plan(c->nx32);
   c->pz = make_plan(c->nz32);
@@ -1200,8 +1227,8 @@ int orc_opp_planes(void *h, const double *Sd, double *out) {
   for (long l = 0; l < (long)nl; ++l) {
     const int k = (int)(l % c->nxh), j = (int)(l / c->nxh);
     const cd *v = S + ix(*c, 1, j, k, 0);
-    o[l] = interp_at(*c, v, c->opp_lo_first, c->opp_lo_vals);
-    o[nl + l] = interp_at(*c, v, c->opp_hi_first, c->opp_hi_vals);
+    o[l] = detect_dot(*c, v, c->opp_lo_row);
+    o[nl + l] = detect_dot(*c, v, c->opp_hi_row);
   }
   return 0;
 }

@@ -129,35 +129,20 @@ extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, doubl
   return 0;
 }
 
-// Opposition-control detection planes with the windowed device sweep.
+// Opposition-control detection planes with the device's per-system code.
 // V: the v field in the device layout [y][kz][kx]; out: 2 planes [kz][kx].
-template <int BW>
-static void opp_all(const pb::LineParams &P, pb::cplx *V, pb::cplx *y, size_t plane, int f0, int f1,
-                    const double *rows, int k, pb::cplx *out) {
-  for (size_t sys = 0; sys < plane; ++sys) {
-    const pb::SV v{V + sys, plane}, ys{y + sys, plane};
-    pb::opp_line<BW>(P, v, ys, f0, f1, rows, k, out + sys, out + plane + sys);
-  }
-}
 extern "C" int emu_opp_planes(const pb_config *cfg, double *V, double *out) {
   Setup S;
   if (make_setup(cfg, S)) return -1;
-  pb::LineParams P = params(cfg, S, 1, 0.0);
-  int f0 = 0, f1 = 0;
   std::vector<double> lo, hi;
-  if (pb::basis_row(S.col, -1.0 + cfg->opp_ydist, f0, lo) || pb::basis_row(S.col, 1.0 - cfg->opp_ydist, f1, hi))
+  if (pb::detection_row(S.col, S.d0lu, S.bw, -1.0 + cfg->opp_ydist, lo) ||
+      pb::detection_row(S.col, S.d0lu, S.bw, 1.0 - cfg->opp_ydist, hi))
     return -3;
   lo.insert(lo.end(), hi.begin(), hi.end());
   const size_t plane = (size_t)(cfg->nz) * (cfg->nx / 2);
-  std::vector<pb::cplx> y(plane * S.n);
   pb::cplx *Vc = (pb::cplx *)V, *o = (pb::cplx *)out;
-  if (S.order > 2 * S.bw + 1 || f0 > f1) return -4;
-  switch (S.bw) {
-    case 3: opp_all<3>(P, Vc, y.data(), plane, f0, f1, lo.data(), S.order, o); break;
-    case 4: opp_all<4>(P, Vc, y.data(), plane, f0, f1, lo.data(), S.order, o); break;
-    case 5: opp_all<5>(P, Vc, y.data(), plane, f0, f1, lo.data(), S.order, o); break;
-    default: return -2;
-  }
+  for (size_t sys = 0; sys < plane; ++sys)
+    pb::opp_dots(pb::SV{Vc + sys, plane}, lo.data(), S.n, o + sys, o + plane + sys);
   return 0;
 }
 

@@ -115,3 +115,26 @@ def test_opposition_detection_plane(oracle_lib):
     yd = 1.0 - cfg.opp_yplus / cfg.re_tau
     exact = (1 - yd * yd) ** 2 * (0.3 + 0.1j)
     assert abs(vd[0, 2, 3] - exact) <= 1e-14 and abs(vd[1, 2, 3] - exact) <= 1e-14
+
+
+def test_detection_row_dense(oracle_lib):
+    """The detection values r . v (r = LU(D0)^-T B(y_d), transposed band
+    solve) equal B(y_d) . D0^-1 v computed with a dense solve, on random
+    columns at both planes and for two band widths."""
+    from oracle import oracle as O
+    from tests.helpers import unpack_band
+    for extra in ({}, {"ny": 65, "labs_k": 9}):
+        cfg = dataclasses.replace(preset("tiny"), opp_alpha=1.0, opp_yplus=20.0, **extra)
+        orc = oracle_lib.Oracle(cfg)
+        rng = np.random.default_rng(6)
+        S = np.zeros((4, cfg.nz, cfg.nxh, cfg.ny), complex)
+        S[1] = rng.standard_normal(S[1].shape) + 1j * rng.standard_normal(S[1].shape)
+        vd = orc.opp_planes(S)
+        D0 = unpack_band(orc.d0, orc.bw)
+        C = np.linalg.solve(D0, S[1].reshape(-1, cfg.ny).T)  # columns: coefficients of each line
+        yd = 1.0 - cfg.opp_yplus / cfg.re_tau
+        for plane, y in ((0, -yd), (1, yd)):
+            first, vals = O.bspline_row(cfg.ny, cfg.labs_k, cfg.knot_map, cfg.stretch, y)
+            want = (vals[:, None] * C[first:first + len(vals)]).sum(axis=0).reshape(cfg.nz, cfg.nxh)
+            scale = np.abs(S[1]).max()
+            assert np.abs(vd[plane] - want).max() <= 1e-12 * scale, plane
