@@ -78,8 +78,9 @@ def test_fused_wavespace_bitwise_cpu(oracle_lib, emu_lib, name, alpha, extra):
                                         ("tiny", {"knot_map": 0, "stretch": 0.0, "labs_k": 6, "ny": 34}),
                                         ("tiny", {"ny": 65, "labs_k": 9})])
 def test_opp_planes_bitwise_cpu(oracle_lib, emu_lib, name, extra):
-    """The windowed detection-plane sweep (solver_fused.h opp_line) gives the
-    oracle's v(y_d) (LU(D0) solve, then the B-spline row) bit for bit."""
+    """The per-system detection dot products (solver.h opp_dots over the rows
+    LU(D0)^-T B(y_d) of bspline.cpp detection_row) give the oracle's v(y_d)
+    bit for bit."""
     cfg = dataclasses.replace(preset(name), opp_alpha=0.7, opp_yplus=20.0, **extra)
     orc = oracle_lib.Oracle(cfg)
     S = make_state(cfg, seed=29)
