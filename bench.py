@@ -26,7 +26,7 @@ STAGES = ["oppvd", "zinv", "xphys", "velmax", "zfwd", "wavespace"]
 # stages that are a single kernel each (the wall-normal stage is a chain of
 # sweep kernels on three streams); the roofline object names the single
 # kernel with the largest time per step
-KERNEL_OF_STAGE = {"oppvd": "opp_kernel", "zinv": "zinv_stream_kernel", "xphys": "xphys_kernel",
+KERNEL_OF_STAGE = {"oppvd": "opp_kernel", "zinv": "zinv_stream_kernel", "xphys": "xphys_stream_kernel",
                    "zfwd": "zfwd_stream_kernel"}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "re1000_1024x385x768_hbm_traffic.txt")
 

@@ -99,7 +99,9 @@ struct pb_ctx {
   bool zs_on = true;   // streamed (double-buffered, one block per CU) z transforms where they apply
   bool zs_ok = false;
   bool zs_inv = true, zs_fwd = true;
-  int ncu = 0;         // compute units: grid of the streamed z transforms
+  bool xs_on = true;   // streamed x pass (one block per CU, next line copied to LDS) where it applies
+  bool xs_ok = false;
+  int ncu = 0;         // compute units: grid of the streamed transforms
   pb::RK3 rk;
   // per-stage event timing
   bool timing = false;
@@ -190,6 +192,7 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
     c->zs_inv = (e[0] != 'f');
     c->zs_fwd = (e[0] != 'i');
   }
+  if (const char *e = getenv("PB_X_STREAM")) c->xs_on = (e[0] != '0');
   int r = pb::build_collocation(c->order, c->ny, cfg->knot_map, cfg->stretch, 2.0 * cfg->lmfy, c->col);
   if (r) { ctx_free(c); return fail(PB_E_ARG, "B-spline set-up failed (%d): check ny, labs_k, knots", r); }
   c->bw = pb::eval_bandwidth(c->col);
@@ -263,10 +266,14 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
     // the streamed z transforms: same passes and twiddles on 4-column tiles,
     // two tile buffers in static LDS
     const int lzs = (2 * pb::zs::NE + pb::zs::NTW) * (int)sizeof(double2);
-    c->zs_ok = c->zs_on && c->ct_z_ok && c->nxh % pb::zs::B == 0 && lzs <= lds_max &&
-               hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c->ncu > 0;
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) c->ncu = 0;
+    c->zs_ok = c->zs_on && c->ct_z_ok && c->nxh % pb::zs::B == 0 && lzs <= lds_max && c->ncu > 0;
     using FX = pb::FftX1536;
     c->ct_x_ok = c->ct_fft && c->nx32 == FX::N && FX::L::SIZE * (int)sizeof(double2) <= lds_max;
+    // the streamed x pass: 4 transform lines, the 4 input rows and the twiddles in static LDS
+    const int lxs = (pb::xs::L4::SIZE + 4 * pb::xs::NH + pb::xs::NTW) * (int)sizeof(double2);
+    c->xs_ok = c->xs_on && c->ct_x_ok && c->nxh == pb::xs::NH && lxs <= lds_max && c->ncu > 0 &&
+               (size_t)c->ncu <= (size_t)c->ny * c->nz32 * 16;  // one vpart entry per block
   }
   c->plane = (size_t)c->nz * c->nxh;
   c->field = c->plane * c->ny;
@@ -577,6 +584,19 @@ static int launch_zfwd_stream(pb_ctx *c, int want_vv) {
   return PB_OK;
 }
 
+#ifndef PB_XS_THREADS
+#define PB_XS_THREADS 512
+#endif
+static int launch_xphys_stream(pb_ctx *c) {
+  constexpr int T = PB_XS_THREADS;
+  StageTimer tm(c, PB_STAGE_XPHYS);
+  hipLaunchKernelGGL(pb::xphys_stream_kernel<T>, dim3(c->ncu), dim3(T), 0, c->stream, c->pad, c->twx, c->nz32, c->ny,
+                     c->pfield, c->ddyr, c->dxr, c->dzr, c->vpart);
+  c->vpart_n = (size_t)c->ncu;  // one partial maximum per block
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+
 static int launch_nonlinear(pb_ctx *c, int update_dt, int want_vv) {
   int r = PB_OK;
   const int ez = pick_ept((size_t)c->nz32 * c->Bz), ex = pick_ept((size_t)c->nx32 * 2);
@@ -584,7 +604,8 @@ static int launch_nonlinear(pb_ctx *c, int update_dt, int want_vv) {
   else if (ct_z(c)) r = launch_zinv_t<pb::FftZ1152>(c);
   else PB_EPT_DISPATCH(ez, r = launch_zinv_t<F>(c));
   if (r) return r;
-  if (ct_x(c)) r = launch_xphys_t<pb::FftX1536>(c, want_vv);
+  if (c->xs_ok && !want_vv) r = launch_xphys_stream(c);
+  else if (ct_x(c)) r = launch_xphys_t<pb::FftX1536>(c, want_vv);
   else PB_EPT_DISPATCH(ex, r = launch_xphys_t<F>(c, want_vv));
   if (r) return r;
   {

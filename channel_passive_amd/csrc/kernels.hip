@@ -766,6 +766,137 @@ __global__ void __launch_bounds__(zs::T, 3) zfwd_stream_kernel(const double2 *__
   }
 }
 
+// ------------------------------------------- streamed x pass (nx32 = 1536)
+// One block per CU walks a strided list of (y, z) lines (line l = y*nz32 + z).
+// The four input rows of the next line (u, v, w, theta, modes 0..nxh-1) are
+// copied global -> LDS directly while the current line is transformed.  All
+// eight products go through one forward transform of four lines
+// (P1 + i P2, P3 + i P4, P5 + i P7, P8 + i P9), so no product is held in
+// registers across a transform and a line takes 15 barriers instead of 24.
+// Same passes, twiddles and arithmetic as xphys_kernel<FftX1536>, hence the
+// same results bit for bit.  The v*v round of statistics time stays with
+// xphys_kernel.  Each block writes one CFL partial maximum, over all its
+// lines (vpart[blockIdx.x]).
+// Vector memory operations count down in issue order (vmcnt), so:
+//  * no global load is issued between a copy and the wait for it (the
+//    twiddles live in LDS, dy is read before the copy); it would also wait
+//    for the copy;
+//  * the wait for line l's copy at the top of the loop lets the previous
+//    line's stores (STORES per thread, all issued after that copy) stay in
+//    flight.
+namespace xs {
+constexpr int N = 1536, NH = 512, NTW = N / 6;  // twiddles W^k, k < N/6 (radix 16, 16, 6)
+using L4 = Lines<N, 4, true>;
+using L2 = Lines<N, 2, true>;  // lines 0 and 1 of the L4 layout (index b*N + n, padded)
+}  // namespace xs
+__shared__ double2 xs_buf[xs::L4::SIZE];
+__shared__ double2 xs_in[4 * xs::NH];
+__shared__ double2 xs_tw[xs::NTW];
+
+// products at this thread's points of lines 0, 1 (u + i v, w + i theta) into
+// lines 0..3; each point is read and rewritten by one thread
+template <int T>
+__device__ __forceinline__ void xs_products(double2 *x, double dxr, double dy, double dzr, double &vmax) {
+  using L4 = xs::L4;
+#pragma unroll
+  for (int q = 0; q < (xs::N + T - 1) / T; ++q) {
+    const int m = threadIdx.x + q * T;
+    if (m < xs::N) {
+      const double2 a = x[L4::at(m, 0)], c = x[L4::at(m, 1)];
+      const double t1 = a.x, t2 = a.y, t3 = c.x, t4 = c.y;
+      const double vel = (fabs(t1) * dxr) + (fabs(t2) * dy) + (fabs(t3) * dzr);
+      vmax = fmax(vel, vmax);
+      const double bv2 = t2 * t2;
+      x[L4::at(m, 0)] = c2(t1 * t1 - bv2, t1 * t2);
+      x[L4::at(m, 1)] = c2(t3 * t3 - bv2, t2 * t3);
+      x[L4::at(m, 2)] = c2(t1 * t3, t1 * t4);
+      x[L4::at(m, 3)] = c2(t2 * t4, t3 * t4);
+    }
+  }
+}
+
+template <int T>
+__global__ void __launch_bounds__(T, 1) xphys_stream_kernel(double2 *__restrict__ pad, const double2 *__restrict__ twg,
+                                                            int nz32, int ny, size_t pfield,
+                                                            const double *__restrict__ dyr, double dxr, double dzr,
+                                                            double *__restrict__ vpart) {
+  using namespace xs;
+  constexpr int NW = T / 64, NCOPY = 4 * NH / 64, NOUT = 4 * NH / T, STORES = 2 * NOUT;
+  static_assert(T % 64 == 0 && NCOPY % NW == 0, "copy instructions split evenly over the waves");
+  static_assert((4 * NH) % T == 0 && STORES < 16, "every thread stores the same count, below the vmcnt field");
+  for (int i = threadIdx.x; i < NTW; i += T) xs_tw[i] = twg[i];
+  const size_t nlines = (size_t)nz32 * ny, G = gridDim.x;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  // row f of line l, element m -> xs_in[f * NH + m]; instruction i fills 64 elements
+  auto issue = [&](size_t l) {
+    const double2 *src = pad + l * NH;
+#pragma unroll
+    for (int k = 0; k < NCOPY / NW; ++k) {
+      const int i = wave + k * NW, f = i / (NH / 64), m = (i % (NH / 64)) * 64 + lane;
+      __builtin_amdgcn_global_load_lds((const void *)(src + (size_t)f * pfield + m),
+                                       (__attribute__((address_space(3))) void *)(xs_in + i * 64), 16, 0, 0);
+    }
+  };
+  size_t l = xcd_rank();
+  double vmax = 0.0;
+  if (l < nlines) issue(l);
+  __builtin_amdgcn_s_waitcnt(0);  // the first line's copy (no stores yet)
+  while (l < nlines) {
+    // line l is in xs_in (vmcnt <= STORES: only the previous line's stores may
+    // still be in flight); every wave is done with the previous line
+    __builtin_amdgcn_s_waitcnt(0x0070 | STORES);  // vmcnt(STORES) expcnt(7) lgkmcnt(0)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    // Hermitian completion of (u + i v) and (w + i theta) into lines 0 and 1
+    for (int m = threadIdx.x; m < N; m += T) {
+      double2 z1 = c2(0.0, 0.0), z2 = c2(0.0, 0.0);
+      if (m == 0) {
+        z1 = c2(xs_in[0].x, xs_in[NH].x);
+        z2 = c2(xs_in[2 * NH].x, xs_in[3 * NH].x);
+      } else if (m < NH) {
+        const double2 a = xs_in[m], b = xs_in[NH + m], c = xs_in[2 * NH + m], d = xs_in[3 * NH + m];
+        z1 = c2(a.x - b.y, a.y + b.x);
+        z2 = c2(c.x - d.y, c.y + d.x);
+      } else if (m > N - NH) {
+        const int q = N - m;
+        const double2 a = xs_in[q], b = xs_in[NH + q], c = xs_in[2 * NH + q], d = xs_in[3 * NH + q];
+        z1 = c2(a.x + b.y, -a.y + b.x);
+        z2 = c2(c.x + d.y, -c.y + d.x);
+      }
+      xs_buf[L4::at(m, 0)] = z1;
+      xs_buf[L4::at(m, 1)] = z2;
+    }
+    const double dy = dyr[l / nz32];
+    pass_sync<true>();  // every wave has read xs_in
+    ct_passes_lds<T, L2, N, 2, +1, 1, 16, 16, 6>(xs_buf, xs_tw);
+    xs_products<T>(xs_buf, dxr, dy, dzr, vmax);
+    if (l + G < nlines) issue(l + G);
+    pass_sync<true>();
+    ct_passes_lds<T, L4, N, 4, -1, 1, 16, 16, 6>(xs_buf, xs_tw);
+    // unpack the two real spectra of each line, keep modes 0..nxh-1
+    double2 *line = pad + l * NH;
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) {
+      const int e = threadIdx.x + o * T, b = e / NH, m = e % NH, q = (m == 0) ? 0 : N - m;
+      const double2 Z = xs_buf[L4::at(m, b)], Zc = xs_buf[L4::at(q, b)];
+      line[(size_t)(2 * b) * pfield + m] = c2((Z.x + Zc.x) * 0.5, (Z.y - Zc.y) * 0.5);
+      line[(size_t)(2 * b + 1) * pfield + m] = c2((Z.y + Zc.y) * 0.5, (Zc.x - Z.x) * 0.5);
+    }
+    l += G;
+  }
+  // the block's CFL maximum: across each wave by lane exchanges, then the waves
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, d, 64));
+  __shared__ double xs_vm[NW];
+  if (lane == 0) xs_vm[wave] = vmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < NW; ++w) vmax = fmax(vmax, xs_vm[w]);
+    vpart[blockIdx.x] = vmax;
+  }
+}
+
 // ------------------------------------------------- CFL maximum and dt update
 // single block: velmax = max(vpart); on substep 1 (update != 0)
 // dt = dt_factor*dt + (1-dt_factor)*cflx/velmax  (dft_x_realspace.f90:489-498)

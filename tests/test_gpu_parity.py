@@ -162,6 +162,29 @@ def test_z_stream_bitwise(lib):
     assert vm1 == vm0
 
 
+@pytest.mark.parametrize("seed", [7, 8])
+def test_x_stream_bitwise(lib, seed):
+    """The streamed x pass (one block per CU, next line copied into LDS, one
+    forward transform of all four product lines) against the one-line-per-block
+    kernel (two forward transforms of two lines): same passes and twiddles,
+    identical products and CFL maximum."""
+    cfg = CFGS["re1000_flat"]
+    S = full_spectrum(cfg, seed=seed)
+    out = {}
+    for mode in ("1", "0"):
+        os.environ["PB_X_STREAM"] = mode
+        try:
+            ch = lib.Channel(cfg)
+        finally:
+            del os.environ["PB_X_STREAM"]
+        out[mode] = ch.nonlinear_host(S, want_vv=False)
+        ch.close()
+    (P1, vm1), (P0, vm0) = out["1"], out["0"]
+    for slot in range(8):
+        assert np.array_equal(P1[slot], P0[slot]), slot
+    assert vm1 == vm0
+
+
 @pytest.mark.parametrize("name", ["re180_64", "re180_192", "re1000_thin"])
 def test_pressure_stats(lib, oracle_lib, name):
     """Statistics-time substep: <vv> bitwise, p (get_pressure) against the
