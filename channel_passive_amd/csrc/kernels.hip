@@ -304,18 +304,21 @@ __device__ __forceinline__ int lds_swz(int i) { return PB_LDS_SWZ ? i ^ ((i >> 4
 // Layout of B lines of N elements in LDS: interleaved (element n of line b
 // at n*B + b, the z transforms, whose global rows hold B consecutive kx;
 // XOR-swizzled, a permutation, as the streamed copies into LDS need) or
-// separate (b*N + n, the pair of x lines; one pad element after every 16,
-// which breaks the power-of-two strides as well and keeps the index of
-// element n0 + r*S, S a multiple of 16, linear in r: the passes address
-// their R elements with constant offsets from one base).
+// separate (b*N + n, the x lines; one pad element after every 32).
+// ds_read_b128 serves 32 consecutive lanes per clock and ds_write_b128 16:
+// with one pad per 32, every aligned run of 32 consecutive elements (all
+// reads of the passes) is conflict-free, and the element index of
+// n0 + r*S stays linear in r for the strides the passes use, so they address
+// their R elements with constant offsets from one base.  Only the
+// stride-16 stores of a first radix-16 pass meet two lanes per bank group.
 template <int N, int B, bool SEP>
 struct Lines {
   static constexpr bool PAD = SEP;
-  static constexpr int SIZE = PAD ? N * B + (N * B) / 16 : N * B;
-  static_assert(!PAD || N % 16 == 0, "padded lines start on a 16-element boundary");
+  static constexpr int SIZE = PAD ? N * B + (N * B) / 32 : N * B;
+  static_assert(!PAD || N % 32 == 0, "padded lines start on a 32-element boundary");
   __device__ static int at(int n, int b) {
     const int i = SEP ? b * N + n : n * B + b;
-    return PAD ? i + (i >> 4) : lds_swz(i);
+    return PAD ? i + (i >> 5) : lds_swz(i);
   }
   // butterfly t of a pass with N/R butterflies per line -> (line, index)
   __device__ static void split(int t, int NR, int &b, int &j) {
@@ -340,11 +343,14 @@ __device__ __forceinline__ void pass_sync() {
   }
 }
 
-// LDS index of element n0 + r*S of line b (ALIGNED: n0 % 16 == 0 and r*S < 16)
-template <class L, int S, bool ALIGNED>
+// LDS index of element n0 + r*S of line b.  KIND 1: n0 % 16 == 0 and
+// r*S < 16 (no pad crossed); KIND 2: S == 16 and n0 % 32 < 16 (one pad per
+// two steps)
+template <class L, int S, int KIND>
 __device__ __forceinline__ int lds_at(int n0, int b, int r) {
-  if constexpr (L::PAD && S % 16 == 0) return L::at(n0, b) + r * (S + S / 16);
-  else if constexpr (L::PAD && ALIGNED) return L::at(n0, b) + r * S;
+  if constexpr (L::PAD && S % 32 == 0) return L::at(n0, b) + r * (S + S / 32);
+  else if constexpr (L::PAD && KIND == 1) return L::at(n0, b) + r * S;
+  else if constexpr (L::PAD && KIND == 2) return L::at(n0, b) + r * S + (r >> 1);
   else return L::at(n0 + r * S, b);
 }
 
@@ -360,7 +366,7 @@ __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ 
       int b, j;
       L::split(t, NR, b, j);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = x[lds_at<L, NR, false>(j, b, r)];
+      for (int r = 0; r < R; ++r) v[q][r] = x[lds_at<L, NR, 0>(j, b, r)];
     }
   }
   pass_sync<LDS_ONLY>();
@@ -392,9 +398,11 @@ __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ 
       }
       dft<R>(v[q], SIGN);
       const int base = (j / Ns) * Ns * R + k;
+      // first pass of a radix-16 plan: base = 16 j; Ns = 16 with Ns*R a
+      // multiple of 32: base % 32 = k < 16
+      constexpr int KIND = (Ns == 1 && R == 16) ? 1 : ((Ns == 16 && (Ns * R) % 32 == 0) ? 2 : 0);
 #pragma unroll
-      // first pass of a radix-16 plan: base = 16 j
-      for (int r = 0; r < R; ++r) x[lds_at<L, Ns, Ns == 1 && R == 16>(base, b, r)] = v[q][r];
+      for (int r = 0; r < R; ++r) x[lds_at<L, Ns, KIND>(base, b, r)] = v[q][r];
     }
   }
   pass_sync<LDS_ONLY>();
