@@ -666,6 +666,23 @@ __device__ __forceinline__ void all_sync() {
   __builtin_amdgcn_s_barrier();
   asm volatile("" ::: "memory");
 }
+// The wait at the top of the streamed x loop.  Vector memory operations count
+// down in issue order (vmcnt), and each thread issues exactly STORES global
+// stores (the previous line's output) after the copy of the line about to
+// be transformed: vmcnt <= STORES means that copy has landed while those
+// stores may still be in flight.  The barrier: every wave is done with the
+// previous line.  Before the first line (no stores yet) the loop waits for
+// everything.  The compiler must be able to count those stores (an unrolled
+// store loop), or it adds waits of its own before the LDS accesses that
+// could alias the older copy.
+template <int STORES>
+__device__ __forceinline__ void stream_sync() {
+  static_assert(STORES >= 0 && STORES < 16, "low vmcnt field");
+  __builtin_amdgcn_s_waitcnt(0x0070 | STORES);  // vmcnt(STORES) expcnt(7) lgkmcnt(0)
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
 // Copy of element e = swz(p) of a tile into LDS slot p (the swizzle is an
 // involution, so slot p = L::at(e / B, e % B)); the wave's 64 lanes fill 64
 // consecutive slots.  row(z) gives the global row of tile row z, or < 0 for
@@ -785,13 +802,11 @@ __global__ void __launch_bounds__(zs::T, 3) zfwd_stream_kernel(const double2 *__
 // same results bit for bit.  The v*v round of statistics time stays with
 // xphys_kernel.  Each block writes one CFL partial maximum, over all its
 // lines (vpart[blockIdx.x]).
-// Vector memory operations count down in issue order (vmcnt), so:
-//  * no global load is issued between a copy and the wait for it (the
-//    twiddles live in LDS, dy is read before the copy); it would also wait
-//    for the copy;
-//  * the wait for line l's copy at the top of the loop lets the previous
-//    line's stores (STORES per thread, all issued after that copy) stay in
-//    flight.
+// Vector memory operations count down in issue order (vmcnt), so no global
+// load is issued between a copy and the wait for it (the twiddles live in
+// LDS, dy is read before the copy): it would also wait for the copy.  The
+// wait at the top of the loop lets the previous line's stores stay in
+// flight (stream_sync).
 namespace xs {
 constexpr int N = 1536, NH = 512, NTW = N / 6;  // twiddles W^k, k < N/6 (radix 16, 16, 6)
 using L4 = Lines<N, 4, true>;
@@ -850,12 +865,7 @@ __global__ void __launch_bounds__(T, 1) xphys_stream_kernel(double2 *__restrict_
   if (l < nlines) issue(l);
   __builtin_amdgcn_s_waitcnt(0);  // the first line's copy (no stores yet)
   while (l < nlines) {
-    // line l is in xs_in (vmcnt <= STORES: only the previous line's stores may
-    // still be in flight); every wave is done with the previous line
-    __builtin_amdgcn_s_waitcnt(0x0070 | STORES);  // vmcnt(STORES) expcnt(7) lgkmcnt(0)
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
+    stream_sync<STORES>();  // line l is in xs_in; every wave is done with the previous line
     // Hermitian completion of (u + i v) and (w + i theta) into lines 0 and 1
     for (int m = threadIdx.x; m < N; m += T) {
       double2 z1 = c2(0.0, 0.0), z2 = c2(0.0, 0.0);
