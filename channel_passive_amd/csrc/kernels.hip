@@ -354,6 +354,31 @@ __device__ __forceinline__ int lds_at(int n0, int b, int r) {
   else return L::at(n0 + r * S, b);
 }
 
+// Twiddles and DFT of one butterfly (index k within Ns) of a Stockham pass
+template <int R, int SIGN, int Ns, int tstep>
+__device__ __forceinline__ void bfly(double2 *v, const double2 *__restrict__ tw, int k) {
+  if (Ns > 1) {
+#if PB_FFT_TWPOW
+    // one table load; W^(r k) = (W^k)^r by binary powering (depth <= log2 R)
+    double2 w[R];
+    w[1] = tw[k * tstep];
+    if (SIGN > 0) w[1].y = -w[1].y;
+#pragma unroll
+    for (int r = 2; r < R; ++r) w[r] = (r & 1) ? cmul2(w[r - 1], w[1]) : cmul2(w[r / 2], w[r / 2]);
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul2(v[r], w[r]);
+#else
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      double2 w = tw[r * k * tstep];
+      if (SIGN > 0) w.y = -w.y;
+      v[r] = cmul2(v[r], w);
+    }
+#endif
+  }
+  dft<R>(v, SIGN);
+}
+
 template <int T, class L, int N, int B, int SIGN, int Ns, int R, bool LDS_ONLY = false>
 __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ tw) {
   constexpr int NR = N / R, NBF = NR * B, ROUNDS = (NBF + T - 1) / T;
@@ -377,26 +402,7 @@ __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ 
       int b, j;
       L::split(t, NR, b, j);
       const int k = j % Ns;
-      if (Ns > 1) {
-#if PB_FFT_TWPOW
-        // one table load; W^(r k) = (W^k)^r by binary powering (depth <= log2 R)
-        double2 w[R];
-        w[1] = tw[k * tstep];
-        if (SIGN > 0) w[1].y = -w[1].y;
-#pragma unroll
-        for (int r = 2; r < R; ++r) w[r] = (r & 1) ? cmul2(w[r - 1], w[1]) : cmul2(w[r / 2], w[r / 2]);
-#pragma unroll
-        for (int r = 1; r < R; ++r) v[q][r] = cmul2(v[q][r], w[r]);
-#else
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          double2 w = tw[r * k * tstep];
-          if (SIGN > 0) w.y = -w.y;
-          v[q][r] = cmul2(v[q][r], w);
-        }
-#endif
-      }
-      dft<R>(v[q], SIGN);
+      bfly<R, SIGN, Ns, tstep>(v[q], tw, k);
       const int base = (j / Ns) * Ns * R + k;
       // first pass of a radix-16 plan: base = 16 j; Ns = 16 with Ns*R a
       // multiple of 32: base % 32 = k < 16
@@ -838,15 +844,52 @@ __device__ __forceinline__ void xs_products(double2 *x, double dxr, double dy, d
   }
 }
 
+// Last forward pass (radix 6 over Ns = 256: butterfly k of line b holds
+// Z[k + 256 r], r < 6) fused with the unpack of the two real spectra of each
+// line.  Thread (b, u) takes butterflies k1 = u and k2 = 256 - u (u = 0:
+// k1 = 0, k2 = 128), which hold both halves of the pairs (m, N - m) for
+// m = k1, k1 + 256, k2, k2 + 256: N - k1 - 256 r = k2 + 256 (5 - r).  The
+// results stay in registers: no LDS store of the last pass, no reads of the
+// unpack, one barrier less.  Same butterflies and unpack arithmetic as
+// ct_pass and xphys_kernel.  Every thread issues exactly 8 stores.
+template <int T>
+__device__ __forceinline__ void xs_last_pass_unpack(const double2 *x, const double2 *tw, double2 *line,
+                                                    size_t pfield) {
+  using L4 = xs::L4;
+  constexpr int N = xs::N, R = 6, Ns = 256, NR = N / R, tstep = N / (Ns * R);
+  static_assert(T == 4 * 128 && NR == Ns && xs::NH == 2 * Ns, "4 lines x 128 butterfly pairs");
+  const int b = threadIdx.x / 128, u = threadIdx.x % 128;
+  const int k1 = u, k2 = u ? Ns - u : Ns / 2;
+  double2 v1[R], v2[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    v1[r] = x[lds_at<L4, NR, 0>(k1, b, r)];
+    v2[r] = x[lds_at<L4, NR, 0>(k2, b, r)];
+  }
+  bfly<R, -1, Ns, tstep>(v1, tw, k1);
+  bfly<R, -1, Ns, tstep>(v2, tw, k2);
+  // Z[N - m]: u > 0 from the other butterfly; u = 0: Z[0] for m = 0 and the
+  // same butterfly (k = 0, 128 pair with themselves)
+  const double2 cA = u ? v2[5] : v1[0], cB = u ? v2[4] : v1[5];
+  const double2 cC = u ? v1[5] : v2[5], cD = u ? v1[4] : v2[4];
+  auto emit = [&](int m, double2 Z, double2 Zc) {
+    line[(size_t)(2 * b) * pfield + m] = c2((Z.x + Zc.x) * 0.5, (Z.y - Zc.y) * 0.5);
+    line[(size_t)(2 * b + 1) * pfield + m] = c2((Z.y + Zc.y) * 0.5, (Zc.x - Z.x) * 0.5);
+  };
+  emit(k1, v1[0], cA);
+  emit(k1 + Ns, v1[1], cB);
+  emit(k2, v2[0], cC);
+  emit(k2 + Ns, v2[1], cD);
+}
+
 template <int T>
 __global__ void __launch_bounds__(T, 1) xphys_stream_kernel(double2 *__restrict__ pad, const double2 *__restrict__ twg,
                                                             int nz32, int ny, size_t pfield,
                                                             const double *__restrict__ dyr, double dxr, double dzr,
                                                             double *__restrict__ vpart) {
   using namespace xs;
-  constexpr int NW = T / 64, NCOPY = 4 * NH / 64, NOUT = 4 * NH / T, STORES = 2 * NOUT;
+  constexpr int NW = T / 64, NCOPY = 4 * NH / 64, STORES = 8;  // stores per thread: xs_last_pass_unpack
   static_assert(T % 64 == 0 && NCOPY % NW == 0, "copy instructions split evenly over the waves");
-  static_assert((4 * NH) % T == 0 && STORES < 16, "every thread stores the same count, below the vmcnt field");
   for (int i = threadIdx.x; i < NTW; i += T) xs_tw[i] = twg[i];
   const size_t nlines = (size_t)nz32 * ny, G = gridDim.x;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -891,16 +934,8 @@ __global__ void __launch_bounds__(T, 1) xphys_stream_kernel(double2 *__restrict_
     xs_products<T>(xs_buf, dxr, dy, dzr, vmax);
     if (l + G < nlines) issue(l + G);
     pass_sync<true>();
-    ct_passes_lds<T, L4, N, 4, -1, 1, 16, 16, 6>(xs_buf, xs_tw);
-    // unpack the two real spectra of each line, keep modes 0..nxh-1
-    double2 *line = pad + l * NH;
-#pragma unroll
-    for (int o = 0; o < NOUT; ++o) {
-      const int e = threadIdx.x + o * T, b = e / NH, m = e % NH, q = (m == 0) ? 0 : N - m;
-      const double2 Z = xs_buf[L4::at(m, b)], Zc = xs_buf[L4::at(q, b)];
-      line[(size_t)(2 * b) * pfield + m] = c2((Z.x + Zc.x) * 0.5, (Z.y - Zc.y) * 0.5);
-      line[(size_t)(2 * b + 1) * pfield + m] = c2((Z.y + Zc.y) * 0.5, (Zc.x - Z.x) * 0.5);
-    }
+    ct_passes_lds<T, L4, N, 4, -1, 1, 16, 16>(xs_buf, xs_tw);  // passes 1, 2; the third one:
+    xs_last_pass_unpack<T>(xs_buf, xs_tw, pad + l * NH, pfield);
     l += G;
   }
   // the block's CFL maximum: across each wave by lane exchanges, then the waves
