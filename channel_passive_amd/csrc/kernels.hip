@@ -803,9 +803,13 @@ __global__ void __launch_bounds__(zs::T, 3) zfwd_stream_kernel(const double2 *__
 // copied global -> LDS directly while the current line is transformed.  All
 // eight products go through one forward transform of four lines
 // (P1 + i P2, P3 + i P4, P5 + i P7, P8 + i P9), so no product is held in
-// registers across a transform and a line takes 15 barriers instead of 24.
-// Same passes, twiddles and arithmetic as xphys_kernel<FftX1536>, hence the
-// same results bit for bit.  The v*v round of statistics time stays with
+// registers across a transform; the completion is folded into the first
+// inverse pass and the unpack into the last forward pass, so a line takes
+// 11 barriers instead of 24.
+// Same passes, twiddles and arithmetic as xphys_kernel<FftX1536>; the
+// complex products inside the transforms may pair into FMAs differently
+// (they are contracted), so the two agree to a few ulp, not bit for bit.
+// The v*v round of statistics time stays with
 // xphys_kernel.  Each block writes one CFL partial maximum, over all its
 // lines (vpart[blockIdx.x]).
 // Vector memory operations count down in issue order (vmcnt), so no global
@@ -844,13 +848,50 @@ __device__ __forceinline__ void xs_products(double2 *x, double dxr, double dy, d
   }
 }
 
+// First inverse pass (radix 16, Ns = 1, lines 0 and 1) fused with the
+// Hermitian completion of (u + i v) and (w + i theta): butterfly j of line b
+// takes elements m = j + 96 r straight from the copied rows (in[f * NH + m]),
+// so the completed lines are never stored to LDS.  Same values as the
+// completion of xphys_kernel (FFTW's c2r ignores Im of the DC bin; modes
+// above nxh are the conjugates of the mirrored ones, the band between them
+// zero) and the same butterflies as ct_pass.  Reads from in, writes to x:
+// one barrier, after the writes.
+template <int T>
+__device__ __forceinline__ void xs_first_pass_completion(const double2 *in, double2 *x) {
+  using L2 = xs::L2;
+  constexpr int N = xs::N, NH = xs::NH, R = 16, NR = N / R, NBF = 2 * NR;
+  static_assert(NBF <= T, "one butterfly per thread");
+  const int t = threadIdx.x;
+  if (t < NBF) {
+    const int b = t / NR, j = t % NR;
+    const double2 *p0 = in + (2 * b) * NH, *p1 = in + (2 * b + 1) * NH;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int m = j + r * NR;
+      const bool lo = m < NH, hi = m > N - NH;
+      const int q = lo ? m : (hi ? N - m : 0);
+      const double2 a = p0[q], c = p1[q];
+      const double cy = lo ? c.y : -c.y, ay = lo ? a.y : -a.y;  // conjugate mirror: exact negations
+      double2 z = c2(a.x - cy, ay + c.x);
+      if (r == 0 && m == 0) z = c2(a.x, c.x);
+      if (!lo && !hi) z = c2(0.0, 0.0);
+      v[r] = z;
+    }
+    bfly<R, +1, 1, 1>(v, nullptr, 0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[lds_at<L2, 1, 1>(16 * j, b, r)] = v[r];
+  }
+  pass_sync<true>();
+}
+
 // Last forward pass (radix 6 over Ns = 256: butterfly k of line b holds
 // Z[k + 256 r], r < 6) fused with the unpack of the two real spectra of each
 // line.  Thread (b, u) takes butterflies k1 = u and k2 = 256 - u (u = 0:
 // k1 = 0, k2 = 128), which hold both halves of the pairs (m, N - m) for
 // m = k1, k1 + 256, k2, k2 + 256: N - k1 - 256 r = k2 + 256 (5 - r).  The
 // results stay in registers: no LDS store of the last pass, no reads of the
-// unpack, one barrier less.  Same butterflies and unpack arithmetic as
+// unpack, one barrier less.  Same butterflies (bfly) and unpack arithmetic as
 // ct_pass and xphys_kernel.  Every thread issues exactly 8 stores.
 template <int T>
 __device__ __forceinline__ void xs_last_pass_unpack(const double2 *x, const double2 *tw, double2 *line,
@@ -909,28 +950,9 @@ __global__ void __launch_bounds__(T, 1) xphys_stream_kernel(double2 *__restrict_
   __builtin_amdgcn_s_waitcnt(0);  // the first line's copy (no stores yet)
   while (l < nlines) {
     stream_sync<STORES>();  // line l is in xs_in; every wave is done with the previous line
-    // Hermitian completion of (u + i v) and (w + i theta) into lines 0 and 1
-    for (int m = threadIdx.x; m < N; m += T) {
-      double2 z1 = c2(0.0, 0.0), z2 = c2(0.0, 0.0);
-      if (m == 0) {
-        z1 = c2(xs_in[0].x, xs_in[NH].x);
-        z2 = c2(xs_in[2 * NH].x, xs_in[3 * NH].x);
-      } else if (m < NH) {
-        const double2 a = xs_in[m], b = xs_in[NH + m], c = xs_in[2 * NH + m], d = xs_in[3 * NH + m];
-        z1 = c2(a.x - b.y, a.y + b.x);
-        z2 = c2(c.x - d.y, c.y + d.x);
-      } else if (m > N - NH) {
-        const int q = N - m;
-        const double2 a = xs_in[q], b = xs_in[NH + q], c = xs_in[2 * NH + q], d = xs_in[3 * NH + q];
-        z1 = c2(a.x + b.y, -a.y + b.x);
-        z2 = c2(c.x + d.y, -c.y + d.x);
-      }
-      xs_buf[L4::at(m, 0)] = z1;
-      xs_buf[L4::at(m, 1)] = z2;
-    }
     const double dy = dyr[l / nz32];
-    pass_sync<true>();  // every wave has read xs_in
-    ct_passes_lds<T, L2, N, 2, +1, 1, 16, 16, 6>(xs_buf, xs_tw);
+    xs_first_pass_completion<T>(xs_in, xs_buf);  // also: every wave is done with xs_in
+    ct_passes_lds<T, L2, N, 2, +1, 16, 16, 6>(xs_buf, xs_tw);
     xs_products<T>(xs_buf, dxr, dy, dzr, vmax);
     if (l + G < nlines) issue(l + G);
     pass_sync<true>();

@@ -163,11 +163,14 @@ def test_z_stream_bitwise(lib):
 
 
 @pytest.mark.parametrize("seed", [7, 8])
-def test_x_stream_bitwise(lib, seed):
-    """The streamed x pass (one block per CU, next line copied into LDS, one
-    forward transform of all four product lines) against the one-line-per-block
-    kernel (two forward transforms of two lines): same passes and twiddles,
-    identical products and CFL maximum."""
+def test_x_stream_matches(lib, seed):
+    """The streamed x pass (one block per CU, next line copied into LDS, the
+    completion folded into the first inverse pass, one forward transform of
+    all four product lines, the unpack folded into the last pass) against the
+    one-line-per-block kernel: same passes and twiddles; the complex products
+    inside the transforms may pair into FMAs differently, so the products
+    agree to 1e-15 of each slot's maximum (measured <= 4.5e-16) and the CFL
+    maximum to 1e-15 relative."""
     cfg = CFGS["re1000_flat"]
     S = full_spectrum(cfg, seed=seed)
     out = {}
@@ -180,9 +183,11 @@ def test_x_stream_bitwise(lib, seed):
         out[mode] = ch.nonlinear_host(S, want_vv=False)
         ch.close()
     (P1, vm1), (P0, vm0) = out["1"], out["0"]
-    for slot in range(8):
-        assert np.array_equal(P1[slot], P0[slot]), slot
-    assert vm1 == vm0
+    for slot in (0, 1, 2, 3, 4, 6, 7, 8):  # the eight product slots
+        scale = np.abs(P0[slot]).max()
+        assert scale > 0, slot
+        assert np.abs(P1[slot] - P0[slot]).max() <= 1e-15 * scale, slot
+    assert abs(vm1 - vm0) <= 1e-15 * vm0
 
 
 @pytest.mark.parametrize("name", ["re180_64", "re180_192", "re1000_thin"])
