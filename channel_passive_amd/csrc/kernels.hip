@@ -394,7 +394,10 @@ __device__ __forceinline__ void ct_pass(double2 *x, const double2 *__restrict__ 
       for (int r = 0; r < R; ++r) v[q][r] = x[lds_at<L, NR, 0>(j, b, r)];
     }
   }
-  pass_sync<LDS_ONLY>();
+  // in the last pass (Ns * R == N: NR == Ns, base == j) every butterfly
+  // writes back exactly the elements it read, so no thread can overwrite
+  // another's inputs and the barrier between reads and writes is not needed
+  if constexpr (Ns * R != N) pass_sync<LDS_ONLY>();
 #pragma unroll
   for (int q = 0; q < ROUNDS; ++q) {
     const int t = threadIdx.x + q * T;
