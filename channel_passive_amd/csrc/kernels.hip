@@ -807,8 +807,8 @@ __global__ void __launch_bounds__(zs::T, 3) zfwd_stream_kernel(const double2 *__
 // eight products go through one forward transform of four lines
 // (P1 + i P2, P3 + i P4, P5 + i P7, P8 + i P9), so no product is held in
 // registers across a transform; the completion is folded into the first
-// inverse pass and the unpack into the last forward pass, so a line takes
-// 11 barriers instead of 24.
+// inverse pass, the products into the last inverse pass and the unpack into
+// the last forward pass, so a line takes 9 barriers instead of 24.
 // Same passes, twiddles and arithmetic as xphys_kernel<FftX1536>; the
 // complex products inside the transforms may pair into FMAs differently
 // (they are contracted), so the two agree to a few ulp, not bit for bit.
@@ -829,36 +829,56 @@ __shared__ double2 xs_buf[xs::L4::SIZE];
 __shared__ double2 xs_in[4 * xs::NH];
 __shared__ double2 xs_tw[xs::NTW];
 
-// products at this thread's points of lines 0, 1 (u + i v, w + i theta) into
-// lines 0..3; each point is read and rewritten by one thread
+// Lanes 32..63 of a trade places with lanes 0..31 of b (v_permlane32_swap,
+// a VALU exchange between the two halves of a wave), per 32-bit half.
+__device__ __forceinline__ void swap_halves(double &a, double &b) {
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  b = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+
+// Last inverse pass (radix 6 over Ns = 256; line 0 = u + i v, line 1 =
+// w + i theta) fused with the products.  Butterflies (0, k) and (1, k) sit in
+// lanes l and l + 32 of one wave (k = 32 w + l % 32).  After the DFT, a swap
+// of the wave halves pairs outputs r and r + 3: lanes 0..31 then hold both
+// lines at m = k + 256 r (r < 3), lanes 32..63 at m = k + 256 (r + 3).  Each
+// lane forms the products (P1 + i P2, P3 + i P4, P5 + i P7, P8 + i P9) and
+// CFL velocities of its three points and stores lines 0..3 there.  A wave
+// writes only positions k + 256 r of its own k (read by its own lanes before)
+// and of lines 2, 3, so no barrier is needed inside.  Same butterflies (bfly)
+// and product arithmetic as ct_pass and xphys_kernel.
 template <int T>
-__device__ __forceinline__ void xs_products(double2 *x, double dxr, double dy, double dzr, double &vmax) {
+__device__ __forceinline__ void xs_last_inverse_products(double2 *x, const double2 *tw, double dxr, double dy,
+                                                         double dzr, double &vmax) {
   using L4 = xs::L4;
+  constexpr int N = xs::N, R = 6, Ns = 256, NR = N / R, tstep = N / (Ns * R);
+  static_assert(T == 8 * 64 && NR == Ns && Ns == 8 * 32, "8 waves x 32 butterfly pairs");
+  const int lane = threadIdx.x % 64, b = lane / 32, k = (threadIdx.x / 64) * 32 + lane % 32;
+  double2 v[R];
 #pragma unroll
-  for (int q = 0; q < (xs::N + T - 1) / T; ++q) {
-    const int m = threadIdx.x + q * T;
-    if (m < xs::N) {
-      const double2 a = x[L4::at(m, 0)], c = x[L4::at(m, 1)];
-      const double t1 = a.x, t2 = a.y, t3 = c.x, t4 = c.y;
-      const double vel = (fabs(t1) * dxr) + (fabs(t2) * dy) + (fabs(t3) * dzr);
-      vmax = fmax(vel, vmax);
-      const double bv2 = t2 * t2;
-      x[L4::at(m, 0)] = c2(t1 * t1 - bv2, t1 * t2);
-      x[L4::at(m, 1)] = c2(t3 * t3 - bv2, t2 * t3);
-      x[L4::at(m, 2)] = c2(t1 * t3, t1 * t4);
-      x[L4::at(m, 3)] = c2(t2 * t4, t3 * t4);
-    }
+  for (int r = 0; r < R; ++r) v[r] = x[lds_at<L4, NR, 0>(k, b, r)];
+  bfly<R, +1, Ns, tstep>(v, tw, k);
+  const int r0 = b ? 3 : 0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double2 A = v[r], Bw = v[r + 3];  // lower half: line 0 at r, r + 3; upper half: line 1
+    swap_halves(A.x, Bw.x);
+    swap_halves(A.y, Bw.y);            // now A = line 0, Bw = line 1 at r + r0
+    const int m = k + Ns * (r + r0);
+    const double t1 = A.x, t2 = A.y, t3 = Bw.x, t4 = Bw.y;
+    const double vel = (fabs(t1) * dxr) + (fabs(t2) * dy) + (fabs(t3) * dzr);
+    vmax = fmax(vel, vmax);
+    const double bv2 = t2 * t2;
+    x[L4::at(m, 0)] = c2(t1 * t1 - bv2, t1 * t2);
+    x[L4::at(m, 1)] = c2(t3 * t3 - bv2, t2 * t3);
+    x[L4::at(m, 2)] = c2(t1 * t3, t1 * t4);
+    x[L4::at(m, 3)] = c2(t2 * t4, t3 * t4);
   }
 }
 
-// First inverse pass (radix 16, Ns = 1, lines 0 and 1) fused with the
-// Hermitian completion of (u + i v) and (w + i theta): butterfly j of line b
-// takes elements m = j + 96 r straight from the copied rows (in[f * NH + m]),
-// so the completed lines are never stored to LDS.  Same values as the
-// completion of xphys_kernel (FFTW's c2r ignores Im of the DC bin; modes
-// above nxh are the conjugates of the mirrored ones, the band between them
-// zero) and the same butterflies as ct_pass.  Reads from in, writes to x:
-// one barrier, after the writes.
 template <int T>
 __device__ __forceinline__ void xs_first_pass_completion(const double2 *in, double2 *x) {
   using L2 = xs::L2;
@@ -955,8 +975,8 @@ __global__ void __launch_bounds__(T, 1) xphys_stream_kernel(double2 *__restrict_
     stream_sync<STORES>();  // line l is in xs_in; every wave is done with the previous line
     const double dy = dyr[l / nz32];
     xs_first_pass_completion<T>(xs_in, xs_buf);  // also: every wave is done with xs_in
-    ct_passes_lds<T, L2, N, 2, +1, 16, 16, 6>(xs_buf, xs_tw);
-    xs_products<T>(xs_buf, dxr, dy, dzr, vmax);
+    ct_passes_lds<T, L2, N, 2, +1, 16, 16>(xs_buf, xs_tw);  // the second pass; the third:
+    xs_last_inverse_products<T>(xs_buf, xs_tw, dxr, dy, dzr, vmax);
     if (l + G < nlines) issue(l + G);
     pass_sync<true>();
     ct_passes_lds<T, L4, N, 4, -1, 1, 16, 16>(xs_buf, xs_tw);  // passes 1, 2; the third one:
