@@ -96,8 +96,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is a design "
-              "in DESIGN.md and is not built", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the library runs on one GPU "
+              "and has no multi-GPU (sharded) path", file=sys.stderr)
         sys.exit(2)
 
     import ctypes

@@ -5,7 +5,8 @@
 ! A Fortran driver built on SOURCE_PASSOPB (main.f90, time_march.f90)
 ! uses this module to replace the body of time_march (time_march.f90:
 ! 63-120: y_to_x x4, dft_x_realspace, x_to_y x8, rescale_data, wavespace)
-! by one pb_substep call on device-resident state; see INTEGRATION.md.
+! by one pb_substep call on device-resident state (pb_set_state once,
+! pb_substep per substep, pb_get_state when the driver needs the fields).
 ! Sizes and layouts are those of the reference: U(ny,nx/2,nz,slot) and
 ! RK_STORE(ny,nx/2,nz,3) complex(c_double_complex), y fastest.
 !--------------------------------------------------------------------
