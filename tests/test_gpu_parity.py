@@ -190,6 +190,31 @@ def test_x_stream_matches(lib, seed):
     assert abs(vm1 - vm0) <= 1e-15 * vm0
 
 
+def test_x_stream_fewer_lines_than_blocks(lib, oracle_lib):
+    """nx = 1024 with nz = 4: 6 x 33 = 198 (y, z) lines for one streamed block
+    per CU, so some blocks get no line and only write a zero CFL partial.
+    Products against the oracle (transform tolerance) and the per-line
+    kernel (1e-15), CFL maximum against both."""
+    cfg = dataclasses.replace(preset("re1000"), ny=33, nz=4, name="re1000_few")
+    S = full_spectrum(cfg, seed=9)
+    P_ref, vm_ref = oracle_lib.Oracle(cfg).nonlinear(S)
+    out = {}
+    for mode in ("1", "0"):
+        os.environ["PB_X_STREAM"] = mode
+        try:
+            ch = lib.Channel(cfg)
+        finally:
+            del os.environ["PB_X_STREAM"]
+        out[mode] = ch.nonlinear_host(S)
+        ch.close()
+    (P1, vm1), (P0, vm0) = out["1"], out["0"]
+    for slot in (0, 1, 2, 3, 4, 6, 7, 8):
+        assert rel(P1[slot], P_ref[slot]) <= FFT_TOL, slot
+        assert np.abs(P1[slot] - P0[slot]).max() <= 1e-15 * np.abs(P0[slot]).max(), slot
+    assert abs(vm1 - vm_ref) <= 1e-13 * vm_ref
+    assert abs(vm1 - vm0) <= 1e-15 * vm0
+
+
 @pytest.mark.parametrize("name", ["re180_64", "re180_192", "re1000_thin"])
 def test_pressure_stats(lib, oracle_lib, name):
     """Statistics-time substep: <vv> bitwise, p (get_pressure) against the
