@@ -14,6 +14,7 @@ on a bounded sample of the same workload and extrapolated to one step.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -22,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+CFL_LIMIT = 1.5  # largest pi*dt*velmax timed (RK3's imaginary-axis limit is sqrt(3))
 STAGES = ["oppvd", "zinv", "xphys", "velmax", "zfwd", "wavespace"]
 # stages that are a single kernel each (the wall-normal stage is a chain of
 # sweep kernels on three streams); the roofline object names the single
@@ -44,6 +46,24 @@ def measured_traffic(kernel):
     except (OSError, ValueError):
         pass
     return None
+
+
+def cfl_number(dt, outs):
+    """pi * dt * velmax of a step (velmax is the CFL maximum of its first
+    substep's input, dft_x_realspace.f90:125-127); nan if not finite."""
+    v = outs[0]["velmax"]
+    return math.pi * dt * v if math.isfinite(v) else float("nan")
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def algorithmic_bytes(cfg):
@@ -84,8 +104,8 @@ def copy_calibration(nbytes=1 << 30, reps=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", default="re1000")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -114,18 +134,28 @@ def main():
     del S
     ch.init_stores()
     print(f"[bench] state set, device bytes {ch.device_bytes/1e9:.1f} GB", file=sys.stderr, flush=True)
+    cfl_seen = 0.0
     for w in range(args.warmup):
-        ch.step(cfg.dt)
-    print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
+        out = ch.step(cfg.dt)
+        cfl_seen = max(cfl_seen, cfl_number(cfg.dt, out))
+        if (w + 1) % 10 == 0 or w + 1 == args.warmup:
+            print(f"[bench] warmup {w + 1}/{args.warmup}: pi*dt*velmax = {cfl_number(cfg.dt, out):.3f}",
+                  file=sys.stderr, flush=True)
+    if not cfl_seen <= CFL_LIMIT:
+        raise SystemExit(f"refusing to time an unstable run: pi*dt*velmax = {cfl_seen:.3f} > {CFL_LIMIT} "
+                         f"(RK3 is stable up to sqrt(3) on the imaginary axis)")
     L = capi.load()
     capi._check(L.pb_timing(ch.h, 1))
     ch.sync()
     t0 = time.perf_counter()
     last = None
+    cfl_timed = 0.0
     for k in range(args.steps):
-        last = ch.step(cfg.dt)
+        last = ch.step(cfg.dt)  # raises on a non-finite state (PB_E_NONFINITE)
+        cfl_timed = max(cfl_timed, last[0]["velmax"])
     ch.sync()
     t1 = time.perf_counter()
+    cfl_timed = math.pi * cfg.dt * cfl_timed
     ms = (ctypes.c_double * 6)()
     cnt = (ctypes.c_int64 * 6)()
     capi._check(L.pb_timing_read(ch.h, ms, cnt))
@@ -143,6 +173,13 @@ def main():
     step_bytes = roofline_bytes_per_step(cfg)
     print(f"[bench] {args.steps} steps in {elapsed:.3f} s; stages: " +
           ", ".join(f"{s} {stage[s]['ms_mean']:.3f} ms" for s in STAGES), file=sys.stderr, flush=True)
+    # every mode of the final state enters the Parseval sums of the
+    # statistics profiles: a NaN or Inf anywhere makes them non-finite
+    stats = ch.statistics()
+    if not all(np.all(np.isfinite(v)) for v in stats.values()):
+        raise SystemExit("the state after the timed steps is not finite")
+    if not cfl_timed <= CFL_LIMIT:
+        raise SystemExit(f"the timed run went CFL-unstable: pi*dt*velmax = {cfl_timed:.3f}")
     ch.close()
 
     cpu = None
@@ -151,7 +188,7 @@ def main():
         nplanes, nlines = 96, 131072
         tn, tw, nt = sample_times(cfg, nplanes, nlines)
         t_step = 3 * (tn * cfg.ny / nplanes + tw * (cfg.nxh * cfg.nz) / nlines)
-        cpu = {"value": 1.0 / t_step, "unit": "steps/s", "cores": nt, "kind": "port",
+        cpu = {"value": 1.0 / t_step, "unit": "steps/s", "cores": nt, "cpu_model": cpu_model(), "kind": "port",
                "sample": f"oracle (reference algorithm restated, C++/OpenMP, own FFT) at the full grid: "
                          f"nonlinear stage on {nplanes} of {cfg.ny} y-planes ({tn:.2f} s) and wavespace on "
                          f"{nlines} of {cfg.nxh*cfg.nz} (kx,kz) systems ({tw:.2f} s), extrapolated to 3 substeps"}
@@ -171,7 +208,9 @@ def main():
         "data": "synthetic (seeded turbulent-mean profile + divergence-free perturbation)",
         "config": {"workload": f"channel Re_tau={cfg.re_tau:g} {cfg.nx}x{cfg.ny}x{cfg.nz}, opposition control "
                                f"alpha={cfg.opp_alpha} at y+={cfg.opp_yplus:g}, Pr={cfg.pr}",
-                   "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "dt": cfg.dt},
+                   "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "dt": cfg.dt, "nu": cfg.nu,
+                   "units": "bulk (U_b = 1, h = 1)",
+                   "cfl_number_max": round(max(cfl_seen, cfl_timed), 4)},
         "roofline": {"bound": "hbm", "kernel": KERNEL_OF_STAGE[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": alg[dom],

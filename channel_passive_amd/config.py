@@ -5,8 +5,19 @@ The physical set-up follows the reference's ``input`` records
 (SOURCE_PASSOPB/read_input_file.f90:39-116): box sizes are given as
 ``LMFx, LMFy, LMFz`` with Lx = 2*pi*LMFx, Ly = 2*LMFy, Lz = 2*pi*LMFz, the
 flow rate and the bulk scalar are held at ``bulk_u`` / ``bulk_t``
-(wave00.f90:202, 612).  Friction-velocity units are used (u_tau = 1,
-nu = 1/Re_tau), so dt = 0.005 at Re_tau = 180 is dt+ = 0.9.
+(wave00.f90:202, 612).
+
+Units are bulk units, as the reference's own cold start uses them
+(initial_step.f90:83: laminar 1.5 (1 - y^2), i.e. U_b = 1, half height
+h = 1): U_b = 1 and nu = 1 / Re_b with Re_b = U_b h / nu taken from Dean's
+correlation Re_tau = 0.09 (2 Re_b)^0.88 for the named Re_tau.  At Re_tau =
+180 this gives nu = 1/2819 and dt = 0.005 is dt+ = 0.057; the friction-unit
+reading dt+ = 0.9 puts pi * dt * velmax near 9 on the 64 x 65 x 64 box,
+far past RK3's imaginary-axis limit sqrt(3).
+
+Each preset's fixed dt is chosen from the reference's CFL rule
+dt <= CFL / (pi * velmax) (read_input_file.f90:107, dft_x_realspace.f90:492)
+on its seeded initial state, with pi * dt * velmax between 0.45 and 0.65.
 """
 from __future__ import annotations
 
@@ -35,10 +46,15 @@ class PbConfig(ctypes.Structure):
     ]
 
 
+def dean_bulk_reynolds(re_tau: float) -> float:
+    """Re_b = U_b h / nu from Dean's correlation Re_tau = 0.09 Re_2h^0.88,
+    Re_2h = 2 h U_b / nu."""
+    return 0.5 * (re_tau / 0.09) ** (1.0 / 0.88)
+
+
 def dean_bulk_velocity(re_tau: float) -> float:
-    """U_b+ from Dean's correlation Re_tau = 0.09 Re_b^0.88, Re_b = 2 h U_b / nu."""
-    re_b = (re_tau / 0.09) ** (1.0 / 0.88)
-    return re_b / (2.0 * re_tau)
+    """U_b / u_tau = Re_b / Re_tau (the bulk velocity in friction units)."""
+    return dean_bulk_reynolds(re_tau) / re_tau
 
 
 @dataclasses.dataclass
@@ -54,7 +70,7 @@ class ChannelConfig:
     knot_map: int = PB_KNOTS_TANH
     stretch: float = 2.0
     pr: float = 0.71
-    bulk_u: float | None = None   # default: Dean's correlation
+    bulk_u: float | None = None   # default 1 (bulk units)
     bulk_w: float = 0.0
     bulk_t: float = 1.0
     u_gal: float = 0.0
@@ -67,9 +83,11 @@ class ChannelConfig:
     rk_step: int = 3
     name: str = ""
 
+    nu_value: float | None = None  # default 1 / Re_b (Dean's correlation)
+
     @property
     def nu(self) -> float:
-        return 1.0 / self.re_tau
+        return self.nu_value if self.nu_value is not None else 1.0 / dean_bulk_reynolds(self.re_tau)
 
     @property
     def nxh(self) -> int:
@@ -85,7 +103,7 @@ class ChannelConfig:
         c.labs_k, c.knot_map, c.stretch = self.labs_k, self.knot_map, self.stretch
         c.lmfx, c.lmfy, c.lmfz = self.lmfx, self.lmfy, self.lmfz
         c.nu, c.pr = self.nu, self.pr
-        c.bulk_u = self.bulk_u if self.bulk_u is not None else dean_bulk_velocity(self.re_tau)
+        c.bulk_u = self.bulk_u_value()
         c.bulk_w, c.bulk_t, c.u_gal = self.bulk_w, self.bulk_t, self.u_gal
         c.cfl, c.dt_factor, c.dy_factor = self.cfl, self.dt_factor, self.dy_factor
         c.opp_alpha = self.opp_alpha
@@ -94,7 +112,7 @@ class ChannelConfig:
         return c
 
     def bulk_u_value(self) -> float:
-        return self.bulk_u if self.bulk_u is not None else dean_bulk_velocity(self.re_tau)
+        return self.bulk_u if self.bulk_u is not None else 1.0
 
 
 def preset(name: str) -> ChannelConfig:
@@ -102,10 +120,10 @@ def preset(name: str) -> ChannelConfig:
     if name == "re180_64":      # small, CPU-runnable; opposition control off
         return ChannelConfig(64, 65, 64, re_tau=180.0, lmfx=1.0, lmfz=0.5, dt=0.005, name=name)
     if name == "re180_192":     # production resolution with opposition control at y+ = 10
-        return ChannelConfig(192, 129, 160, re_tau=180.0, lmfx=2.0, lmfz=2.0 / 3.0, dt=0.005,
+        return ChannelConfig(192, 129, 160, re_tau=180.0, lmfx=2.0, lmfz=2.0 / 3.0, dt=0.0025,
                              opp_alpha=1.0, opp_yplus=10.0, name=name)
     if name == "re1000":        # large single-GPU share; opposition control at y+ = 15
-        return ChannelConfig(1024, 385, 768, re_tau=1000.0, lmfx=1.0, lmfz=0.5, dt=2.0e-4,
+        return ChannelConfig(1024, 385, 768, re_tau=1000.0, lmfx=1.0, lmfz=0.5, dt=4.0e-4,
                              opp_alpha=1.0, opp_yplus=15.0, name=name)
     if name == "tiny":          # unit-test size
         return ChannelConfig(16, 33, 12, re_tau=180.0, lmfx=1.0, lmfz=0.5, dt=0.005, name=name)

@@ -132,8 +132,7 @@ struct pb_ctx {
   }
 };
 
-// scal layout: [0] dt, [1] velmax, [2 + 4*(s-1) + q] mean outputs of substep s
-static constexpr int SCAL_N = 2 + 4 * 3;
+using pb::SCAL_N;
 // padded z-pencil fields: 4 state fields in, 8 products out, + v*v at statistics time
 static constexpr int kPadFields = 9;
 
@@ -597,7 +596,8 @@ static int launch_xphys_stream(pb_ctx *c) {
   return PB_OK;
 }
 
-static int launch_nonlinear(pb_ctx *c, int update_dt, int want_vv) {
+// update: velmax_kernel's VM_STORE / VM_DT bits
+static int launch_nonlinear(pb_ctx *c, int update, int want_vv) {
   int r = PB_OK;
   const int ez = pick_ept((size_t)c->nz32 * c->Bz), ex = pick_ept((size_t)c->nx32 * 2);
   if (c->zs_ok && c->zs_inv) r = launch_zinv_stream(c);
@@ -611,7 +611,7 @@ static int launch_nonlinear(pb_ctx *c, int update_dt, int want_vv) {
   {
     StageTimer tm(c, PB_STAGE_VELMAX);
     hipLaunchKernelGGL(pb::velmax_kernel, dim3(1), dim3(pb::VELMAX_THREADS), 0, c->stream, c->vpart, c->vpart_n,
-                       c->scal, update_dt, c->cfg.dt_factor, c->cflx);
+                       c->scal, update, c->cfg.dt_factor, c->cflx);
     HIPCHK(hipGetLastError());
   }
   if (c->zs_ok && c->zs_fwd) r = launch_zfwd_stream(c, want_vv);
@@ -672,7 +672,7 @@ static pb::WsArgs ws_args(pb_ctx *c, int s) {
   A.lmfxr = c->lmfxr;
   A.lmfzr = c->lmfzr;
   A.opp_alpha = c->cfg.opp_alpha;
-  A.mean_out = c->scal + 2 + 4 * (s - 1);
+  A.mean_out = c->scal + pb::SCAL_MEAN0 + 4 * (s - 1);
   return A;
 }
 
@@ -745,7 +745,7 @@ static int substep_front(pb_ctx *c, int s, bool stats) {
   int r;
   if (stats && (r = launch_stats_pre(c))) return r;
   if (c->cfg.opp_alpha != 0.0 && (r = launch_oppvd(c, c->U))) return r;
-  if ((r = launch_nonlinear(c, s == 1, stats))) return r;
+  if ((r = launch_nonlinear(c, s == 1 ? (pb::VM_STORE | pb::VM_DT) : 0, stats))) return r;
   if (stats && (r = launch_pressure(c))) return r;
   return PB_OK;
 }
@@ -756,20 +756,41 @@ static int substep_async(pb_ctx *c, int s) {
   return launch_wavespace(c, s);
 }
 
+// dt into the device scalars, the non-finite flag cleared (through the
+// pinned mirror; the previous read_outputs has synchronised, so the mirror is
+// free)
+static int upload_dt(pb_ctx *c, double dt) {
+  static_assert(pb::SCAL_DT == 0 && pb::SCAL_NONFINITE == 1, "dt and the flag are uploaded together");
+  c->hscal[pb::SCAL_DT] = dt;
+  c->hscal[pb::SCAL_NONFINITE] = 0.0;
+  HIPCHK(hipMemcpyAsync(c->scal, c->hscal, 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return PB_OK;
+}
+
 static int read_outputs(pb_ctx *c, int s0, int s1, pb_substep_out *out, double *dt_inout) {
   HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SCAL_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  *dt_inout = c->hscal[0];
+  const double *h = c->hscal;
+  *dt_inout = h[pb::SCAL_DT];
   if (out)
     for (int s = s0; s <= s1; ++s) {
       pb_substep_out &o = out[s - s0];
-      o.dt = c->hscal[0];
-      o.velmax = c->hscal[1];
-      o.phi_u = c->hscal[2 + 4 * (s - 1) + 0];
-      o.phi_t = c->hscal[2 + 4 * (s - 1) + 1];
-      o.tau_w = c->hscal[2 + 4 * (s - 1) + 2];
-      o.q_w = c->hscal[2 + 4 * (s - 1) + 3];
+      const double *m = h + pb::SCAL_MEAN0 + 4 * (s - 1);
+      o.dt = h[pb::SCAL_DT];
+      o.velmax = h[pb::SCAL_VELMAX];
+      o.phi_u = m[0];
+      o.phi_t = m[1];
+      o.tau_w = m[2];
+      o.q_w = m[3];
     }
+  // a non-finite velocity seen by any CFL reduction of these substeps, or
+  // non-finite mean-line outputs: the state has blown up
+  bool bad = h[pb::SCAL_NONFINITE] != 0.0 || !std::isfinite(h[pb::SCAL_VELMAX]);
+  for (int s = s0; s <= s1; ++s)
+    for (int q = 0; q < 4; ++q) bad = bad || !std::isfinite(h[pb::SCAL_MEAN0 + 4 * (s - 1) + q]);
+  if (bad)
+    return fail(PB_E_NONFINITE, "state not finite in substeps %d..%d (velmax = %g, phi_u = %g)", s0, s1,
+                h[pb::SCAL_VELMAX], h[pb::SCAL_MEAN0 + 4 * (s1 - 1)]);
   return check_dt(*dt_inout);
 }
 
@@ -779,7 +800,7 @@ int pb_substep(pb_ctx *c, int32_t s, double *dt_inout, pb_substep_out *out) {
   if (s < 1 || s > c->cfg.rk_step) return fail(PB_E_ARG, "substep %d out of range", s);
   int r = check_dt(*dt_inout);
   if (r && !(s == 1 && c->cfg.dt_factor != 1.0)) return r;
-  HIPCHK(hipMemcpyAsync(c->scal, dt_inout, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if ((r = upload_dt(c, *dt_inout))) return r;
   if ((r = substep_async(c, s))) return r;
   return read_outputs(c, s, s, out, dt_inout);
 }
@@ -792,7 +813,7 @@ int pb_substep_stats(pb_ctx *c, int32_t s, double *dt_inout, pb_substep_out *out
   if (!c->have_state) return fail(PB_E_STATE, "pb_substep_stats before pb_set_state");
   int r = check_dt(*dt_inout);
   if (r && c->cfg.dt_factor == 1.0) return r;
-  HIPCHK(hipMemcpyAsync(c->scal, dt_inout, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if ((r = upload_dt(c, *dt_inout))) return r;
   if ((r = substep_front(c, s, true))) return r;
   // p leaves through the staging buffer before the solver reuses slot 5
   if (p_out && (r = download_ref(c, p_out, 1, kSlot5, c->U))) return r;
@@ -859,7 +880,7 @@ int pb_step(pb_ctx *c, double *dt_inout, pb_substep_out *out) {
   if (!c->have_state) return fail(PB_E_STATE, "pb_step before pb_set_state");
   int r = check_dt(*dt_inout);
   if (r && c->cfg.dt_factor == 1.0) return r;
-  HIPCHK(hipMemcpyAsync(c->scal, dt_inout, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if ((r = upload_dt(c, *dt_inout))) return r;
   for (int s = 1; s <= c->cfg.rk_step; ++s)
     if ((r = substep_async(c, s))) return r;
   return read_outputs(c, 1, c->cfg.rk_step, out, dt_inout);
@@ -899,14 +920,14 @@ int pb_nonlinear_host(pb_ctx *c, const double *U, double *P, int32_t want_vv, do
   int r = upload_ref(c, U, 4, kStateMap, c->U);
   if (r) return r;
   c->have_state = true;
-  if ((r = launch_nonlinear(c, 0, want_vv != 0))) return r;
+  if ((r = launch_nonlinear(c, pb::VM_STORE, want_vv != 0))) return r;
   static const int map8[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
   // reference slots 0-8 of P (slot 5, v*v, only meaningful with want_vv)
   r = download_ref(c, P, 9, map8, c->U);
   if (r) return r;
   if (velmax) {
     HIPCHK(hipMemcpy(c->hscal, c->scal, SCAL_N * sizeof(double), hipMemcpyDeviceToHost));
-    *velmax = c->hscal[1];
+    *velmax = c->hscal[pb::SCAL_VELMAX];
   }
   return PB_OK;
 }
@@ -927,7 +948,7 @@ int pb_wavespace_host(pb_ctx *c, int32_t s, double dt, double *U, double *rk, co
   if ((r = upload_ref(c, U, 9, map9, c->U))) return r;
   if ((r = upload_ref(c, rk, 3, kRKMap, c->RK))) return r;
   c->have_state = true;
-  HIPCHK(hipMemcpyAsync(c->scal, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if ((r = upload_dt(c, dt))) return r;
   if ((r = launch_wavespace(c, s))) return r;
   if ((r = download_ref(c, U, 9, map9, c->U))) return r;
   if ((r = download_ref(c, rk, 3, kRKMap, c->RK))) return r;

@@ -36,6 +36,9 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 #pragma clang fp contract(fast)
   return c2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+// CFL maximum that keeps a NaN: fmax (IEEE maxNum) would return the other
+// operand, so a non-finite state could reduce to a finite velmax.
+__device__ __forceinline__ double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }
 // multiply by sign*i  (sign = -1 forward, +1 backward)
 __device__ __forceinline__ double2 mul_si(double2 a, int sign) { return sign < 0 ? c2(a.y, -a.x) : c2(-a.y, a.x); }
 
@@ -546,13 +549,13 @@ __global__ void __launch_bounds__(F::T, F::MINW) xphys_kernel(double2 *__restric
       const double2 a = lds[F::at(m, 0, 2)], c = lds[F::at(m, 1, 2)];
       pu[q] = a.x; pv[q] = a.y; pw[q] = c.x; pt[q] = c.y;
       const double vel = (fabs(pu[q]) * dxr) + (fabs(pv[q]) * dy) + (fabs(pw[q]) * dzr);
-      vmax = fmax(vel, vmax);
+      vmax = nan_max(vel, vmax);
     }
   }
   // CFL maximum: across the wave by lane exchanges, one value per wave
   // (velmax_kernel takes the maximum of all of them; max is order-free)
 #pragma unroll
-  for (int d = 32; d > 0; d >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, d, 64));
+  for (int d = 32; d > 0; d >>= 1) vmax = nan_max(vmax, __shfl_xor(vmax, d, 64));
   constexpr int NW = (F::T + 63) / 64;
   if ((threadIdx.x & 63) == 0) vpart[((size_t)y * nz32 + z) * NW + threadIdx.x / 64] = vmax;
   // two pairs of products per round: (P1 + i P2, P3 + i P4), (P5 + i P7, P8 + i P9);
@@ -870,7 +873,7 @@ __device__ __forceinline__ void xs_last_inverse_products(double2 *x, const doubl
     const int m = k + Ns * (r + r0);
     const double t1 = A.x, t2 = A.y, t3 = Bw.x, t4 = Bw.y;
     const double vel = (fabs(t1) * dxr) + (fabs(t2) * dy) + (fabs(t3) * dzr);
-    vmax = fmax(vel, vmax);
+    vmax = nan_max(vel, vmax);
     const double bv2 = t2 * t2;
     x[L4::at(m, 0)] = c2(t1 * t1 - bv2, t1 * t2);
     x[L4::at(m, 1)] = c2(t3 * t3 - bv2, t2 * t3);
@@ -985,20 +988,27 @@ __global__ void __launch_bounds__(T, 1) xphys_stream_kernel(double2 *__restrict_
   }
   // the block's CFL maximum: across each wave by lane exchanges, then the waves
 #pragma unroll
-  for (int d = 32; d > 0; d >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, d, 64));
+  for (int d = 32; d > 0; d >>= 1) vmax = nan_max(vmax, __shfl_xor(vmax, d, 64));
   __shared__ double xs_vm[NW];
   if (lane == 0) xs_vm[wave] = vmax;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < NW; ++w) vmax = fmax(vmax, xs_vm[w]);
+    for (int w = 1; w < NW; ++w) vmax = nan_max(vmax, xs_vm[w]);
     vpart[blockIdx.x] = vmax;
   }
 }
 
 // ------------------------------------------------- CFL maximum and dt update
-// single block: velmax = max(vpart); on substep 1 (update != 0)
-// dt = dt_factor*dt + (1-dt_factor)*cflx/velmax  (dft_x_realspace.f90:489-498)
+// Device scalars of a substep: [0] dt, [1] sticky non-finite flag, [2]
+// velmax (of the last substep 1), [3 + 4 (s-1) + q] the mean-line outputs of
+// substep s (phi_u, phi_t, tau_w, q_w).
+constexpr int SCAL_DT = 0, SCAL_NONFINITE = 1, SCAL_VELMAX = 2, SCAL_MEAN0 = 3;
+constexpr int SCAL_N = SCAL_MEAN0 + 4 * 3;
+// single block: velmax = max(vpart), stored when update has VM_STORE; with
+// VM_DT (substep 1) dt = dt_factor*dt + (1-dt_factor)*cflx/velmax
+// (dft_x_realspace.f90:489-498)
 constexpr int VELMAX_THREADS = 1024;
+constexpr int VM_STORE = 1, VM_DT = 2;  // velmax_kernel's update bits
 __global__ void __launch_bounds__(VELMAX_THREADS) velmax_kernel(const double *__restrict__ vpart, size_t n,
                                                                 double *__restrict__ scal, int update,
                                                                 double dt_factor, double cflx) {
@@ -1007,21 +1017,28 @@ __global__ void __launch_bounds__(VELMAX_THREADS) velmax_kernel(const double *__
   const size_t T = VELMAX_THREADS;
   size_t i = threadIdx.x;
   for (; i + 3 * T < n; i += 4 * T) {
-    v0 = fmax(v0, vpart[i]);
-    v1 = fmax(v1, vpart[i + T]);
-    v2 = fmax(v2, vpart[i + 2 * T]);
-    v3 = fmax(v3, vpart[i + 3 * T]);
+    v0 = nan_max(v0, vpart[i]);
+    v1 = nan_max(v1, vpart[i + T]);
+    v2 = nan_max(v2, vpart[i + 2 * T]);
+    v3 = nan_max(v3, vpart[i + 3 * T]);
   }
-  for (; i < n; i += T) v0 = fmax(v0, vpart[i]);
-  double v = fmax(fmax(v0, v1), fmax(v2, v3));
+  for (; i < n; i += T) v0 = nan_max(v0, vpart[i]);
+  double v = nan_max(nan_max(v0, v1), nan_max(v2, v3));
 #pragma unroll
-  for (int d = 32; d > 0; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
+  for (int d = 32; d > 0; d >>= 1) v = nan_max(v, __shfl_xor(v, d, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < VELMAX_THREADS / 64; ++w) red[0] = fmax(red[0], red[w]);
-    scal[1] = red[0];
-    if (update) scal[0] = dt_factor * scal[0] + (1.0 - dt_factor) * cflx / red[0];
+    for (int w = 1; w < VELMAX_THREADS / 64; ++w) red[0] = nan_max(red[0], red[w]);
+    const double vm = red[0];
+    // velmax is a substep-1 quantity (dft_x_realspace.f90:125-127); the
+    // other substeps only check that the state is finite
+    if (update & VM_STORE) scal[SCAL_VELMAX] = vm;
+    // a non-finite velocity anywhere: raise the sticky flag the host reads
+    // after the substep (the state is left as it is)
+    if (!(vm <= 1.79769313486231570e308)) scal[SCAL_NONFINITE] = 1.0;
+    else if ((update & VM_DT) && dt_factor != 1.0)
+      scal[SCAL_DT] = dt_factor * scal[SCAL_DT] + (1.0 - dt_factor) * cflx / vm;
   }
 }
 

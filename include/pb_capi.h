@@ -38,6 +38,9 @@ extern "C" {
 #define PB_E_HIP        -2   /* HIP runtime error (no device, OOM, launch) */
 #define PB_E_NUMERIC    -3   /* dt out of range (dft_x_realspace.f90:494) */
 #define PB_E_STATE      -4   /* call out of order (e.g. substep before set_state) */
+#define PB_E_NONFINITE  -5   /* the state holds a NaN or Inf (a CFL reduction or a mean-line
+                                output of the substep was not finite); the reference has no
+                                check and would only trip dt_factor's range test one step later */
 
 /* knot maps for the B-spline breakpoints (labs_fill_full.cpp:80-101) */
 #define PB_KNOTS_REFERENCE 0 /* uniform if |stretch| <= 1e-16, else sin map (line 92) */
