@@ -43,6 +43,20 @@
 
 namespace {
 
+// Planted mistakes for the tests that check the oracle's own pins (0 = the
+// restatement as written).  Each nonzero value changes one term the way a
+// plausible transcription slip would; a pin that does not fail under it does
+// not cover that term.
+//   1 hvhg: sign of (kz^2 - kx^2) P5 in h_g
+//   2 hvhg: i (kx^3 + kx kz^2) P2 dropped from h_v
+//   3 solve_g/solve_t store: (1 + a k^2) in place of (1 - a k^2)
+//   4 solve_g/solve_t store: zeta of substep s in place of s + 1
+//   5 solve_v_f: C2 v_h(i) in place of C2 v_h(ny + 1 - i) in v
+//   6 solve_v_f: Poisson operator D2 + k^2 D0
+//   7 Helmholtz operator (1 - b k^2) D0 - b D2
+//   8 wave00: + c d(uv)/dy in place of - c d(uv)/dy in the mean-U right-hand side
+static int g_mutant = 0;
+
 // ----------------------------------------------------------------- complex
 struct cd {
   double re, im;
@@ -565,13 +579,13 @@ static void hvhg(Line &L, cd *u1, cd *u2, cd *u3, cd *u4, cd *u5, cd *u7, cd *u8
   const cd ikz3 = imag_unit_times(kz * kz * kz);
   for (int i = 0; i < n; ++i) {
     const cd dummy = u1[i];
-    u1[i] = add(mulr(sub(u1[i], u3[i]), kxkz), mulr(u5[i], kz2 - kx2));
+    u1[i] = add(mulr(sub(u1[i], u3[i]), kxkz), mulr(u5[i], g_mutant == 1 ? kx2 - kz2 : kz2 - kx2));
     u3[i] = sub(sub(mulr(dummy, -kx2), mulr(u5[i], 2.0 * kxkz)), mulr(u3[i], kz2));
   }
   labs_bs(L.D0lu, n, bw, u3);
   labs_mv(L.D1, n, bw, u3, u5);
   const cd a = add(ikx3, ikxkz2), b = add(ikx2kz, ikz3);
-  for (int i = 0; i < n; ++i) u5[i] = add(add(u5[i], mul(a, u2[i])), mul(b, u4[i]));
+  for (int i = 0; i < n; ++i) u5[i] = add(add(u5[i], g_mutant == 2 ? C(0.0, 0.0) : mul(a, u2[i])), mul(b, u4[i]));
   for (int i = 0; i < n; ++i) {
     const cd t3 = sub(mul(ikx, u4[i]), mul(ikz, u2[i]));
     const cd t4 = add(mul(ikz, u4[i]), mul(ikx, u2[i]));
@@ -599,7 +613,7 @@ static void hvhg(Line &L, cd *u1, cd *u2, cd *u3, cd *u4, cd *u5, cd *u7, cd *u8
 static void helmholtz_lhs(Line &L, double k2, double beta, bool mean) {
   const int n = L.n, W = 2 * L.bw + 1, m = L.bw;
   double *Db = L.Dbuf.data();
-  const double dummy = 1.0 + beta * k2;
+  const double dummy = g_mutant == 7 ? 1.0 - beta * k2 : 1.0 + beta * k2;
   for (int i = 0; i < W; ++i) Db[i] = 0.0;
   Db[m] = mean ? (L.D0[m] - beta * L.D2[m]) : (dummy * L.D0[m] - beta * L.D2[m]);
   for (int j = 1; j < n - 1; ++j)
@@ -636,7 +650,7 @@ static void solve_gt(Line &L, cd *u1, cd *store, cd *buf1, cd *buf2, double kx, 
   buf1[0] = C(0.0, 0.0);
   buf1[n - 1] = C(0.0, 0.0);
   if (s != S) {
-    const double zeta = dt * L.coef(3, s + 1);
+    const double zeta = dt * L.coef(3, g_mutant == 4 ? s : s + 1);
     for (int i = 0; i < n; ++i) store[i] = mulr(u1[i], zeta);
   }
   labs_bs(L.Dbuf.data(), n, bw, buf1);
@@ -644,7 +658,7 @@ static void solve_gt(Line &L, cd *u1, cd *store, cd *buf1, cd *buf2, double kx, 
   labs_mv(L.D2, n, bw, buf1, buf2);
   if (s != S) {
     const double alpha = visc(L.coef(0, s + 1));
-    const double dummy = 1.0 - alpha * k2;
+    const double dummy = g_mutant == 3 ? 1.0 + alpha * k2 : 1.0 - alpha * k2;
     for (int i = 0; i < n; ++i) store[i] = add(add(store[i], mulr(u1[i], dummy)), mulr(buf2[i], alpha));
   } else {
     for (int i = 0; i < n; ++i) store[i] = u1[i];
@@ -697,7 +711,7 @@ static void solve_v_f(Line &L, cd *v, cd *dv, cd *hv, cd *store_phi, cd *buf1, c
   for (int j = 1; j < n - 1; ++j)
     for (int i = 0; i < W; ++i) {
       const size_t q = (size_t)j * W + i;
-      Db[q] = -k2 * L.D0[q] + L.D2[q];
+      Db[q] = (g_mutant == 6 ? k2 : -k2) * L.D0[q] + L.D2[q];
     }
   const size_t e = (size_t)(n - 1) * W;
   for (int i = 0; i < W; ++i) Db[e + i] = 0.0;
@@ -734,7 +748,7 @@ static void solve_v_f(Line &L, cd *v, cd *dv, cd *hv, cd *store_phi, cd *buf1, c
     for (int i = 0; i < n; ++i) store_phi[i] = add(add(store_phi[i], mul(c1, buf3[i])), mul(c2, buf3[n - 1 - i]));
   }
   labs_mv(L.D0, n, bw, buf1, hv);
-  for (int i = 0; i < n; ++i) v[i] = add(add(v[i], mul(c1, hv[i])), mul(c2, hv[n - 1 - i]));
+  for (int i = 0; i < n; ++i) v[i] = add(add(v[i], mul(c1, hv[i])), mul(c2, hv[g_mutant == 5 ? i : n - 1 - i]));
 }
 
 // solve_u_w.f90:18-42 (g -> u, f -> w in place)
@@ -781,12 +795,12 @@ static void wave00(Line &L, cd *u1, cd *u2, cd *u3, cd *u4, cd *u5, cd *u7, cd *
     labs_mv(L.D1, n, bw, u2, u3);
     const double alpha = dt * L.coef(0, s) * nu;
     const double gamma = dt * L.coef(2, s);
-    for (int i = 1; i < n; ++i) buf1[i] = sub(add(st1[i], mulr(buf1[i], alpha)), mulr(u3[i], gamma));
+    for (int i = 1; i < n; ++i) buf1[i] = sub(add(st1[i], mulr(buf1[i], alpha)), mulr(u3[i], g_mutant == 8 ? -gamma : gamma));
   } else {
     labs_bs(L.D0lu, n, bw, u2);
     labs_mv(L.D1, n, bw, u2, u3);
     const double gamma = dt * L.coef(2, s);
-    for (int i = 1; i < n; ++i) buf1[i] = sub(st1[i], mulr(u3[i], gamma));
+    for (int i = 1; i < n; ++i) buf1[i] = sub(st1[i], mulr(u3[i], g_mutant == 8 ? -gamma : gamma));
   }
   buf1[0] = C(0.0, 0.0);
   for (int i = 1; i < n - 1; ++i) buf1[i] = C(buf1[i].re + u_gal, 1.0);
@@ -1282,6 +1296,28 @@ int orc_wavespace(void *h, int s, double dt, double *Ud, double *RKd, const doub
   }
   return 0;
 }
+
+// hvhg alone (hvhg.f90:141-276) on every line except kx = kz = 0, in place on
+// reference-layout products U(ny,nxh,nz,10): afterwards slot 0 holds h_g,
+// slot 4 h_v and slot 6 h_theta (slots 1-3, 7, 8 are scratch).
+int orc_hvhg(void *h, double *Ud) {
+  Ctx *c = static_cast<Ctx *>(h);
+  cd *U = reinterpret_cast<cd *>(Ud);
+  Line L;
+  L.c = c; L.n = c->ny; L.bw = c->bw;
+  L.D0lu = c->d0lu.data(); L.D0 = c->d0.data(); L.D1 = c->d1.data(); L.D2 = c->d2.data();
+  L.dt = 0.0; L.s = 1;
+  for (int j = 0; j < c->nz; ++j)
+    for (int k = 0; k < c->nxh; ++k) {
+      if (c->kx[k] == 0 && c->kz[j] == 0) continue;
+      auto slot = [&](int f) { return U + ix(*c, f, j, k, 0); };
+      hvhg(L, slot(0), slot(1), slot(2), slot(3), slot(4), slot(6), slot(7), slot(8), c->kx[k] * c->lmfxr,
+           c->kz[j] * c->lmfzr);
+    }
+  return 0;
+}
+
+void orc_set_mutant(int m) { g_mutant = m; }
 
 // Bounded CPU-baseline sample at full grid size without full-size arrays:
 // the nonlinear stage on nplanes y planes (plane-local buffers) and the

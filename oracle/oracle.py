@@ -55,6 +55,9 @@ def lib():
         L.orc_band_solve.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_band_matvec.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_fft.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
+        L.orc_hvhg.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_set_mutant.argtypes = [ctypes.c_int]
+        L.orc_set_mutant.restype = None
         _lib = L
     return _lib
 
@@ -121,6 +124,23 @@ def fft(x: np.ndarray, sign: int) -> np.ndarray:
     return out
 
 
+class mutant:
+    """Context manager planting mistake number m in the oracle's per-line
+    formulas (the list is in oracle.cpp); for tests that show a pin fails on
+    a wrong oracle."""
+
+    def __init__(self, m: int):
+        self.m = m
+
+    def __enter__(self):
+        lib().orc_set_mutant(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_mutant(0)
+        return False
+
+
 # ----------------------------------------------------------------- solver
 class Oracle:
     """One channel configuration on the CPU, reference layout throughout."""
@@ -172,6 +192,13 @@ class Oracle:
                             None if vd is None else vd.ctypes.data_as(_dp), 0, cfg.nxh * cfg.nz, _p(mean))
         return mean
 
+    def hvhg(self, P: np.ndarray) -> np.ndarray:
+        """hvhg alone on products P (10 slots, reference layout) of every
+        line except the mean: returns (h_g, h_v, h_theta), each (nz, nxh, ny)."""
+        Q = np.ascontiguousarray(P, dtype=np.complex128).copy()
+        lib().orc_hvhg(self.h, Q.ctypes.data_as(_dp))
+        return Q[0].copy(), Q[4].copy(), Q[6].copy()
+
     def pressure(self, S: np.ndarray, P: np.ndarray) -> np.ndarray:
         """get_pressure on the products P (10 slots, v*v in slot 5) of the
         state S: slot 5 of P becomes p; returns <vv>(y)."""
@@ -215,7 +242,13 @@ class Oracle:
         return dict(zip(["dt", "velmax", "phi_u", "phi_t", "tau_w", "q_w"], out))
 
     def step(self, S, RK, dt):
-        outs = [self.substep(S, RK, s, dt) for s in (1, 2, 3)]
+        """Three substeps; substep 1 may replace dt (CFL rule), and substeps
+        2 and 3 run with that dt, as the reference's global dt does
+        (dft_x_realspace.f90:492, time_march.f90:36-128)."""
+        outs = []
+        for s in (1, 2, 3):
+            outs.append(self.substep(S, RK, s, dt))
+            dt = outs[-1]["dt"]
         return outs
 
 

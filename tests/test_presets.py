@@ -111,3 +111,17 @@ def test_re180_64_hundred_steps(oracle_lib):
     assert np.abs(S[1]).max() <= 3.0 * v0
     c = oracle_lib.band_solve(orc.d0lu, orc.bw, S[0, 0, 0].copy())
     assert abs(0.5 * orc.integ @ c.real - 1.0) <= 1e-12
+
+
+def test_oracle_adaptive_dt_threads_through(oracle_lib):
+    """dt_factor = 0: substep 1 sets dt = CFL / (pi * velmax)
+    (dft_x_realspace.f90:489-498) and substeps 2 and 3 use that dt."""
+    import dataclasses
+    cfg = dataclasses.replace(preset("tiny"), dt_factor=0.0, cfl=0.8)
+    orc = oracle_lib.Oracle(cfg)
+    S = make_state(cfg, seed=9)
+    RK = orc.init_stores(S)
+    outs = orc.step(S, RK, 1.0)
+    assert outs[0]["dt"] == outs[1]["dt"] == outs[2]["dt"]
+    assert abs(outs[0]["dt"] - 0.8 / math.pi / outs[0]["velmax"]) <= 1e-15 * outs[0]["dt"]
+    assert np.all(np.isfinite(S))
