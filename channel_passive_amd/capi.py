@@ -60,6 +60,13 @@ SIGNATURES = {
     "pb_wavespace_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp,
                                          ctypes.POINTER(PbSubstepOut)]),
     "pb_nonlinear_host": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_int32, _dp]),
+    "pb_y_to_x_host": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "pb_realspace_host": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_int32, _dp]),
+    "pb_x_to_y_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, _dp, ctypes.c_double]),
+    "pb_pressure_host": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "pb_labs_lu": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp]),
+    "pb_labs_bs": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
+    "pb_labs_mat_mul": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32, _dp, _dp]),
 }
 
 _lib = None
@@ -204,9 +211,77 @@ class Channel:
                                         ctypes.byref(vm)))
         return P, vm.value
 
+    def pshape(self, nslot):
+        """x-pencil padded layout Upad(nx/2, 3nz/2, ny, slot) as numpy
+        (slot, ny, 3nz/2, nx/2)."""
+        c = self.cfg
+        return (nslot, c.ny, 3 * c.nz // 2, c.nxh)
+
+    def y_to_x_host(self, S: np.ndarray) -> np.ndarray:
+        """y_to_x of u, v, w, theta: (4, nz, nx/2, ny) -> (4, ny, 3nz/2, nx/2)."""
+        assert S.shape == self.shape(4)
+        out = np.zeros(self.pshape(4), np.complex128)
+        _check(load().pb_y_to_x_host(self.h, _ptr(np.ascontiguousarray(S)), _ptr(out)))
+        return out
+
+    def realspace_host(self, Upad: np.ndarray, want_vv: bool = False):
+        """dft_x_realspace: 4 padded fields -> 10 padded slots (reference
+        order) and velmax."""
+        assert Upad.shape == self.pshape(4)
+        out = np.zeros(self.pshape(10), np.complex128)
+        vm = ctypes.c_double(0)
+        _check(load().pb_realspace_host(self.h, _ptr(np.ascontiguousarray(Upad)), _ptr(out), int(want_vv),
+                                        ctypes.byref(vm)))
+        return out, vm.value
+
+    def x_to_y_host(self, Upad: np.ndarray, scale: float = 1.0) -> np.ndarray:
+        """x_to_y (times scale) of nfield padded fields -> (nfield, nz, nx/2, ny)."""
+        nf = Upad.shape[0]
+        assert Upad.shape == self.pshape(nf)
+        out = np.zeros(self.shape(nf), np.complex128)
+        _check(load().pb_x_to_y_host(self.h, nf, _ptr(np.ascontiguousarray(Upad)), _ptr(out), scale))
+        return out
+
+    def pressure_host(self, S: np.ndarray, P: np.ndarray):
+        """get_pressure from the state S (4 slots) and its products P (10
+        slots, v*v in slot 5): (p (nz, nx/2, ny), <vv> (ny,))."""
+        p = np.zeros(self.shape(1)[1:], np.complex128)
+        vv = np.zeros(self.cfg.ny)
+        _check(load().pb_pressure_host(self.h, _ptr(np.ascontiguousarray(S)), _ptr(np.ascontiguousarray(P)),
+                                       _ptr(p), vv.ctypes.data_as(_dp)))
+        return p, vv
+
     def wavespace_host(self, s: int, dt: float, P: np.ndarray, RK: np.ndarray, v_old: np.ndarray | None = None):
         o = PbSubstepOut()
         _check(load().pb_wavespace_host(self.h, s, dt, _ptr(P), _ptr(RK),
                                         None if v_old is None else _ptr(np.ascontiguousarray(v_old)),
                                         ctypes.byref(o)))
         return o.as_dict()
+
+
+# ---- LABS band operations (module_labs.f90:462, 499, 570) on the device
+def labs_lu(A: np.ndarray, bw: int) -> np.ndarray:
+    """LABS_lu of packed bands A (..., n, 2bw+1) (one or a stack)."""
+    A = np.ascontiguousarray(A, dtype=np.float64).copy()
+    n = A.shape[-2]
+    nmat = int(np.prod(A.shape[:-2])) if A.ndim > 2 else 1
+    _check(load().pb_labs_lu(n, bw, nmat, A.ctypes.data_as(_dp)))
+    return A
+
+
+def labs_bs(LU: np.ndarray, bw: int, b: np.ndarray) -> np.ndarray:
+    """LABS_bs: LU^-1 b for columns b (..., n) complex."""
+    x = np.ascontiguousarray(b, dtype=np.complex128).copy()
+    n = LU.shape[0]
+    _check(load().pb_labs_bs(n, bw, np.ascontiguousarray(LU).ctypes.data_as(_dp), x.size // n, _ptr(x)))
+    return x
+
+
+def labs_mat_mul(A: np.ndarray, bw: int, x: np.ndarray) -> np.ndarray:
+    """LABS_mat_mul: A x for columns x (..., n) complex."""
+    xx = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(xx)
+    n = A.shape[0]
+    _check(load().pb_labs_mat_mul(n, bw, np.ascontiguousarray(A).ctypes.data_as(_dp), xx.size // n, _ptr(xx),
+                                  _ptr(out)))
+    return out

@@ -542,11 +542,11 @@ static int launch_xphys_t(pb_ctx *c, int want_vv) {
   return PB_OK;
 }
 template <class F>
-static int launch_zfwd_t(pb_ctx *c, int want_vv) {
+static int launch_zfwd_t(pb_ctx *c, int want_vv, double scale) {
   const size_t ldsz = (size_t)c->nz32 * c->Bz * sizeof(double2);
   StageTimer tm(c, PB_STAGE_ZFWD);
   hipLaunchKernelGGL(pb::zfwd_kernel<F>, dim3(c->nxh / c->Bz, c->ny, want_vv ? 9 : 8), dim3(F::T), ldsz, c->stream,
-                     c->pad, c->U, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield, c->rescale);
+                     c->pad, c->U, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield, scale);
   HIPCHK(hipGetLastError());
   return PB_OK;
 }
@@ -575,10 +575,10 @@ static int launch_zinv_stream(pb_ctx *c) {
   HIPCHK(hipGetLastError());
   return PB_OK;
 }
-static int launch_zfwd_stream(pb_ctx *c, int want_vv) {
+static int launch_zfwd_stream(pb_ctx *c, int want_vv, double scale) {
   StageTimer tm(c, PB_STAGE_ZFWD);
   hipLaunchKernelGGL(pb::zfwd_stream_kernel, dim3(c->ncu), dim3(pb::zs::T), 0, c->stream, c->pad, c->U, c->twz,
-                     c->nxh, c->nz, c->ny, c->field, c->pfield, c->rescale, want_vv ? 9 : 8);
+                     c->nxh, c->nz, c->ny, c->field, c->pfield, scale, want_vv ? 9 : 8);
   HIPCHK(hipGetLastError());
   return PB_OK;
 }
@@ -596,28 +596,50 @@ static int launch_xphys_stream(pb_ctx *c) {
   return PB_OK;
 }
 
-// update: velmax_kernel's VM_STORE / VM_DT bits
-static int launch_nonlinear(pb_ctx *c, int update, int want_vv) {
+// y_to_x of the four state fields (device slots 0, 1, 2, 6 -> pad fields 0-3)
+static int launch_zinv_any(pb_ctx *c) {
   int r = PB_OK;
-  const int ez = pick_ept((size_t)c->nz32 * c->Bz), ex = pick_ept((size_t)c->nx32 * 2);
+  const int ez = pick_ept((size_t)c->nz32 * c->Bz);
   if (c->zs_ok && c->zs_inv) r = launch_zinv_stream(c);
   else if (ct_z(c)) r = launch_zinv_t<pb::FftZ1152>(c);
   else PB_EPT_DISPATCH(ez, r = launch_zinv_t<F>(c));
-  if (r) return r;
+  return r;
+}
+// dft_x_realspace: pad fields 0-3 -> products in pad fields 0-7 (8 = v*v),
+// per-block CFL partials in vpart
+static int launch_xphys_any(pb_ctx *c, int want_vv) {
+  int r = PB_OK;
+  const int ex = pick_ept((size_t)c->nx32 * 2);
   if (c->xs_ok && !want_vv) r = launch_xphys_stream(c);
   else if (ct_x(c)) r = launch_xphys_t<pb::FftX1536>(c, want_vv);
   else PB_EPT_DISPATCH(ex, r = launch_xphys_t<F>(c, want_vv));
-  if (r) return r;
-  {
-    StageTimer tm(c, PB_STAGE_VELMAX);
-    hipLaunchKernelGGL(pb::velmax_kernel, dim3(1), dim3(pb::VELMAX_THREADS), 0, c->stream, c->vpart, c->vpart_n,
-                       c->scal, update, c->cfg.dt_factor, c->cflx);
-    HIPCHK(hipGetLastError());
-  }
-  if (c->zs_ok && c->zs_fwd) r = launch_zfwd_stream(c, want_vv);
-  else if (ct_z(c)) r = launch_zfwd_t<pb::FftZ1152>(c, want_vv);
-  else PB_EPT_DISPATCH(ez, r = launch_zfwd_t<F>(c, want_vv));
   return r;
+}
+// x_to_y (+ rescale_data when scale = 1/(nx32 nz32)): pad fields 0-7 (8)
+// -> device slots 0, 1, 2, 3, 4, 6, 7, 8 (5)
+static int launch_zfwd_any(pb_ctx *c, int want_vv, double scale) {
+  int r = PB_OK;
+  const int ez = pick_ept((size_t)c->nz32 * c->Bz);
+  if (c->zs_ok && c->zs_fwd) r = launch_zfwd_stream(c, want_vv, scale);
+  else if (ct_z(c)) r = launch_zfwd_t<pb::FftZ1152>(c, want_vv, scale);
+  else PB_EPT_DISPATCH(ez, r = launch_zfwd_t<F>(c, want_vv, scale));
+  return r;
+}
+// update: velmax_kernel's VM_STORE / VM_DT bits
+static int launch_velmax(pb_ctx *c, int update) {
+  StageTimer tm(c, PB_STAGE_VELMAX);
+  hipLaunchKernelGGL(pb::velmax_kernel, dim3(1), dim3(pb::VELMAX_THREADS), 0, c->stream, c->vpart, c->vpart_n,
+                     c->scal, update, c->cfg.dt_factor, c->cflx);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+
+static int launch_nonlinear(pb_ctx *c, int update, int want_vv) {
+  int r;
+  if ((r = launch_zinv_any(c))) return r;
+  if ((r = launch_xphys_any(c, want_vv))) return r;
+  if ((r = launch_velmax(c, update))) return r;
+  return launch_zfwd_any(c, want_vv, c->rescale);
 }
 
 extern "C++" {
@@ -954,6 +976,130 @@ int pb_wavespace_host(pb_ctx *c, int32_t s, double dt, double *U, double *rk, co
   if ((r = download_ref(c, rk, 3, kRKMap, c->RK))) return r;
   double d = dt;
   return read_outputs(c, s, s, out, &d);
+}
+
+int pb_y_to_x_host(pb_ctx *c, const double *U, double *Upad) {
+  if (!c || !U || !Upad) return fail(PB_E_ARG, "null argument");
+  int r;
+  if ((r = upload_ref(c, U, 4, kStateMap, c->U))) return r;
+  c->have_state = true;
+  if ((r = launch_zinv_any(c))) return r;
+  HIPCHK(hipMemcpyAsync(Upad, c->pad, 4 * c->pfield * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PB_OK;
+}
+
+int pb_realspace_host(pb_ctx *c, const double *Upad_in, double *Upad_out, int32_t want_vv, double *velmax) {
+  if (!c || !Upad_in || !Upad_out) return fail(PB_E_ARG, "null argument");
+  const size_t pf = c->pfield;
+  HIPCHK(hipMemcpyAsync(c->pad, Upad_in, 4 * pf * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  int r;
+  if ((r = launch_xphys_any(c, want_vv != 0))) return r;
+  if ((r = launch_velmax(c, pb::VM_STORE))) return r;
+  // pad fields P1..P5, P7..P9 (, v*v) -> reference slots 0-4, 6-8 (, 5)
+  double2 *out = reinterpret_cast<double2 *>(Upad_out);
+  static const int dst[9] = {0, 1, 2, 3, 4, 6, 7, 8, 5};
+  for (int g = 0; g < (want_vv ? 9 : 8); ++g)
+    HIPCHK(hipMemcpyAsync(out + dst[g] * pf, c->pad + g * pf, pf * sizeof(double2), hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SCAL_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!want_vv) std::memset(out + 5 * pf, 0, pf * sizeof(double2));
+  // slot 10 of the reference keeps the substep's v (dft_x_realspace.f90:123)
+  std::memcpy(out + 9 * pf, reinterpret_cast<const double2 *>(Upad_in) + pf, pf * sizeof(double2));
+  if (velmax) *velmax = c->hscal[pb::SCAL_VELMAX];
+  return PB_OK;
+}
+
+int pb_x_to_y_host(pb_ctx *c, int32_t nfield, const double *Upad, double *U, double scale) {
+  if (!c || !Upad || !U) return fail(PB_E_ARG, "null argument");
+  if (nfield < 1 || nfield > 9) return fail(PB_E_ARG, "pb_x_to_y_host: nfield %d outside 1..9", nfield);
+  const size_t pf = c->pfield;
+  HIPCHK(hipMemcpyAsync(c->pad, Upad, nfield * pf * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  if (nfield < 8) HIPCHK(hipMemsetAsync(c->pad + nfield * pf, 0, (8 - nfield) * pf * sizeof(double2), c->stream));
+  int r;
+  if ((r = launch_zfwd_any(c, nfield == 9, scale))) return r;
+  static const int slot_of_pad[9] = {0, 1, 2, 3, 4, 6, 7, 8, 5};
+  if ((r = download_ref(c, U, nfield, slot_of_pad, c->U))) return r;
+  c->have_state = false;  // the device slots now hold transformed fields, not a state
+  return PB_OK;
+}
+
+int pb_pressure_host(pb_ctx *c, const double *U, const double *P, double *p_out, double *vv_out) {
+  if (!c || !U || !P || !p_out) return fail(PB_E_ARG, "null argument");
+  int r;
+  if ((r = upload_ref(c, U, 4, kStateMap, c->U))) return r;
+  if ((r = launch_stats_pre(c))) return r;
+  static const int map9[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+  if ((r = upload_ref(c, P, 9, map9, c->U))) return r;
+  if ((r = launch_pressure(c))) return r;
+  if ((r = download_ref(c, p_out, 1, kSlot5, c->U))) return r;
+  if (vv_out) {
+    HIPCHK(hipMemcpyAsync(vv_out, c->dvv, c->ny * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  c->have_state = false;
+  return PB_OK;
+}
+
+// ---- LABS band operations on host arrays (device kernels, one thread per
+// matrix / column; the null stream, synchronous)
+static int labs_check(int32_t n, int32_t bw, int32_t count) {
+  if (bw < 1 || n < 2 * bw + 1 || count < 1) return fail(PB_E_ARG, "labs: n %d, bw %d, count %d", n, bw, count);
+  return PB_OK;
+}
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+int pb_labs_lu(int32_t n, int32_t bw, int32_t nmat, double *A) {
+  int r;
+  if (!A) return fail(PB_E_ARG, "null argument");
+  if ((r = labs_check(n, bw, nmat))) return r;
+  const size_t bytes = (size_t)nmat * n * (2 * bw + 1) * sizeof(double);
+  DevBuf d;
+  HIPCHK(hipMalloc(&d.p, bytes));
+  HIPCHK(hipMemcpy(d.p, A, bytes, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(pb::labs_lu_kernel, dim3(blocks_for(nmat, 64)), dim3(64), 0, 0, n, bw, nmat, (double *)d.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(A, d.p, bytes, hipMemcpyDeviceToHost));
+  return PB_OK;
+}
+
+int pb_labs_bs(int32_t n, int32_t bw, const double *A_lu, int32_t nrhs, double *b) {
+  int r;
+  if (!A_lu || !b) return fail(PB_E_ARG, "null argument");
+  if ((r = labs_check(n, bw, nrhs))) return r;
+  const size_t ab = (size_t)n * (2 * bw + 1) * sizeof(double), bb = (size_t)nrhs * n * sizeof(double2);
+  DevBuf da, db;
+  HIPCHK(hipMalloc(&da.p, ab));
+  HIPCHK(hipMalloc(&db.p, bb));
+  HIPCHK(hipMemcpy(da.p, A_lu, ab, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(db.p, b, bb, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(pb::labs_bs_kernel, dim3(blocks_for(nrhs, 64)), dim3(64), 0, 0, n, bw, (const double *)da.p,
+                     nrhs, (pb::cplx *)db.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(b, db.p, bb, hipMemcpyDeviceToHost));
+  return PB_OK;
+}
+
+int pb_labs_mat_mul(int32_t n, int32_t bw, const double *A, int32_t nrhs, const double *x, double *b) {
+  int r;
+  if (!A || !x || !b) return fail(PB_E_ARG, "null argument");
+  if ((r = labs_check(n, bw, nrhs))) return r;
+  const size_t ab = (size_t)n * (2 * bw + 1) * sizeof(double), vb = (size_t)nrhs * n * sizeof(double2);
+  DevBuf da, dx, db;
+  HIPCHK(hipMalloc(&da.p, ab));
+  HIPCHK(hipMalloc(&dx.p, vb));
+  HIPCHK(hipMalloc(&db.p, vb));
+  HIPCHK(hipMemcpy(da.p, A, ab, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dx.p, x, vb, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(pb::labs_mv_kernel, dim3(blocks_for(nrhs, 64)), dim3(64), 0, 0, n, bw, (const double *)da.p,
+                     nrhs, (const pb::cplx *)dx.p, (pb::cplx *)db.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(b, db.p, vb, hipMemcpyDeviceToHost));
+  return PB_OK;
 }
 
 }  // extern "C"

@@ -1378,4 +1378,28 @@ __global__ void __launch_bounds__(64) init_stores_kernel(LineParams P, cplx *U, 
   for (int i = 0; i < n; ++i) r3[i] = th[i];
 }
 
+// ------------------------------------------- LABS band operations, batched
+// Packed bands (2bw+1, n) one after the other, Fortran order (slot fastest);
+// one thread per matrix / right-hand side.  Same functions as the solver.
+__global__ void __launch_bounds__(64) labs_lu_kernel(int n, int bw, int nmat, double *A) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmat) return;
+  const BandGeom g{n, bw};
+  band_lu(g, OwnBand{A + (size_t)m * n * g.W(), 1, g.W()});
+}
+// b: nrhs complex columns of n values, one after the other
+__global__ void __launch_bounds__(64) labs_bs_kernel(int n, int bw, const double *LU, int nrhs, cplx *b) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nrhs) return;
+  const BandGeom g{n, bw};
+  band_bs(g, SharedBand{LU, g.W()}, SV{b + (size_t)m * n, 1});
+}
+__global__ void __launch_bounds__(64) labs_mv_kernel(int n, int bw, const double *A, int nrhs, const cplx *x,
+                                                     cplx *b) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nrhs) return;
+  const BandGeom g{n, bw};
+  band_mv(g, SharedBand{A, g.W()}, SV{const_cast<cplx *>(x) + (size_t)m * n, 1}, SV{b + (size_t)m * n, 1});
+}
+
 }  // namespace pb

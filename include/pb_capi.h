@@ -195,6 +195,52 @@ int pb_wavespace_host(pb_ctx *ctx, int32_t s, double dt, double *U, double *rk,
 int pb_nonlinear_host(pb_ctx *ctx, const double *U, double *P, int32_t want_vv,
                       double *velmax);
 
+/* The stages of pb_nonlinear_host one by one, in the layouts the reference
+ * hands between them.  The x-pencil layout of the padded fields is
+ * Upad(nx/2, 3nz/2, ny, slot) complex, kx fastest (module_ppdfftw.f90:
+ * 1019-1146 with one rank): the z direction is in physical space on the
+ * 3/2-padded grid, x still spectral.
+ *
+ * y_to_x (y_to_x.f90:21-49, module_ppdfftw.f90:830-861) of u, v, w, theta:
+ * U(ny,nx/2,nz,4) -> Upad(nx/2,3nz/2,ny,4): kz padded to 3nz/2 (Nyquist slot
+ * and gap zero), unnormalised backward DFT in z. */
+int pb_y_to_x_host(pb_ctx *ctx, const double *U, double *Upad);
+/* dft_x_realspace (dft_x_realspace.f90:24-213): Upad_in(nx/2,3nz/2,ny,4)
+ * holds u, v, w, theta after y_to_x; Upad_out(nx/2,3nz/2,ny,10) receives the
+ * product spectra in the reference slots 1-5, 7-9 (0-based 0-4, 6-8), v*v in
+ * slot 6 (0-based 5) when want_vv (else zero) and the input v in slot 10
+ * (0-based 9), as the reference keeps it for vold.  velmax receives the CFL
+ * maximum max(|u| dxr + |v| dyr + |w| dzr) (lines 125-127). */
+int pb_realspace_host(pb_ctx *ctx, const double *Upad_in, double *Upad_out, int32_t want_vv,
+                      double *velmax);
+/* x_to_y (x_to_y.f90:21-45, module_ppdfftw.f90:794-823) of nfield (1..9)
+ * padded fields Upad(nx/2,3nz/2,ny,nfield) -> U(ny,nx/2,nz,nfield):
+ * unnormalised forward DFT in z, kz truncated to nz with the Nyquist slot
+ * zero, every value times scale (1 = x_to_y alone; 1/(3nx/2 * 3nz/2) =
+ * x_to_y followed by rescale_data, rescale_data.f90:10-39). */
+int pb_x_to_y_host(pb_ctx *ctx, int32_t nfield, const double *Upad, double *U, double scale);
+/* get_pressure (get_pressure.f90:1-138) with the statistics-time inputs of
+ * get_statistics_1: U(ny,nx/2,nz,4) the substep's input state (<vv>(y) and
+ * the PPE wall values nu D2(wall row) LU(D0)^-1 v, get_statistics.f90:
+ * 689-716, 982-1017) and P(ny,nx/2,nz,10) its rescaled product spectra with
+ * v*v in slot 6 (0-based 5).  p_out (ny,nx/2,nz) receives p; vv_out (ny
+ * doubles) <vv> when not NULL. */
+int pb_pressure_host(pb_ctx *ctx, const double *U, const double *P, double *p_out, double *vv_out);
+
+/* ---- LABS band operations (module_labs.f90) on host arrays -------------- */
+/* Packed bands A(2bw+1, n) in the reference's near-boundary format
+ * (LABS_full_to_band, module_labs.f90:301-328), one after the other; vectors
+ * are n complex values, one after the other.  They run the device solver's
+ * band kernels (one thread per matrix or vector) on the current device and
+ * synchronise; count = 1 is the reference's single call.
+ * LABS_lu (module_labs.f90:462; no-pivot Crout, fixed_bandec*): in place. */
+int pb_labs_lu(int32_t n, int32_t bw, int32_t nmat, double *A);
+/* LABS_bs (module_labs.f90:499; fixed_banbks*): b <- A^-1 b for nrhs columns
+ * with the factor A_lu of pb_labs_lu. */
+int pb_labs_bs(int32_t n, int32_t bw, const double *A_lu, int32_t nrhs, double *b);
+/* LABS_mat_mul (module_labs.f90:570): b = A x for nrhs columns. */
+int pb_labs_mat_mul(int32_t n, int32_t bw, const double *A, int32_t nrhs, const double *x, double *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,7 +15,7 @@ module poongback_gpu
   implicit none
 
   integer(c_int), parameter :: PB_OK = 0, PB_E_ARG = -1, PB_E_HIP = -2, &
-       PB_E_NUMERIC = -3, PB_E_STATE = -4
+       PB_E_NUMERIC = -3, PB_E_STATE = -4, PB_E_NONFINITE = -5
   integer(c_int32_t), parameter :: PB_KNOTS_REFERENCE = 0, PB_KNOTS_TANH = 1
 
   type, bind(C) :: pb_config
@@ -51,16 +51,43 @@ module poongback_gpu
        import :: c_ptr
      end function pb_last_error
 
+     integer(c_int) function pb_get_info(ctx, bw, device_bytes) bind(C, name="pb_get_info")
+       import :: c_int, c_int32_t, c_int64_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: bw
+       integer(c_int64_t), intent(out) :: device_bytes
+     end function pb_get_info
+
+     ! pointers may be c_null_ptr (pb_get_labs skips them)
+     integer(c_int) function pb_get_labs(ctx, y, d0, d1, d2, d0_lu, integ) bind(C, name="pb_get_labs")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, y, d0, d1, d2, d0_lu, integ
+     end function pb_get_labs
+
+     ! U(ny,nx/2,nz,4) = u, v, w, theta; rk = RK_STORE or c_null_ptr
+     integer(c_int) function pb_set_state(ctx, U, rk) bind(C, name="pb_set_state")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, U, rk
+     end function pb_set_state
+
+     integer(c_int) function pb_get_state(ctx, U, rk) bind(C, name="pb_get_state")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, U, rk
+     end function pb_get_state
+
+     ! rk: c_loc(RK_STORE) or c_null_ptr (then pb_init_stores forms the stores)
      integer(c_int) function pb_set_state_slots(ctx, u, v, w, theta, rk) bind(C, name="pb_set_state_slots")
        import :: c_int, c_ptr, c_double_complex
        type(c_ptr), value :: ctx
-       complex(c_double_complex), intent(in) :: u(*), v(*), w(*), theta(*), rk(*)
+       complex(c_double_complex), intent(in) :: u(*), v(*), w(*), theta(*)
+       type(c_ptr), value :: rk
      end function pb_set_state_slots
 
      integer(c_int) function pb_get_state_slots(ctx, u, v, w, theta, rk) bind(C, name="pb_get_state_slots")
        import :: c_int, c_ptr, c_double_complex
        type(c_ptr), value :: ctx
-       complex(c_double_complex), intent(out) :: u(*), v(*), w(*), theta(*), rk(*)
+       complex(c_double_complex), intent(out) :: u(*), v(*), w(*), theta(*)
+       type(c_ptr), value :: rk
      end function pb_get_state_slots
 
      integer(c_int) function pb_init_stores(ctx) bind(C, name="pb_init_stores")
@@ -76,15 +103,36 @@ module poongback_gpu
        type(pb_substep_out), intent(out) :: out
      end function pb_substep
 
+     ! p: c_loc of a (ny,nx/2,nz) complex array, vv: c_loc of ny doubles; either may be c_null_ptr
      integer(c_int) function pb_substep_stats(ctx, s, dt, out, p, vv) bind(C, name="pb_substep_stats")
-       import :: c_int, c_int32_t, c_double, c_ptr, c_double_complex, pb_substep_out
+       import :: c_int, c_int32_t, c_double, c_ptr, pb_substep_out
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: s
        real(c_double), intent(inout) :: dt
        type(pb_substep_out), intent(out) :: out
-       complex(c_double_complex), intent(out) :: p(*)
-       real(c_double), intent(out) :: vv(*)
+       type(c_ptr), value :: p, vv
      end function pb_substep_stats
+
+     ! the three substeps of main.f90:134-156; out(rk_step)
+     integer(c_int) function pb_step(ctx, dt, out) bind(C, name="pb_step")
+       import :: c_int, c_double, c_ptr, pb_substep_out
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: dt
+       type(pb_substep_out), intent(out) :: out(*)
+     end function pb_step
+
+     integer(c_int) function pb_timing(ctx, enable) bind(C, name="pb_timing")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: enable
+     end function pb_timing
+
+     integer(c_int) function pb_timing_read(ctx, ms, launches) bind(C, name="pb_timing_read")
+       import :: c_int, c_double, c_int64_t, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: ms(*)
+       integer(c_int64_t), intent(out) :: launches(*)
+     end function pb_timing_read
 
      integer(c_int) function pb_statistics(ctx, out) bind(C, name="pb_statistics")
        import :: c_int, c_ptr, c_double
@@ -103,13 +151,14 @@ module poongback_gpu
        type(c_ptr), value :: ctx
      end function pb_sync
 
+     ! v_old: c_loc(U(:,10)) with opposition control, else c_null_ptr
      integer(c_int) function pb_wavespace_host(ctx, s, dt, U, rk, v_old, out) bind(C, name="pb_wavespace_host")
        import :: c_int, c_int32_t, c_double, c_ptr, c_double_complex, pb_substep_out
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: s
        real(c_double), value :: dt
        complex(c_double_complex), intent(inout) :: U(*), rk(*)
-       complex(c_double_complex), intent(in) :: v_old(*)
+       type(c_ptr), value :: v_old
        type(pb_substep_out), intent(out) :: out
      end function pb_wavespace_host
 
@@ -121,5 +170,65 @@ module poongback_gpu
        integer(c_int32_t), value :: want_vv
        real(c_double), intent(out) :: velmax
      end function pb_nonlinear_host
+
+     ! y_to_x of u, v, w, theta: U(ny,nx/2,nz,4) -> Upad(nx/2,3nz/2,ny,4)
+     integer(c_int) function pb_y_to_x_host(ctx, U, Upad) bind(C, name="pb_y_to_x_host")
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: U(*)
+       complex(c_double_complex), intent(out) :: Upad(*)
+     end function pb_y_to_x_host
+
+     ! dft_x_realspace: Upad_in(nx/2,3nz/2,ny,4) -> Upad_out(nx/2,3nz/2,ny,10)
+     integer(c_int) function pb_realspace_host(ctx, Upad_in, Upad_out, want_vv, velmax) &
+          bind(C, name="pb_realspace_host")
+       import :: c_int, c_int32_t, c_double, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: Upad_in(*)
+       complex(c_double_complex), intent(out) :: Upad_out(*)
+       integer(c_int32_t), value :: want_vv
+       real(c_double), intent(out) :: velmax
+     end function pb_realspace_host
+
+     ! x_to_y of nfield fields, times scale (1/(nx32*nz32) adds rescale_data)
+     integer(c_int) function pb_x_to_y_host(ctx, nfield, Upad, U, scale) bind(C, name="pb_x_to_y_host")
+       import :: c_int, c_int32_t, c_double, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nfield
+       complex(c_double_complex), intent(in) :: Upad(*)
+       complex(c_double_complex), intent(out) :: U(*)
+       real(c_double), value :: scale
+     end function pb_x_to_y_host
+
+     ! get_pressure: state U(ny,nx/2,nz,4), products P(ny,nx/2,nz,10) with v*v in slot 6
+     integer(c_int) function pb_pressure_host(ctx, U, P, p_out, vv_out) bind(C, name="pb_pressure_host")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: U(*), P(*)
+       complex(c_double_complex), intent(out) :: p_out(*)
+       real(c_double), intent(out) :: vv_out(*)
+     end function pb_pressure_host
+
+     ! LABS_lu / LABS_bs / LABS_mat_mul (module_labs.f90:462, 499, 570) on the device
+     integer(c_int) function pb_labs_lu(n, bw, nmat, A) bind(C, name="pb_labs_lu")
+       import :: c_int, c_int32_t, c_double
+       integer(c_int32_t), value :: n, bw, nmat
+       real(c_double), intent(inout) :: A(*)
+     end function pb_labs_lu
+
+     integer(c_int) function pb_labs_bs(n, bw, A_lu, nrhs, b) bind(C, name="pb_labs_bs")
+       import :: c_int, c_int32_t, c_double, c_double_complex
+       integer(c_int32_t), value :: n, bw, nrhs
+       real(c_double), intent(in) :: A_lu(*)
+       complex(c_double_complex), intent(inout) :: b(*)
+     end function pb_labs_bs
+
+     integer(c_int) function pb_labs_mat_mul(n, bw, A, nrhs, x, b) bind(C, name="pb_labs_mat_mul")
+       import :: c_int, c_int32_t, c_double, c_double_complex
+       integer(c_int32_t), value :: n, bw, nrhs
+       real(c_double), intent(in) :: A(*)
+       complex(c_double_complex), intent(in) :: x(*)
+       complex(c_double_complex), intent(out) :: b(*)
+     end function pb_labs_mat_mul
   end interface
 end module poongback_gpu

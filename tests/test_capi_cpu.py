@@ -52,3 +52,17 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(capi, "_lib", None)
     with pytest.raises(capi.PbError):
         capi.load()
+
+
+def test_fortran_module_binds_every_entry_point():
+    """include/poongback_gpu.f90 has an interface with bind(C, name=...) for
+    every function of the header, with the same argument count."""
+    f90 = open(os.path.join(ROOT, "include", "poongback_gpu.f90")).read()
+    bound = dict(re.findall(r"function\s+(pb_\w+)\s*\(([^)]*)\)\s*(?:&\s*)?bind\(C", f90))
+    text = open(HEADER).read()
+    for name in declared():
+        assert name in bound, name
+        m = re.search(r"\b" + name + r"\s*\(([^)]*)\)\s*;", text)
+        n_c = 0 if m.group(1).strip() in ("", "void") else len(m.group(1).split(","))
+        n_f = 0 if not bound[name].strip() else len(bound[name].split(","))
+        assert n_c == n_f, (name, n_c, n_f)
