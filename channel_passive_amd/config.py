@@ -27,6 +27,10 @@ import math
 
 PB_KNOTS_REFERENCE = 0
 PB_KNOTS_TANH = 1
+# scalar forcing: the reference's held-bulk uniform source, or uniform wall
+# heat flux with the mixed-type condition (source (nu/Pr) u / U_b)
+PB_SCALAR_SOURCE = 0
+PB_SCALAR_MIXED = 1
 
 
 class PbConfig(ctypes.Structure):
@@ -42,7 +46,7 @@ class PbConfig(ctypes.Structure):
         ("u_gal", ctypes.c_double),
         ("cfl", ctypes.c_double), ("dt_factor", ctypes.c_double), ("dy_factor", ctypes.c_double),
         ("opp_alpha", ctypes.c_double), ("opp_ydist", ctypes.c_double),
-        ("rk_step", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("rk_step", ctypes.c_int32), ("scalar_bc", ctypes.c_int32),
     ]
 
 
@@ -81,6 +85,7 @@ class ChannelConfig:
     opp_yplus: float = 10.0
     dt: float = 0.005
     rk_step: int = 3
+    scalar_bc: int = PB_SCALAR_SOURCE
     name: str = ""
 
     nu_value: float | None = None  # default 1 / Re_b (Dean's correlation)
@@ -109,6 +114,7 @@ class ChannelConfig:
         c.opp_alpha = self.opp_alpha
         c.opp_ydist = self.opp_yplus / self.re_tau
         c.rk_step = self.rk_step
+        c.scalar_bc = self.scalar_bc
         return c
 
     def bulk_u_value(self) -> float:
@@ -119,9 +125,9 @@ def preset(name: str) -> ChannelConfig:
     """The workloads of the benchmark description, by name."""
     if name == "re180_64":      # small, CPU-runnable; opposition control off
         return ChannelConfig(64, 65, 64, re_tau=180.0, lmfx=1.0, lmfz=0.5, dt=0.005, name=name)
-    if name == "re180_192":     # production resolution with opposition control at y+ = 10
+    if name == "re180_192":     # production resolution, opposition control at y+ = 10, mixed-type scalar BC
         return ChannelConfig(192, 129, 160, re_tau=180.0, lmfx=2.0, lmfz=2.0 / 3.0, dt=0.0025,
-                             opp_alpha=1.0, opp_yplus=10.0, name=name)
+                             opp_alpha=1.0, opp_yplus=10.0, scalar_bc=PB_SCALAR_MIXED, name=name)
     if name == "re1000":        # large single-GPU share; opposition control at y+ = 15
         return ChannelConfig(1024, 385, 768, re_tau=1000.0, lmfx=1.0, lmfz=0.5, dt=4.0e-4,
                              opp_alpha=1.0, opp_yplus=15.0, name=name)

@@ -78,6 +78,7 @@ struct pb_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
   // device memory
   double2 *U = nullptr, *RK = nullptr, *pad = nullptr, *buf = nullptr, *vd = nullptr;
+  double2 *usrc = nullptr;  // mixed-type scalar condition: u of the substep's input state
   double *dbuf = nullptr, *dbuf2 = nullptr, *rbuf = nullptr, *vpart = nullptr, *scal = nullptr;
   double2 *ppe_bc = nullptr;  // statistics time: pressure wall values [2][kz][kx]
   double *dcenter = nullptr;  // B_k(0): centre-line values of the basis (get_statistics.f90:566-570)
@@ -144,7 +145,7 @@ static int ctx_free(pb_ctx *c) {
   if (!c) return 0;
   void *ptrs[] = {c->U, c->RK, c->pad, c->buf, c->vd, c->dbuf, c->dbuf2, c->rbuf, c->vpart, c->scal, c->dD0, c->dD1, c->dD2,
                   c->dD0LU, c->dinteg, c->ddyr, c->drows, c->twz, c->twx, c->slotmap, c->mean_db,
-                  c->ppe_bc, c->dvv, c->dcenter, c->dstat};
+                  c->ppe_bc, c->dvv, c->dcenter, c->dstat, c->usrc};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -213,6 +214,10 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
     else if (i == c->ny - 1) d = std::fabs(y[c->ny - 1] - y[c->ny - 2]);
     else d = std::min(std::fabs(y[i] - y[i - 1]), std::fabs(y[i] - y[i + 1]));
     c->dyr[i] = 1.0 / (cfg->dy_factor * d);
+  }
+  if (cfg->scalar_bc != PB_SCALAR_SOURCE && cfg->scalar_bc != PB_SCALAR_MIXED) {
+    ctx_free(c);
+    return fail(PB_E_ARG, "scalar_bc %d is neither PB_SCALAR_SOURCE nor PB_SCALAR_MIXED", cfg->scalar_bc);
   }
   if (cfg->opp_alpha != 0.0) {
     // detection rows r with v(y_d) = r . v (= B(y_d) . LU(D0)^-1 v)
@@ -303,6 +308,7 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   rc = rc ? rc : dalloc((void **)&c->pad, kPadFields * c->pfield * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->buf, 2 * c->field * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->vd, 2 * c->plane * sizeof(double2));
+  if (cfg->scalar_bc == PB_SCALAR_MIXED) rc = rc ? rc : dalloc((void **)&c->usrc, c->field * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->dbuf, (size_t)c->W * c->field * sizeof(double));
   rc = rc ? rc : dalloc((void **)&c->dbuf2, (size_t)c->W * c->field * sizeof(double));
   rc = rc ? rc : dalloc((void **)&c->rbuf, c->field * sizeof(double));
@@ -686,6 +692,8 @@ static pb::WsArgs ws_args(pb_ctx *c, int s) {
   A.dbuf = c->dbuf;
   A.rbuf = c->rbuf;
   A.vd = c->cfg.opp_alpha != 0.0 ? (const pb::cplx *)c->vd : nullptr;
+  A.usrc = c->cfg.scalar_bc == PB_SCALAR_MIXED ? (const pb::cplx *)c->usrc : nullptr;
+  A.src_scale = c->cfg.nu / c->cfg.pr / c->cfg.bulk_u;
   A.plane = c->plane;
   A.field = c->field;
   A.nxh = c->nxh;
@@ -767,6 +775,10 @@ static int substep_front(pb_ctx *c, int s, bool stats) {
   int r;
   if (stats && (r = launch_stats_pre(c))) return r;
   if (c->cfg.opp_alpha != 0.0 && (r = launch_oppvd(c, c->U))) return r;
+  // mixed-type scalar condition: the source needs u after the transforms
+  // have overwritten slot 0 with P1
+  if (c->usrc)
+    HIPCHK(hipMemcpyAsync(c->usrc, c->U, c->field * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
   if ((r = launch_nonlinear(c, s == 1 ? (pb::VM_STORE | pb::VM_DT) : 0, stats))) return r;
   if (stats && (r = launch_pressure(c))) return r;
   return PB_OK;
@@ -966,6 +978,11 @@ int pb_wavespace_host(pb_ctx *c, int32_t s, double dt, double *U, double *rk, co
     static const int map1[1] = {1};
     if ((r = upload_ref(c, v_old, 1, map1, c->U))) return r;
     if ((r = launch_oppvd(c, c->U))) return r;
+  }
+  if (c->usrc) {
+    // mixed-type scalar condition: u of the substep's input state in slot 9
+    static const int map0[1] = {0};
+    if ((r = upload_ref(c, U + 9 * 2 * c->field, 1, map0, c->usrc))) return r;
   }
   if ((r = upload_ref(c, U, 9, map9, c->U))) return r;
   if ((r = upload_ref(c, rk, 3, kRKMap, c->RK))) return r;

@@ -1083,8 +1083,9 @@ __global__ void __launch_bounds__(MEAN_THREADS) mean_kernel(LineParams P, WsArgs
     src[1] = gU(1); src[3] = gR(0);
   } else if (f == 1) {  // u2 u3 u4 u5 st2 buf1
     src[2] = gU(3); src[4] = gR(1);
-  } else {              // u7 u8 u9 st3 buf1
+  } else {              // u7 u8 u9 st3 buf1 (usrc)
     src[1] = gU(7); src[3] = gR(2);
+    if (A.usrc) src[5] = const_cast<cplx *>(A.usrc);
   }
   for (int q = 0; q < MEAN_COLS; ++q)
     if (src[q])
@@ -1102,7 +1103,8 @@ __global__ void __launch_bounds__(MEAN_THREADS) mean_kernel(LineParams P, WsArgs
     } else if (f == 1) {
       mean_w(Q, Db, true, L(0), L(1), L(2), L(3), L(4), L(5));
     } else {
-      mean_t(Q, Db, L(0), L(1), L(2), L(3), L(4), o4);
+      const SV us = L(5);
+      mean_t(Q, Db, L(0), L(1), L(2), L(3), L(4), o4, A.usrc ? &us : nullptr, A.src_scale);
       if (A.mean_out) { A.mean_out[1] = o4[1]; A.mean_out[3] = o4[3]; }
     }
   }

@@ -396,7 +396,14 @@ PB_HD void mean_w(const LineParams &P, const OwnBand &Db, bool factor, SV u2, SV
 
 // Mean scalar (wave00.f90:473-667): u7 <- Theta, st3 updated, u8/u9 left as
 // scratch; out4[1] = phi_t, out4[3] = q_w (last substep).
-PB_HD void mean_t(const LineParams &P, const OwnBand &Db, SV u7, SV u8, SV u9, SV st3, SV buf1, double *out4) {
+// usrc (mixed-type scalar condition): the mean u of the substep's input
+// state; the source ks U enters as -h, u9 = d(v theta)/dy - ks U, and no bulk
+// is held (phi_t = 0).  Null: the reference's held-bulk uniform source.
+PB_HD void mixed_source(const SV &u9, const SV &usrc, int n, double ks) {
+  for (int i = 0; i < n; ++i) u9[i] = csub(u9[i], cscale(usrc[i], ks));
+}
+PB_HD void mean_t(const LineParams &P, const OwnBand &Db, SV u7, SV u8, SV u9, SV st3, SV buf1, double *out4,
+                  const SV *usrc, double ks) {
   const int n = P.g.n, s = P.s, S = P.S;
   const double dt = P.dt, nu = P.nu, pr = P.pr;
   helmholtz_lu(P, Db, 0.0, dt * P.coef(1, s) * nu / pr, true);
@@ -406,19 +413,21 @@ PB_HD void mean_t(const LineParams &P, const OwnBand &Db, SV u7, SV u8, SV u9, S
     band_mv(P.g, P.D2, u7, buf1);
     band_bs(P.g, P.D0LU, u8);
     band_mv(P.g, P.D1, u8, u9);
+    if (usrc) mixed_source(u9, *usrc, n, ks);
     const double alpha = dt * P.coef(0, s) * nu / pr, gamma = dt * P.coef(2, s);
     for (int i = 1; i < n; ++i) buf1[i] = csub(cadd(st3[i], cscale(buf1[i], alpha)), cscale(u9[i], gamma));
   } else {
     band_bs(P.g, P.D0LU, u8);
     band_mv(P.g, P.D1, u8, u9);
+    if (usrc) mixed_source(u9, *usrc, n, ks);
     const double gamma = dt * P.coef(2, s);
     for (int i = 1; i < n; ++i) buf1[i] = csub(st3[i], cscale(u9[i], gamma));
   }
   buf1[0] = mk(0.0, 0.0);
-  for (int i = 1; i < n - 1; ++i) buf1[i] = mk(buf1[i].re, 1.0);
+  for (int i = 1; i < n - 1; ++i) buf1[i] = mk(buf1[i].re, usrc ? 0.0 : 1.0);
   buf1[n - 1] = mk(0.0, 0.0);
   band_bs(P.g, Db, buf1);
-  const double phi = calc_phi(P.bulk_t, integrate(P, buf1));
+  const double phi = usrc ? 0.0 : calc_phi(P.bulk_t, integrate(P, buf1));
   for (int i = 0; i < n; ++i) buf1[i] = mk(buf1[i].re + phi * buf1[i].im, 0.0);
   out4[1] = phi;
   if (s == S) out4[3] = 0.5 * nu / pr * wall_d1(P, buf1);
@@ -434,10 +443,10 @@ PB_HD void mean_t(const LineParams &P, const OwnBand &Db, SV u7, SV u8, SV u9, S
 
 // wave00.f90:41-685 with calculate_phi (704-714); out4 = phi_u, phi_t, tau_w, q_w
 PB_HD void wave00(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3, SV u4, SV u5, SV u7, SV u8, SV u9,
-                  SV st1, SV st2, SV st3, SV buf1, double *out4) {
+                  SV st1, SV st2, SV st3, SV buf1, double *out4, const SV *usrc, double ks) {
   mean_u(P, Db, u1, u2, u3, st1, buf1, out4);
   mean_w(P, Db, false, u2, u3, u4, u5, st2, buf1);
-  mean_t(P, Db, u7, u8, u9, st3, buf1, out4);
+  mean_t(P, Db, u7, u8, u9, st3, buf1, out4, usrc, ks);
 }
 
 // ------------------------------------------------- statistics-time pressure
@@ -548,6 +557,11 @@ struct WsArgs {
   double *dbuf;   // W*ny doubles per system
   double *rbuf;   // one real scratch column per system (fused sweeps only)
   const cplx *vd; // detection-plane v (2 planes) or nullptr
+  // mixed-type scalar condition: u of the substep's input state (one
+  // field, device layout) and the source factor (nu/pr)/bulk_u; nullptr for
+  // the reference's held-bulk uniform source
+  const cplx *usrc;
+  double src_scale;
   size_t plane;   // nz*nxh
   size_t field;   // ny*plane
   int nxh, nz, nh;
@@ -561,10 +575,11 @@ PB_HD void wavespace_line(const LineParams &P, const WsArgs &A, size_t sys) {
   auto col = [&](cplx *base, int slot) { return SV{base + (size_t)slot * A.field + sys, A.plane}; };
   OwnBand Db{A.dbuf + sys, A.plane, P.g.W()};
   SV b1 = col(A.buf, 0), b2 = col(A.buf, 1);
+  const SV us{const_cast<cplx *>(A.usrc) + sys, A.plane};
   if (kxi == 0 && kzi == 0) {
     double o4[4] = {0, 0, 0, 0};
     wave00(P, Db, col(A.U, 0), col(A.U, 1), col(A.U, 2), col(A.U, 3), col(A.U, 4), col(A.U, 6), col(A.U, 7),
-           col(A.U, 8), col(A.RK, 0), col(A.RK, 1), col(A.RK, 2), b1, o4);
+           col(A.U, 8), col(A.RK, 0), col(A.RK, 1), col(A.RK, 2), b1, o4, A.usrc ? &us : nullptr, A.src_scale);
     if (A.mean_out)
       for (int q = 0; q < 4; ++q) A.mean_out[q] = o4[q];
     return;
@@ -580,6 +595,10 @@ PB_HD void wavespace_line(const LineParams &P, const WsArgs &A, size_t sys) {
   }
   solve_v_f(P, Db, col(A.U, 1), col(A.U, 2), col(A.U, 4), col(A.RK, 1), col(A.U, 3), b1, b2, kx, kz, vlo, vhi);
   solve_u_w(P.g.n, col(A.U, 0), col(A.U, 2), kx, kz);
+  if (A.usrc) {  // mixed-type scalar condition: h_theta += ks u
+    const SV ht = col(A.U, 6);
+    for (int i = 0; i < P.g.n; ++i) ht[i] = cadd(ht[i], cscale(us[i], A.src_scale));
+  }
   solve_gt(P, Db, col(A.U, 6), col(A.RK, 2), b1, b2, kx, kz, true);
 }
 

@@ -547,7 +547,8 @@ PB_HD void sweep_Bt(const LineParams &P, const WsArgs &A, const SysCtx &c) {
     c8.push(bwd_row<BW, MID>(g, r, ur, c8, s7[r]));
     if (S1) c3.push(bwd_row<BW, MID>(g, r, ur, c3, y3[r]));
     for_completed_rows<BW>(n, r, [&](int m) {
-      const cplx ht = cneg(cadd(s8[m], mv_row<BW>(g, P.D1, m, r, c8)));
+      cplx ht = cneg(cadd(s8[m], mv_row<BW>(g, P.D1, m, r, c8)));
+      if (A.usrc) ht = cadd(ht, cscale(A.usrc[c.sys + (size_t)m * A.plane], A.src_scale));  // mixed-type source
       const cplx d2s = S1 ? mv_row<BW>(g, P.D2, m, r, c3) : mk(0.0, 0.0);
       s7[m] = rhs_row(P, q, m, n, r3, d2s, ht);
     });

@@ -25,7 +25,7 @@ import math
 
 import numpy as np
 
-from .config import ChannelConfig, physical_wavenumbers
+from .config import PB_SCALAR_MIXED, ChannelConfig, physical_wavenumbers
 
 _MASK = (1 << 64) - 1
 
@@ -96,6 +96,17 @@ def mean_profile(cfg: ChannelConfig, y: np.ndarray, turbulent: bool) -> np.ndarr
     return u * (ub / bulk)
 
 
+def scalar_mean_profile(cfg: ChannelConfig, y: np.ndarray) -> np.ndarray:
+    """Laminar scalar profile of the configuration's forcing: 1.5 Theta_b
+    (1 - y^2) under the held-bulk uniform source, (5 - 6 y^2 + y^4) / 8 under
+    uniform wall heat flux with the mixed-type condition (the steady solution
+    of (nu/Pr) Theta'' = -(nu/Pr) U/U_b on U = 1.5 U_b (1 - y^2))."""
+    if cfg.scalar_bc == PB_SCALAR_MIXED:
+        y2 = y * y
+        return (5.0 - 6.0 * y2 + y2 * y2) / 8.0
+    return 1.5 * cfg.bulk_t * (1.0 - y * y)
+
+
 def make_state(cfg: ChannelConfig, seed: int = 0xC0FFEE, amp: float = 0.1, kmax: int = 8,
                turbulent_mean: bool | None = None) -> np.ndarray:
     """State S(4, nz, nx/2, ny): u, v, w, theta collocation values."""
@@ -105,7 +116,7 @@ def make_state(cfg: ChannelConfig, seed: int = 0xC0FFEE, amp: float = 0.1, kmax:
     S = np.zeros((4, cfg.nz, cfg.nxh, cfg.ny), dtype=np.complex128)
     U = mean_profile(cfg, y, turbulent_mean)
     S[0, 0, 0, :] = U
-    S[3, 0, 0, :] = 1.5 * cfg.bulk_t * (1.0 - y * y)
+    S[3, 0, 0, :] = scalar_mean_profile(cfg, y)
     _, _, kxp, kzp = physical_wavenumbers(cfg)
     nh = cfg.nz // 2
     rng = SplitMix64(seed)

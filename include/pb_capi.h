@@ -46,6 +46,22 @@ extern "C" {
 #define PB_KNOTS_REFERENCE 0 /* uniform if |stretch| <= 1e-16, else sin map (line 92) */
 #define PB_KNOTS_TANH      1 /* 0.5*Ly*tanh(s*xi)/tanh(s)  (line 89, commented out in the reference) */
 
+/* Scalar forcing and wall condition.
+ * PB_SCALAR_SOURCE: the reference's: theta = 0 at both walls and a
+ *   y-uniform source whose strength phi_t is chosen every substep to hold the
+ *   bulk scalar at bulk_t (wave00.f90:473-667, calculate_phi 704-714).
+ * PB_SCALAR_MIXED: uniform wall heat flux with the mixed-type condition
+ *   (Kasagi, Tomita & Kuroda 1992): the wall temperature rises linearly in x,
+ *   the scalar theta = T - T_w obeys the transport equation with the source
+ *   (nu/Pr) u / bulk_u on every mode (u the streamwise velocity of the
+ *   substep's input state, treated explicitly like the convective terms)
+ *   and theta = 0 at both walls; the wall flux per wall is nu/Pr in steady
+ *   state, the bulk scalar is free (phi_t = 0).  Laminar Poiseuille flow has
+ *   theta = (5 - 6 y^2 + y^4)/8 and a mixed-mean Nusselt number 70/17
+ *   (based on 2h).  Not in the reference (its scalar has the SOURCE form). */
+#define PB_SCALAR_SOURCE 0
+#define PB_SCALAR_MIXED  1
+
 /* Physical and numerical parameters.  Mirrors the `input` file records that
  * reach the time step (read_input_file.f90:39-116) plus the opposition
  * control the reference leaves commented out (solve_v_f.f90:253-257). */
@@ -62,7 +78,7 @@ typedef struct pb_config {
   double  opp_alpha;         /* opposition-control gain alpha; 0 = reference behaviour (wall v = 0) */
   double  opp_ydist;         /* detection plane distance from each wall in outer units (y+_d / Re_tau) */
   int32_t rk_step;           /* number of RK substeps per step; 3 (read_input_file.f90:121) */
-  int32_t reserved;
+  int32_t scalar_bc;         /* PB_SCALAR_* */
 } pb_config;
 
 typedef struct pb_ctx pb_ctx;
@@ -180,7 +196,9 @@ int pb_timing_read(pb_ctx *ctx, double *ms, int64_t *launches);
 
 /* wavespace(U, RK_STORE, ...)  (wavespace.f90:22): U(ny,nx/2,nz,10) holds the
  * rescaled product spectra in slots 1-5, 7-9 (0-based 0-4, 6-8) and slot 10
- * (0-based 9) is vold; on return slots 0,1,2,6 hold u, v, w, theta.
+ * (0-based 9) is vold, which the library reads only under PB_SCALAR_MIXED,
+ * where it must hold u of the substep's input state (the scalar source);
+ * on return slots 0,1,2,6 hold u, v, w, theta.
  * rk(ny,nx/2,nz,3) is updated in place.  v_old (ny,nx/2,nz) is the v of
  * the substep's input state (used by opposition control only; may be NULL
  * when opp_alpha == 0). */

@@ -17,6 +17,7 @@ module poongback_gpu
   integer(c_int), parameter :: PB_OK = 0, PB_E_ARG = -1, PB_E_HIP = -2, &
        PB_E_NUMERIC = -3, PB_E_STATE = -4, PB_E_NONFINITE = -5
   integer(c_int32_t), parameter :: PB_KNOTS_REFERENCE = 0, PB_KNOTS_TANH = 1
+  integer(c_int32_t), parameter :: PB_SCALAR_SOURCE = 0, PB_SCALAR_MIXED = 1
 
   type, bind(C) :: pb_config
      integer(c_int64_t) :: nx, ny, nz
@@ -28,7 +29,7 @@ module poongback_gpu
      real(c_double) :: u_gal
      real(c_double) :: cfl, dt_factor, dy_factor
      real(c_double) :: opp_alpha, opp_ydist
-     integer(c_int32_t) :: rk_step, reserved
+     integer(c_int32_t) :: rk_step, scalar_bc
   end type pb_config
 
   type, bind(C) :: pb_substep_out

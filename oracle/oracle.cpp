@@ -55,6 +55,7 @@ namespace {
 //   6 solve_v_f: Poisson operator D2 + k^2 D0
 //   7 Helmholtz operator (1 - b k^2) D0 - b D2
 //   8 wave00: + c d(uv)/dy in place of - c d(uv)/dy in the mean-U right-hand side
+//   9 mixed-type scalar source with the wrong sign, -(nu/pr) u / bulk_u
 static int g_mutant = 0;
 
 // ----------------------------------------------------------------- complex
@@ -766,15 +767,24 @@ static void solve_u_w(int n, cd *g, cd *f, double kx, double kz) {
   }
 }
 
+// Mixed-type scalar condition, mean line: -h_Theta = d(v theta)/dy - ks U
+// (u9 holds d(v theta)/dy), ks = (nu/pr) / bulk_u.
+static void mixed_source(cd *u9, const cd *usrc, int n, double ks) {
+  for (int i = 0; i < n; ++i) u9[i] = sub(u9[i], mulr(usrc[i], ks));
+}
+
 struct MeanOut {
   double phi_u = 0, phi_t = 0, tau_w = 0, q_w = 0;
   bool wall = false;
 };
 
 // wave00.f90:41-685 and calculate_phi (704-714)
+// usrc: with the mixed-type scalar condition, the mean u of the substep's
+// input state (source (nu/pr) U / bulk_u in the Theta equation, no held
+// bulk); nullptr for the reference's held-bulk uniform source.
 static void wave00(Line &L, cd *u1, cd *u2, cd *u3, cd *u4, cd *u5, cd *u7, cd *u8, cd *u9,
                    cd *st1, cd *st2, cd *st3, cd *buf1, double nu, double pr, double bulk_u,
-                   double bulk_w, double bulk_t, double u_gal, MeanOut &mo) {
+                   double bulk_w, double bulk_t, double u_gal, const cd *usrc, MeanOut &mo) {
   const int n = L.n, bw = L.bw, s = L.s, S = L.c->cfg.rk_step, W = 2 * bw + 1, m = bw;
   const double dt = L.dt;
   const std::vector<double> &integ = L.c->integ;
@@ -865,20 +875,22 @@ static void wave00(Line &L, cd *u1, cd *u2, cd *u3, cd *u4, cd *u5, cd *u7, cd *
     labs_mv(L.D2, n, bw, u7, buf1);
     labs_bs(L.D0lu, n, bw, u8);
     labs_mv(L.D1, n, bw, u8, u9);
+    if (usrc) mixed_source(u9, usrc, n, (g_mutant == 9 ? -1.0 : 1.0) * nu / pr / bulk_u);
     const double alpha = dt * L.coef(0, s) * nu / pr;
     const double gamma = dt * L.coef(2, s);
     for (int i = 1; i < n; ++i) buf1[i] = sub(add(st3[i], mulr(buf1[i], alpha)), mulr(u9[i], gamma));
   } else {
     labs_bs(L.D0lu, n, bw, u8);
     labs_mv(L.D1, n, bw, u8, u9);
+    if (usrc) mixed_source(u9, usrc, n, (g_mutant == 9 ? -1.0 : 1.0) * nu / pr / bulk_u);
     const double gamma = dt * L.coef(2, s);
     for (int i = 1; i < n; ++i) buf1[i] = sub(st3[i], mulr(u9[i], gamma));
   }
   buf1[0] = C(0.0, 0.0);
-  for (int i = 1; i < n - 1; ++i) buf1[i] = C(buf1[i].re, 1.0);
+  for (int i = 1; i < n - 1; ++i) buf1[i] = C(buf1[i].re, usrc ? 0.0 : 1.0);
   buf1[n - 1] = C(0.0, 0.0);
   labs_bs(L.Dbuf.data(), n, bw, buf1);
-  phi = calc_phi(bulk_t, labs_integrate(integ, buf1, n));
+  phi = usrc ? 0.0 : calc_phi(bulk_t, labs_integrate(integ, buf1, n));
   for (int i = 0; i < n; ++i) buf1[i] = C(buf1[i].re + phi * buf1[i].im, 0.0);
   mo.phi_t = phi;
   if (s == S) mo.q_w = 0.5 * nu / pr * wall_d1(buf1);
@@ -1252,12 +1264,18 @@ int orc_opp_planes(void *h, const double *Sd, double *out) {
 // u,v,w,theta in slots 0,1,2,6 on exit; RK(ny,nxh,nz,3) is updated.
 // vd (nxh,nz,2) = detection-plane v for opposition control (may be NULL).
 // mean[4] = phi_u, phi_t, tau_w, q_w (written by the line that holds (0,0)).
-int orc_wavespace(void *h, int s, double dt, double *Ud, double *RKd, const double *vdd, int l0, int l1,
-                  double *mean) {
+int orc_wavespace(void *h, int s, double dt, double *Ud, double *RKd, const double *vdd, const double *usrcd,
+                  int l0, int l1, double *mean) {
   Ctx *c = static_cast<Ctx *>(h);
   cd *U = reinterpret_cast<cd *>(Ud);
   cd *RK = reinterpret_cast<cd *>(RKd);
   const cd *vd = reinterpret_cast<const cd *>(vdd);
+  // mixed-type scalar condition: u of the substep's input state, reference
+  // layout (ny, nxh, nz)
+  const bool mixed = c->cfg.scalar_bc == PB_SCALAR_MIXED;
+  if (mixed && !usrcd) return PB_E_ARG;
+  const cd *usrc = mixed ? reinterpret_cast<const cd *>(usrcd) : nullptr;
+  const double ks = (g_mutant == 9 ? -1.0 : 1.0) * c->cfg.nu / c->cfg.pr / c->cfg.bulk_u;
   const int n = c->ny;
   const size_t nl = (size_t)c->nxh * c->nz;
   const pb_config &g = c->cfg;
@@ -1277,7 +1295,8 @@ int orc_wavespace(void *h, int s, double dt, double *Ud, double *RKd, const doub
       if (c->kx[k] == 0 && c->kz[j] == 0) {
         MeanOut mo;
         wave00(L, slot(0), slot(1), slot(2), slot(3), slot(4), slot(6), slot(7), slot(8), rk(0), rk(1), rk(2),
-               b1.data(), g.nu, g.pr, g.bulk_u, g.bulk_w, g.bulk_t, g.u_gal, mo);
+               b1.data(), g.nu, g.pr, g.bulk_u, g.bulk_w, g.bulk_t, g.u_gal, usrc ? usrc + ix(*c, 0, j, k, 0) : nullptr,
+               mo);
         if (mean) { mean[0] = mo.phi_u; mean[1] = mo.phi_t; mean[2] = mo.tau_w; mean[3] = mo.q_w; }
       } else {
         const double kxp = c->kx[k] * c->lmfxr, kzp = c->kz[j] * c->lmfzr;
@@ -1290,6 +1309,11 @@ int orc_wavespace(void *h, int s, double dt, double *Ud, double *RKd, const doub
         }
         solve_v_f(L, slot(1), slot(2), slot(4), rk(1), slot(3), b1.data(), b2.data(), kxp, kzp, g.nu, wlo, whi);
         solve_u_w(n, slot(0), slot(2), kxp, kzp);
+        if (usrc) {  // h_theta += (nu/pr) u / bulk_u
+          cd *ht = slot(6);
+          const cd *us = usrc + ix(*c, 0, j, k, 0);
+          for (int i = 0; i < n; ++i) ht[i] = add(ht[i], mulr(us[i], ks));
+        }
         solve_gt(L, slot(6), rk(2), b1.data(), b2.data(), kxp, kzp, g.nu, g.pr, true);
       }
     }
@@ -1400,7 +1424,9 @@ int orc_substep(void *h, double *Sd, double *RKd, int s, double *dt_inout, doubl
     if (dt <= 1e-14 || dt > 1e5) return PB_E_NUMERIC;
   }
   double mean[4] = {0, 0, 0, 0};
-  orc_wavespace(h, s, dt, P.data(), RKd, vd.empty() ? nullptr : vd.data(), 0, (int)(c->nxh * c->nz), mean);
+  const int rw = orc_wavespace(h, s, dt, P.data(), RKd, vd.empty() ? nullptr : vd.data(), Sd, 0,
+                               (int)(c->nxh * c->nz), mean);
+  if (rw) return rw;
   // copy u,v,w,theta back into the state
   const int map[4] = {0, 1, 2, 6};
   for (int f = 0; f < 4; ++f) std::memcpy(Sd + (size_t)f * N * 2, P.data() + (size_t)map[f] * N * 2, N * 16);

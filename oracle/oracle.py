@@ -40,7 +40,7 @@ def lib():
         L.orc_init_stores.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_nonlinear.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_opp_planes.argtypes = [ctypes.c_void_p, _dp, _dp]
-        L.orc_wavespace.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp,
+        L.orc_wavespace.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp,
                                     ctypes.c_int, ctypes.c_int, _dp]
         L.orc_substep.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp]
         L.orc_pressure.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
@@ -185,11 +185,20 @@ class Oracle:
         lib().orc_opp_planes(self.h, S.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
         return out
 
-    def wavespace(self, s: int, dt: float, P: np.ndarray, RK: np.ndarray, vd: np.ndarray | None = None):
+    def wavespace(self, s: int, dt: float, P: np.ndarray, RK: np.ndarray, vd: np.ndarray | None = None,
+                  usrc: np.ndarray | None = None):
+        """wavespace on products P (10 slots) and stores RK, in place; vd the
+        detection-plane v (opposition control), usrc the u of the substep's
+        input state (mixed-type scalar condition).  Returns phi_u, phi_t,
+        tau_w, q_w."""
         cfg = self.cfg
         mean = np.zeros(4)
-        lib().orc_wavespace(self.h, s, dt, P.ctypes.data_as(_dp), RK.ctypes.data_as(_dp),
-                            None if vd is None else vd.ctypes.data_as(_dp), 0, cfg.nxh * cfg.nz, _p(mean))
+        us = None if usrc is None else np.ascontiguousarray(usrc, dtype=np.complex128)
+        r = lib().orc_wavespace(self.h, s, dt, P.ctypes.data_as(_dp), RK.ctypes.data_as(_dp),
+                                None if vd is None else vd.ctypes.data_as(_dp),
+                                None if us is None else us.ctypes.data_as(_dp), 0, cfg.nxh * cfg.nz, _p(mean))
+        if r:
+            raise RuntimeError(f"orc_wavespace failed: {r}")
         return mean
 
     def hvhg(self, P: np.ndarray) -> np.ndarray:

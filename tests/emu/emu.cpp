@@ -63,8 +63,11 @@ int emu_setup(const pb_config *cfg, int *bw, double *y, double *d0, double *d1, 
   return 0;
 }
 
-// U: 9 slots, RK: 3 slots, device layout [slot][y][kz][kx]; vd: 2 planes or NULL
-int emu_wavespace(const pb_config *cfg, int s, double dt, double *U, double *RK, const double *vd, double *mean4) {
+// U: 9 slots, RK: 3 slots, device layout [slot][y][kz][kx]; vd: 2 planes or
+// NULL; usrc: u of the substep's input state (one field) under the mixed-type
+// scalar condition, else NULL
+int emu_wavespace(const pb_config *cfg, int s, double dt, double *U, double *RK, const double *vd,
+                  const double *usrc, double *mean4) {
   Setup S;
   if (make_setup(cfg, S)) return -1;
   pb::LineParams P = params(cfg, S, s, dt);
@@ -76,6 +79,8 @@ int emu_wavespace(const pb_config *cfg, int s, double dt, double *U, double *RK,
   A.U = (pb::cplx *)U; A.RK = (pb::cplx *)RK; A.buf = buf.data(); A.dbuf = dbuf.data(); A.rbuf = nullptr;
   A.vd = (const pb::cplx *)vd; A.plane = plane; A.field = field; A.nxh = nxh; A.nz = nz; A.nh = nz / 2;
   A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha; A.mean_out = mean4;
+  A.usrc = cfg->scalar_bc == PB_SCALAR_MIXED ? (const pb::cplx *)usrc : nullptr;
+  A.src_scale = cfg->nu / cfg->pr / cfg->bulk_u;
   for (size_t sys = 0; sys < plane; ++sys) pb::wavespace_line(P, A, sys);
   return 0;
 }
@@ -103,7 +108,7 @@ static void run_id(int id, const pb::LineParams &P, const pb::WsArgs &A, double 
 }
 
 extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, double *U, double *RK, const double *vd,
-                                   double *mean4) {
+                                   const double *usrc, double *mean4) {
   Setup S;
   if (make_setup(cfg, S)) return -1;
   pb::LineParams P = params(cfg, S, s, dt);
@@ -115,6 +120,8 @@ extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, doubl
   A.U = (pb::cplx *)U; A.RK = (pb::cplx *)RK; A.buf = buf.data(); A.dbuf = dbuf.data(); A.rbuf = rbuf.data();
   A.vd = (const pb::cplx *)vd; A.plane = plane; A.field = field; A.nxh = nxh; A.nz = nz; A.nh = nz / 2;
   A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha; A.mean_out = mean4;
+  A.usrc = cfg->scalar_bc == PB_SCALAR_MIXED ? (const pb::cplx *)usrc : nullptr;
+  A.src_scale = cfg->nu / cfg->pr / cfg->bulk_u;
   if (S.bw < 3 || S.bw > 5) return -2;
   pb::wavespace_line(P, A, 0);  // (0,0): mean modes
   // sweep-major, as the device runs them: each sweep over all systems

@@ -13,9 +13,10 @@ from channel_passive_amd.initial import make_state
 dp = ctypes.POINTER(ctypes.c_double)
 
 
-@pytest.mark.parametrize("name,alpha", [("tiny", 0.0), ("tiny", 0.8), ("re180_64", 0.0)])
-def test_wavespace_bitwise_cpu(oracle_lib, emu_lib, name, alpha):
-    cfg = dataclasses.replace(preset(name), opp_alpha=alpha, opp_yplus=20.0)
+@pytest.mark.parametrize("name,alpha,sbc", [("tiny", 0.0, 0), ("tiny", 0.8, 0), ("re180_64", 0.0, 0),
+                                            ("tiny", 0.8, 1)])
+def test_wavespace_bitwise_cpu(oracle_lib, emu_lib, name, alpha, sbc):
+    cfg = dataclasses.replace(preset(name), opp_alpha=alpha, opp_yplus=20.0, scalar_bc=sbc)
     orc = oracle_lib.Oracle(cfg)
     c = cfg.to_c()
     S = make_state(cfg, seed=13)
@@ -26,11 +27,12 @@ def test_wavespace_bitwise_cpu(oracle_lib, emu_lib, name, alpha):
         vd = orc.opp_planes(S) if alpha else None
         Ud = np.ascontiguousarray(np.transpose(P[:9], (0, 3, 1, 2)))
         RKd = np.ascontiguousarray(np.transpose(RK, (0, 3, 1, 2)))
+        us = np.ascontiguousarray(np.transpose(S[0], (2, 0, 1)))  # u of the input state, device layout
         m_e = np.zeros(4)
         assert emu_lib.emu_wavespace(ctypes.byref(c), s, ctypes.c_double(cfg.dt), Ud.ctypes.data_as(dp),
                                      RKd.ctypes.data_as(dp), None if vd is None else vd.ctypes.data_as(dp),
-                                     m_e.ctypes.data_as(dp)) == 0
-        m_o = orc.wavespace(s, cfg.dt, P, RK, vd)
+                                     us.ctypes.data_as(dp), m_e.ctypes.data_as(dp)) == 0
+        m_o = orc.wavespace(s, cfg.dt, P, RK, vd, usrc=S[0])
         Ue, RKe = np.transpose(Ud, (0, 2, 3, 1)), np.transpose(RKd, (0, 2, 3, 1))
         for f in (0, 1, 2, 6):
             assert np.array_equal(Ue[f], P[f]), (s, f)
@@ -43,10 +45,12 @@ def test_wavespace_bitwise_cpu(oracle_lib, emu_lib, name, alpha):
 
 @pytest.mark.parametrize("name,alpha,extra", [("tiny", 0.0, {}), ("tiny", 0.8, {}), ("re180_64", 0.0, {}),
                                               ("tiny", 0.5, {"knot_map": 0, "stretch": 0.0, "labs_k": 6, "ny": 34}),
-                                              ("tiny", 0.0, {"ny": 65, "labs_k": 9})])
+                                              ("tiny", 0.0, {"ny": 65, "labs_k": 9}),
+                                              ("tiny", 0.8, {"scalar_bc": 1})])
 def test_fused_wavespace_bitwise_cpu(oracle_lib, emu_lib, name, alpha, extra):
     """The seven-sweep organisation of the device solver (solver_fused.h)
-    reproduces the oracle bit for bit (bw = 4, 3 and 5 grids)."""
+    reproduces the oracle bit for bit (bw = 4, 3 and 5 grids; the mixed-type
+    scalar condition)."""
     cfg = dataclasses.replace(preset(name), opp_alpha=alpha, opp_yplus=20.0, **extra)
     orc = oracle_lib.Oracle(cfg)
     assert orc.bw in (3, 4, 5)
@@ -59,11 +63,12 @@ def test_fused_wavespace_bitwise_cpu(oracle_lib, emu_lib, name, alpha, extra):
         vd = orc.opp_planes(S) if alpha else None
         Ud = np.ascontiguousarray(np.transpose(P[:9], (0, 3, 1, 2)))
         RKd = np.ascontiguousarray(np.transpose(RK, (0, 3, 1, 2)))
+        us = np.ascontiguousarray(np.transpose(S[0], (2, 0, 1)))
         m_e = np.zeros(4)
         assert emu_lib.emu_wavespace_fused(ctypes.byref(c), s, ctypes.c_double(cfg.dt), Ud.ctypes.data_as(dp),
                                            RKd.ctypes.data_as(dp), None if vd is None else vd.ctypes.data_as(dp),
-                                           m_e.ctypes.data_as(dp)) == 0
-        m_o = orc.wavespace(s, cfg.dt, P, RK, vd)
+                                           us.ctypes.data_as(dp), m_e.ctypes.data_as(dp)) == 0
+        m_o = orc.wavespace(s, cfg.dt, P, RK, vd, usrc=S[0])
         Ue, RKe = np.transpose(Ud, (0, 2, 3, 1)), np.transpose(RKd, (0, 2, 3, 1))
         for f in (0, 1, 2, 6):
             assert np.array_equal(Ue[f], P[f]), (s, f, np.abs(Ue[f] - P[f]).max())

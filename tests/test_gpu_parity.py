@@ -29,7 +29,7 @@ def rel(a, b):
 
 def configs():
     small = preset("re180_64")
-    opp = preset("re180_192")
+    opp = preset("re180_192")  # opposition control and the mixed-type scalar condition
     thin385 = dataclasses.replace(preset("re1000"), nx=16, nz=12, name="re1000_thin")  # ny = 385
     tiny_sin = dataclasses.replace(preset("tiny"), knot_map=0, stretch=0.9, labs_k=6, name="tiny_sin")
     flat = dataclasses.replace(preset("re1000"), ny=33, name="re1000_flat")  # production x/z transform sizes
@@ -251,8 +251,11 @@ def test_wavespace_bitwise(lib, oracle_lib, name, s):
     P, _ = orc.nonlinear(S)
     vd = orc.opp_planes(S) if cfg.opp_alpha else None
     v_old = np.ascontiguousarray(S[1:2]) if cfg.opp_alpha else None
+    usrc = S[0] if cfg.scalar_bc == 1 else None  # mixed-type scalar source: u of the input state
     Pg, RKg = P.copy(), RK.copy()
-    mean = orc.wavespace(s, cfg.dt, P, RK, vd)
+    if usrc is not None:
+        Pg[9] = usrc  # the host drop-in reads it from slot 10 (0-based 9)
+    mean = orc.wavespace(s, cfg.dt, P, RK, vd, usrc=usrc)
     ch = lib.Channel(cfg)
     out = ch.wavespace_host(s, cfg.dt, Pg, RKg, v_old)
     for slot in (0, 1, 2, 6):
@@ -485,3 +488,28 @@ def test_spectra(lib, oracle_lib, name):
         assert np.array_equal(ex[q], ex_ref[q]), (k, "x", np.abs(ex[q] - ex_ref[q]).max())
         assert np.array_equal(ez[q], ez_ref[q]), (k, "z", np.abs(ez[q] - ez_ref[q]).max())
     assert np.array_equal(S2, S) and np.array_equal(RK2, RK)
+
+
+def test_mixed_scalar_laminar_on_device(lib, oracle_lib):
+    """Mixed-type scalar condition on the GPU: laminar Poiseuille flow with
+    Theta = (5 - 6 y^2 + y^4)/8 stays steady over three steps, the wall flux
+    per wall is nu/Pr and phi_t = 0 (the bulk is free)."""
+    from channel_passive_amd.config import PB_SCALAR_MIXED
+    from channel_passive_amd.initial import scalar_mean_profile
+    cfg = dataclasses.replace(preset("re180_64"), scalar_bc=PB_SCALAR_MIXED)
+    orc = oracle_lib.Oracle(cfg)
+    y = orc.y
+    S = np.zeros((4, cfg.nz, cfg.nxh, cfg.ny), complex)
+    S[0, 0, 0] = 1.5 * (1 - y * y)
+    S[3, 0, 0] = scalar_mean_profile(cfg, y)
+    ch = lib.Channel(cfg)
+    ch.set_state(S)
+    ch.init_stores()
+    for _ in range(3):
+        outs = ch.step(cfg.dt)
+    Sg, _ = ch.get_state()
+    ch.close()
+    kappa = cfg.nu / cfg.pr
+    assert np.abs(Sg[3, 0, 0] - S[3, 0, 0]).max() <= 1e-12
+    assert all(o["phi_t"] == 0.0 for o in outs)
+    assert abs(outs[-1]["q_w"] - kappa) <= 1e-11 * kappa

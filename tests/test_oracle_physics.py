@@ -138,3 +138,49 @@ def test_detection_row_dense(oracle_lib):
             want = (vals[:, None] * C[first:first + len(vals)]).sum(axis=0).reshape(cfg.nz, cfg.nxh)
             scale = np.abs(S[1]).max()
             assert np.abs(vd[plane] - want).max() <= 1e-12 * scale, plane
+
+
+@pytest.mark.parametrize("name", ["tiny", "re180_64"])
+def test_mixed_scalar_laminar_closed_form(oracle_lib, name):
+    """Uniform wall heat flux, mixed-type condition (source (nu/Pr) u/U_b,
+    theta = 0 at the walls): on laminar Poiseuille flow U = 1.5 U_b (1 - y^2)
+    the scalar Theta = (5 - 6 y^2 + y^4)/8 is steady, the wall flux per wall is
+    q_w = nu/Pr, the bulk is free (phi_t = 0) and the mixed-mean Nusselt number
+    (based on 2h) is 70/17."""
+    from channel_passive_amd.config import PB_SCALAR_MIXED
+    from channel_passive_amd.initial import scalar_mean_profile
+    from channel_passive_amd import observables
+    cfg = dataclasses.replace(preset(name), scalar_bc=PB_SCALAR_MIXED)
+    orc = oracle_lib.Oracle(cfg)
+    y = orc.y
+    S = np.zeros((4, cfg.nz, cfg.nxh, cfg.ny), complex)
+    S[0, 0, 0] = 1.5 * (1 - y * y)
+    S[3, 0, 0] = scalar_mean_profile(cfg, y)
+    assert np.allclose(S[3, 0, 0].real, (5 - 6 * y ** 2 + y ** 4) / 8, rtol=0, atol=1e-15)
+    S0 = S.copy()
+    RK = orc.init_stores(S)
+    for _ in range(3):
+        outs = orc.step(S, RK, cfg.dt)
+    kappa = cfg.nu / cfg.pr
+    assert np.abs(S[3, 0, 0] - S0[3, 0, 0]).max() <= 1e-12
+    assert all(o["phi_t"] == 0.0 for o in outs)
+    assert abs(outs[-1]["q_w"] - kappa) <= 1e-11 * kappa
+    st = orc.statistics(S)
+    quad = observables.Quadrature({"d0": orc.d0, "integ": orc.integ}, orc.bw)
+    theta_m = observables.theta_mean(st, quad, "mixed")
+    nu_mixed = observables.nusselt(outs[-1]["q_w"], cfg.nu, cfg.pr, theta_m)
+    assert abs(nu_mixed - 70.0 / 17.0) <= 1e-11 * 70.0 / 17.0, nu_mixed
+
+
+def test_mixed_scalar_perturbation_bounded(oracle_lib):
+    """With the mixed-type condition the source acts on every mode: a
+    perturbed state keeps theta = 0 at the walls on every line and finite."""
+    from channel_passive_amd.config import PB_SCALAR_MIXED
+    cfg = dataclasses.replace(preset("tiny"), scalar_bc=PB_SCALAR_MIXED)
+    orc = oracle_lib.Oracle(cfg)
+    S = make_state(cfg, seed=3)
+    RK = orc.init_stores(S)
+    for _ in range(5):
+        orc.step(S, RK, cfg.dt)
+    assert np.all(np.isfinite(S))
+    assert np.abs(S[3][..., 0]).max() == 0.0 and np.abs(S[3][..., -1]).max() == 0.0

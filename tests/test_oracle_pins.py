@@ -200,11 +200,11 @@ def mean_dense(D0, D2, w, b, rhs, bulk):
     return x[:n], x[n]
 
 
-def dense_step_cfg():
+def dense_step_cfg(scalar_bc=0):
     # a viscosity large enough that the implicit terms move the result by
     # O(1e-2) of the fields; opposition control moves the walls
     return dataclasses.replace(preset("tiny"), nx=16, nz=8, ny=33, nu_value=0.02, dt=0.01, opp_alpha=0.6,
-                               opp_yplus=30.0, bulk_u=0.9, bulk_t=1.3, name="dense")
+                               opp_yplus=30.0, bulk_u=0.9, bulk_t=1.3, scalar_bc=scalar_bc, name="dense")
 
 
 def run_dense_check(oracle_lib, cfg, seed=17):
@@ -228,6 +228,14 @@ def run_dense_check(oracle_lib, cfg, seed=17):
         Ps.append(P)
     vd = cplx(2, cfg.nz, cfg.nxh)
     vd[:, nh] = 0.0
+    # mixed-type scalar condition: the source (nu/Pr) u / U_b with u of each
+    # substep's input state (real on the mean line)
+    mixed = cfg.scalar_bc == 1
+    ks = nu / pr / cfg.bulk_u_value()
+    us = [cplx(cfg.nz, cfg.nxh, n) for _ in range(3)]
+    for u in us:
+        u[nh] = 0.0
+        u[0, 0] = u[0, 0].real
     hs = [orc.hvhg(P) for P in Ps]  # pinned by test_hvhg_from_definitions
     dy = D1 @ D0i
     d2 = D2 @ D0i
@@ -248,7 +256,7 @@ def run_dense_check(oracle_lib, cfg, seed=17):
         a_s, b_s, c_s = SMR_ALPHA[s - 1] * dt, SMR_BETA[s - 1] * dt, SMR_GAMMA[s - 1] * dt
         z_s = SMR_ZETA[s - 1] * dt
         U = Ps[s - 1].copy()
-        mean = orc.wavespace(s, dt, U, RKo, vd)
+        mean = orc.wavespace(s, dt, U, RKo, vd, usrc=us[s - 1] if mixed else None)
         for (j, k), (g0, p0, t0) in q.items():
             kx, kz = kxp[k], kzp[j]
             k2 = kx * kx + kz * kz
@@ -257,10 +265,14 @@ def run_dense_check(oracle_lib, cfg, seed=17):
                 # mean lines: h_U = -d(uv)/dy, h_W = -d(vw)/dy, h_T = -d(v theta)/dy
                 out = []
                 for f, (slot, fac, bulk, src) in enumerate(((1, 1.0, cfg.bulk_u_value(), 0),
-                                                             (3, 1.0, None, 2), (7, 1.0 / pr, cfg.bulk_t, 6))):
+                                                             (3, 1.0, None, 2),
+                                                             (7, 1.0 / pr, None if mixed else cfg.bulk_t, 6))):
                     qq = (g0, p0, t0)[f].real
                     h = -(dy @ Ps[s - 1][slot, 0, 0].real)
                     hprev = 0.0 if s == 1 else -(dy @ Ps[s - 2][slot, 0, 0].real)
+                    if f == 2 and mixed:  # heat-flux source on the mean scalar
+                        h = h + ks * us[s - 1][0, 0].real
+                        hprev = hprev if s == 1 else hprev + ks * us[s - 2][0, 0].real
                     rhs = qq + a_s * nu * fac * (d2 @ qq) + c_s * h + z_s * hprev
                     c, force = mean_dense(D0, D2, orc.integ, b_s * nu * fac, rhs, bulk)
                     new = D0 @ c
@@ -279,6 +291,10 @@ def run_dense_check(oracle_lib, cfg, seed=17):
                 q[(j, k)] = out
                 continue
             hg, hv, ht = hs[s - 1][0][j, k], hs[s - 1][1][j, k], hs[s - 1][2][j, k]
+            if mixed:
+                ht = ht + ks * us[s - 1][j, k]
+                if s > 1:
+                    hp[2] = hp[2] + ks * us[s - 2][j, k]
             new_vals, stores = [], []
             for f, (q0, h, fac) in enumerate(((g0, hg, 1.0), (t0, ht, 1.0 / pr))):
                 rhs = q0 + a_s * nu * fac * (d2 @ q0 - k2 * q0) + c_s * h
@@ -324,17 +340,20 @@ def run_dense_check(oracle_lib, cfg, seed=17):
 DENSE_TOL = 1e-10
 
 
-def test_wall_normal_solves_dense_recurrence(oracle_lib):
-    err = run_dense_check(oracle_lib, dense_step_cfg())
-    print(f"dense RK3-CN recurrence: worst relative error {err:.2e}")
+@pytest.mark.parametrize("scalar_bc", [0, 1])
+def test_wall_normal_solves_dense_recurrence(oracle_lib, scalar_bc):
+    """scalar_bc 1: the mixed-type scalar condition, h_theta + (nu/Pr) u/U_b
+    on every line and no held bulk on the mean scalar."""
+    err = run_dense_check(oracle_lib, dense_step_cfg(scalar_bc))
+    print(f"dense RK3-CN recurrence (scalar_bc {scalar_bc}): worst relative error {err:.2e}")
     assert err <= DENSE_TOL
 
 
-@pytest.mark.parametrize("m", [3, 4, 5, 6, 7, 8])
-def test_dense_recurrence_fails_on_planted_mistake(oracle_lib, m):
+@pytest.mark.parametrize("m,scalar_bc", [(3, 0), (4, 0), (5, 0), (6, 0), (7, 0), (8, 0), (9, 1)])
+def test_dense_recurrence_fails_on_planted_mistake(oracle_lib, m, scalar_bc):
     from oracle import oracle as O
     with O.mutant(m):
-        err = run_dense_check(oracle_lib, dense_step_cfg())
+        err = run_dense_check(oracle_lib, dense_step_cfg(scalar_bc))
     assert err > 1e3 * DENSE_TOL, (m, err)
 
 
