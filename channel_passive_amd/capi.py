@@ -67,6 +67,7 @@ SIGNATURES = {
     "pb_labs_lu": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp]),
     "pb_labs_bs": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
     "pb_labs_mat_mul": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32, _dp, _dp]),
+    "pb_labs_bs_noalloc": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
 }
 
 _lib = None
@@ -274,6 +275,14 @@ def labs_bs(LU: np.ndarray, bw: int, b: np.ndarray) -> np.ndarray:
     x = np.ascontiguousarray(b, dtype=np.complex128).copy()
     n = LU.shape[0]
     _check(load().pb_labs_bs(n, bw, np.ascontiguousarray(LU).ctypes.data_as(_dp), x.size // n, _ptr(x)))
+    return x
+
+
+def labs_bs_noalloc(LU: np.ndarray, bw: int, b: np.ndarray) -> np.ndarray:
+    """labs_bs through the library's device scratch (no allocation)."""
+    x = np.ascontiguousarray(b, dtype=np.complex128).copy()
+    n = LU.shape[0]
+    _check(load().pb_labs_bs_noalloc(n, bw, np.ascontiguousarray(LU).ctypes.data_as(_dp), x.size // n, _ptr(x)))
     return x
 
 

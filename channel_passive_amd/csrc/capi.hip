@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1098,6 +1099,54 @@ int pb_labs_bs(int32_t n, int32_t bw, const double *A_lu, int32_t nrhs, double *
                      nrhs, (pb::cplx *)db.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(b, db.p, bb, hipMemcpyDeviceToHost));
+  return PB_OK;
+}
+
+// ---- pb_labs_bs without allocation: the band factor and the right-hand
+// sides are staged through `scratch`, file-scope device memory of
+// PB_NOALLOC_SCRATCH_BYTES that every GPU holds (its own copy) until the
+// process ends.  Host code uses the per-device address from
+// hipGetSymbolAddress, never the array's name.
+static __device__ __align__(256) double scratch[PB_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices];
+
+// Address of `scratch` on the current device, looked up on the first call
+// on that device.
+static int scratch_ptr_current(double **out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(PB_E_HIP, "device %d outside 0..%d", dev, kMaxDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(PB_E_HIP, "hipGetSymbolAddress(scratch) returned null on device %d", dev);
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PB_OK;
+}
+
+int pb_labs_bs_noalloc(int32_t n, int32_t bw, const double *A_lu, int32_t nrhs, double *b) {
+  int r;
+  if (!A_lu || !b) return fail(PB_E_ARG, "null argument");
+  if ((r = labs_check(n, bw, nrhs))) return r;
+  constexpr size_t cap = PB_NOALLOC_SCRATCH_BYTES / sizeof(double);
+  const size_t na = (size_t)n * (2 * bw + 1), nb = (size_t)nrhs * n * 2;
+  const size_t off_b = (na + 31) & ~(size_t)31;  // right-hand sides 256-byte aligned after the factor
+  if (off_b + nb > cap)
+    return fail(PB_E_ARG, "pb_labs_bs_noalloc: factor and %d right-hand sides of %d need %zu bytes, above the %zu-byte "
+                "scratch", nrhs, n, (off_b + nb) * sizeof(double), (size_t)PB_NOALLOC_SCRATCH_BYTES);
+  double *ptr = nullptr;
+  if ((r = scratch_ptr_current(&ptr))) return r;
+  HIPCHK(hipMemcpy(ptr, A_lu, na * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ptr + off_b, b, nb * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(pb::labs_bs_kernel, dim3(blocks_for(nrhs, 64)), dim3(64), 0, 0, n, bw, (const double *)ptr, nrhs,
+                     (pb::cplx *)(ptr + off_b));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(b, ptr + off_b, nb * sizeof(double), hipMemcpyDeviceToHost));
   return PB_OK;
 }
 

@@ -258,6 +258,17 @@ int pb_labs_lu(int32_t n, int32_t bw, int32_t nmat, double *A);
 int pb_labs_bs(int32_t n, int32_t bw, const double *A_lu, int32_t nrhs, double *b);
 /* LABS_mat_mul (module_labs.f90:570): b = A x for nrhs columns. */
 int pb_labs_mat_mul(int32_t n, int32_t bw, const double *A, int32_t nrhs, const double *x, double *b);
+/* pb_labs_bs without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (tight loops that call
+ * LABS_bs per line, harnesses that forbid hipMalloc).  Same arguments and the
+ * same kernel as pb_labs_bs (bit-identical results); the factor and the
+ * right-hand sides are staged through a library-owned device array of
+ * PB_NOALLOC_SCRATCH_BYTES on the current GPU (one per GPU, held until the
+ * process ends), so n*(2bw+1) doubles, rounded up to 32, plus 2*nrhs*n
+ * doubles must fit in it (PB_E_ARG otherwise).  Calls that overlap on one
+ * GPU (other threads) share that array and race. */
+#define PB_NOALLOC_SCRATCH_BYTES (8u << 20)
+int pb_labs_bs_noalloc(int32_t n, int32_t bw, const double *A_lu, int32_t nrhs, double *b);
 
 #ifdef __cplusplus
 }

@@ -224,6 +224,14 @@ module poongback_gpu
        complex(c_double_complex), intent(inout) :: b(*)
      end function pb_labs_bs
 
+     ! pb_labs_bs staged through the library's device scratch (8 MiB): no allocation
+     integer(c_int) function pb_labs_bs_noalloc(n, bw, A_lu, nrhs, b) bind(C, name="pb_labs_bs_noalloc")
+       import :: c_int, c_int32_t, c_double, c_double_complex
+       integer(c_int32_t), value :: n, bw, nrhs
+       real(c_double), intent(in) :: A_lu(*)
+       complex(c_double_complex), intent(inout) :: b(*)
+     end function pb_labs_bs_noalloc
+
      integer(c_int) function pb_labs_mat_mul(n, bw, A, nrhs, x, b) bind(C, name="pb_labs_mat_mul")
        import :: c_int, c_int32_t, c_double, c_double_complex
        integer(c_int32_t), value :: n, bw, nrhs
