@@ -101,6 +101,92 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     print(f"[bench] copy calibration: {reps} copies of {nbytes} bytes", file=sys.stderr, flush=True)
 
 
+def ensemble_leg(steps, warmup, reps=3):
+    """The ensemble workload (24 uneven Re_tau = 180 channels, seeded gains,
+    detection planes and Prandtl numbers): every channel stepped with pb_step
+    one after another (the plain path), against the tuned ensemble, one
+    captured graph per ensemble step.  Both run warmup + steps steps from the
+    same states; the final states must be bit-identical."""
+    import collections
+    import numpy as np
+    from channel_passive_amd import capi
+    from channel_passive_amd.config import ensemble_configs
+    from channel_passive_amd.initial import make_state
+
+    cfgs = ensemble_configs(24)
+    # every channel takes its dt from the CFL rule on substep 1 (dt_factor =
+    # 0); the values passed in are only starting values
+    dts = [c.dt for c in cfgs]
+    ens = capi.Ensemble(cfgs)
+    tuned = ens.tune(reps=reps)
+    plain = []
+    for i, c in enumerate(cfgs):
+        S = make_state(c, seed=0xC0FFEE + 3 + i)
+        ens.channel(i).set_state(S)
+        ens.channel(i).init_stores()
+        ch = capi.Channel(c)
+        ch.set_state(S)
+        ch.init_stores()
+        plain.append(ch)
+    print(f"[bench] ensemble: {len(cfgs)} channels set, tuned step {tuned[-1]:.3f} ms", file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        ens.step(dts)
+        for c, ch in zip(cfgs, plain):
+            ch.step(c.dt)
+    for ch in plain:
+        ch.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for c, ch in zip(cfgs, plain):
+            ch.step(c.dt)
+    t_seq = time.perf_counter() - t0
+    cfl_max = 0.0
+    print(f"[bench] ensemble: sequential plain {1e3 * t_seq / steps:.3f} ms per step", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        outs = ens.step(dts)
+    t_ens = time.perf_counter() - t0
+    for c, o in zip(cfgs, outs):
+        cfl_max = max(cfl_max, math.pi * o[0]["dt"] * o[0]["velmax"])
+    print(f"[bench] ensemble: one graph {1e3 * t_ens / steps:.3f} ms per step", file=sys.stderr, flush=True)
+    same = True
+    for i, ch in enumerate(plain):
+        Sa, Ra = ens.channel(i).get_state()
+        Sb, Rb = ch.get_state()
+        same = same and bool(np.array_equal(Sa, Sb) and np.array_equal(Ra, Rb))
+        ch.close()
+    ens.close()
+    grids = collections.Counter(f"{c.nx}x{c.ny}x{c.nz}" for c in cfgs)
+    return {"channels": len(cfgs), "grids": dict(grids), "steps": steps, "warmup": warmup,
+            "sequential_ms_per_step": 1e3 * t_seq / steps, "ensemble_ms_per_step": 1e3 * t_ens / steps,
+            "speedup": t_seq / t_ens, "bitwise_equal_to_plain": same, "dt": "CFL rule, CFL = 0.8",
+            "cfl_number_last": round(cfl_max, 4),
+            "tuned_channel_ms": [round(float(x), 4) for x in tuned[:-1]], "tuned_ensemble_ms": float(tuned[-1])}
+
+
+ENSEMBLE_HW_QUEUES = "16"
+
+
+def ensemble_subprocess(steps):
+    """The ensemble comparison in a child process with GPU_MAX_HW_QUEUES = 16
+    (the HIP runtime reads it at start-up; 24 channels on the default 4
+    hardware queues overlap only four at a time).  Failures are reported in
+    the object, the headline stands."""
+    import subprocess
+    env = dict(os.environ, GPU_MAX_HW_QUEUES=ENSEMBLE_HW_QUEUES)
+    cmd = [sys.executable, os.path.abspath(__file__), "--ensemble-only", "--ensemble-steps", str(steps)]
+    try:
+        res = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+        sys.stderr.write(res.stderr[-2000:])
+        if res.returncode != 0:
+            return {"error": f"exit {res.returncode}", "stderr_tail": res.stderr[-500:]}
+        obj = json.loads(res.stdout.strip().splitlines()[-1])
+        obj["gpu_max_hw_queues"] = int(ENSEMBLE_HW_QUEUES)
+        return obj
+    except Exception as exc:  # reported, not fatal
+        return {"error": f"{type(exc).__name__}: {exc}"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,12 +194,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", default="re1000")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 24-channel ensemble comparison")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the ensemble comparison and print its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.ensemble_only:
+        print(json.dumps(ensemble_leg(args.ensemble_steps, 10)))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the library runs on one GPU "
@@ -219,6 +312,8 @@ def main():
         "stages_roofline_frac": stage_frac,
         "cpu_baseline": cpu,
     }
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_subprocess(args.ensemble_steps)
     print(json.dumps(out))
 
 

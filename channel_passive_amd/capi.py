@@ -68,6 +68,12 @@ SIGNATURES = {
     "pb_labs_bs": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
     "pb_labs_mat_mul": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32, _dp, _dp]),
     "pb_labs_bs_noalloc": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
+    "pb_step_graph": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(PbSubstepOut)]),
+    "pb_ensemble_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(PbConfig), ctypes.POINTER(ctypes.c_void_p)]),
+    "pb_ensemble_channel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "pb_ensemble_step": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(PbSubstepOut)]),
+    "pb_ensemble_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp]),
+    "pb_ensemble_destroy": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 _lib = None
@@ -104,19 +110,23 @@ def _ptr(a: np.ndarray):
 class Channel:
     """One channel on the current GPU, state resident in HBM."""
 
-    def __init__(self, cfg: ChannelConfig):
+    def __init__(self, cfg: ChannelConfig, handle: ctypes.c_void_p | None = None):
+        """handle: a context owned by an Ensemble (not destroyed by close)."""
         self.cfg = cfg
         self._c = cfg.to_c()
-        h = ctypes.c_void_p()
-        _check(load().pb_create(ctypes.byref(self._c), ctypes.byref(h)))
-        self.h = h
+        self._owned = handle is None
+        if handle is None:
+            handle = ctypes.c_void_p()
+            _check(load().pb_create(ctypes.byref(self._c), ctypes.byref(handle)))
+        self.h = handle
         bw, nbytes = ctypes.c_int32(0), ctypes.c_int64(0)
         _check(load().pb_get_info(self.h, ctypes.byref(bw), ctypes.byref(nbytes)))
         self.bw, self.device_bytes = bw.value, nbytes.value
 
     def close(self):
         if getattr(self, "h", None):
-            load().pb_destroy(self.h)
+            if self._owned:
+                load().pb_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -200,6 +210,14 @@ class Channel:
         d = ctypes.c_double(dt)
         o = (PbSubstepOut * self.cfg.rk_step)()
         _check(load().pb_step(self.h, ctypes.byref(d), o))
+        return [x.as_dict() for x in o]
+
+    def step_graph(self, dt: float):
+        """pb_step_graph: the step captured in a hipGraph on the first call,
+        replayed after it."""
+        d = ctypes.c_double(dt)
+        o = (PbSubstepOut * self.cfg.rk_step)()
+        _check(load().pb_step_graph(self.h, ctypes.byref(d), o))
         return [x.as_dict() for x in o]
 
     def sync(self):
@@ -294,3 +312,55 @@ def labs_mat_mul(A: np.ndarray, bw: int, x: np.ndarray) -> np.ndarray:
     _check(load().pb_labs_mat_mul(n, bw, np.ascontiguousarray(A).ctypes.data_as(_dp), xx.size // n, _ptr(xx),
                                   _ptr(out)))
     return out
+
+
+class Ensemble:
+    """Independent channels of uneven size advanced together: every channel's
+    step in one captured graph, one launch per ensemble step
+    (pb_ensemble_*).  channel(i) gives a Channel view for the state exchange."""
+
+    def __init__(self, cfgs):
+        self.cfgs = list(cfgs)
+        arr = (PbConfig * len(self.cfgs))(*[c.to_c() for c in self.cfgs])
+        h = ctypes.c_void_p()
+        _check(load().pb_ensemble_create(len(self.cfgs), arr, ctypes.byref(h)))
+        self.h = h
+        self.channels = []
+        for i, c in enumerate(self.cfgs):
+            ch = ctypes.c_void_p()
+            _check(load().pb_ensemble_channel(self.h, i, ctypes.byref(ch)))
+            self.channels.append(Channel(c, handle=ch))
+
+    def channel(self, i: int) -> Channel:
+        return self.channels[i]
+
+    def tune(self, reps: int = 5):
+        """The autotuner (before the states are set): per-channel step times
+        and the ensemble's, in ms."""
+        ms = np.zeros(len(self.cfgs) + 1)
+        _check(load().pb_ensemble_tune(self.h, reps, ms.ctypes.data_as(_dp)))
+        return ms
+
+    def step(self, dts):
+        dt = np.ascontiguousarray(dts, dtype=np.float64).copy()
+        nout = sum(c.rk_step for c in self.cfgs)
+        o = (PbSubstepOut * nout)()
+        _check(load().pb_ensemble_step(self.h, dt.ctypes.data_as(_dp), o))
+        outs, k = [], 0
+        for c in self.cfgs:
+            outs.append([o[k + q].as_dict() for q in range(c.rk_step)])
+            k += c.rk_step
+        return outs
+
+    def close(self):
+        if getattr(self, "h", None):
+            for ch in self.channels:
+                ch.h = None
+            load().pb_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

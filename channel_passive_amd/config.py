@@ -147,6 +147,46 @@ def physical_wavenumbers(cfg: ChannelConfig):
     return kx, kz, kx / cfg.lmfx, kz / cfg.lmfz
 
 
+# grids of the ensemble workload (Re_tau = 180), with the box and a CFL-stable dt
+ENSEMBLE_GRIDS = (
+    (64, 65, 64, 1.0, 0.5, 0.005),
+    (96, 97, 96, 1.0, 0.5, 0.0033),
+    (128, 97, 128, 1.0, 0.5, 0.0025),
+    (192, 129, 160, 2.0, 2.0 / 3.0, 0.0025),
+)
+
+
+ENSEMBLE_ALPHA_MAX = 1.0
+
+
+def ensemble_configs(n: int = 24, seed: int = 0xC0FFEE + 3) -> list:
+    """The ensemble workload: n independent Re_tau = 180 channels, grid drawn
+    from ENSEMBLE_GRIDS, opposition gain uniform in [0, ENSEMBLE_ALPHA_MAX],
+    detection plane y+ in {5, 10, 15, 20}, Prandtl number in {0.71, 1.0, 2.0},
+    all from one seeded generator.  The gain stops at 1: the wall value
+    v(+-1) = -alpha v(y_d) is set from the substep's input, so a near-wall
+    mode is multiplied by about -alpha every substep, and gains above 1
+    diverge (alpha = 1.26 at y+ = 5 and 20 on 64 x 65 x 64: the CFL number
+    grows from 0.65 to 2.6 within 12 steps).  The time step follows the
+    reference's CFL rule every step (dt_factor = 0, CFL = 0.8,
+    dft_x_realspace.f90:489-498): the wall transpiration of the control grows
+    as the perturbation develops, and |v(wall)| dyr(wall) with the fine
+    near-wall spacing raises the CFL number (with a fixed dt, alpha = 0.25 at
+    y+ = 20 goes from 0.62 to 1.03 in 150 steps, the uncontrolled flow stays
+    at 0.62-0.68 over 300)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nx, ny, nz, lmfx, lmfz, dt = ENSEMBLE_GRIDS[int(rng.integers(len(ENSEMBLE_GRIDS)))]
+        alpha = float(rng.uniform(0.0, ENSEMBLE_ALPHA_MAX))
+        yplus = float(rng.choice([5.0, 10.0, 15.0, 20.0]))
+        pr = float(rng.choice([0.71, 1.0, 2.0]))
+        out.append(ChannelConfig(nx, ny, nz, re_tau=180.0, lmfx=lmfx, lmfz=lmfz, dt=dt, pr=pr, opp_alpha=alpha,
+                                 opp_yplus=yplus, dt_factor=0.0, cfl=0.8, name=f"ens{i}_{nx}x{ny}x{nz}"))
+    return out
+
+
 def roofline_bytes_per_step(cfg: ChannelConfig) -> int:
     """Algorithmic HBM bytes of one RK3 step: 3 substeps x 30 spectral
     field passes x 16 B x ny*(nx/2)*nz."""

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -80,6 +81,12 @@ struct pb_ctx {
   // device memory
   double2 *U = nullptr, *RK = nullptr, *pad = nullptr, *buf = nullptr, *vd = nullptr;
   double2 *usrc = nullptr;  // mixed-type scalar condition: u of the substep's input state
+  hipGraph_t step_graph = nullptr;         // pb_step_graph: one step, captured on the first call
+  hipGraphExec_t step_exec = nullptr;
+  // enqueue everything on `stream` (no side-stream forks): for graphs that
+  // must run beside other graphs (a graph's own branches share the
+  // runtime's internal streams with every other graph's)
+  bool linear = false;
   double *dbuf = nullptr, *dbuf2 = nullptr, *rbuf = nullptr, *vpart = nullptr, *scal = nullptr;
   double2 *ppe_bc = nullptr;  // statistics time: pressure wall values [2][kz][kx]
   double *dcenter = nullptr;  // B_k(0): centre-line values of the basis (get_statistics.f90:566-570)
@@ -150,6 +157,8 @@ static int ctx_free(pb_ctx *c) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
+  if (c->step_graph) (void)hipGraphDestroy(c->step_graph);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
@@ -667,16 +676,19 @@ static int launch_sweeps(pb_ctx *c, const pb::LineParams &P, const pb::WsArgs &A
   if (s1) SWEEP(pb::SW_BG1, c->stream) else SWEEP(pb::SW_BG, c->stream);
   // theta chain (slots 5, 6, 7, 8, RK3, dbuf2) beside B_v, C_nu, D_nu (slots
   // 0-4, RK1, RK2, dbuf); joined before E and F overwrite slots 5, 7, 8
-  HIPCHK(hipEventRecord(c->ev_fork2, c->stream));
-  HIPCHK(hipStreamWaitEvent(c->side2, c->ev_fork2, 0));
-  if (s1) SWEEP(pb::SW_BT1, c->side2) else SWEEP(pb::SW_BT, c->side2);
-  SWEEP(pb::SW_CT, c->side2);
-  SWEEP(pb::SW_DT, c->side2);
-  HIPCHK(hipEventRecord(c->ev_join2, c->side2));
+  hipStream_t th = c->linear ? c->stream : c->side2;
+  if (!c->linear) {
+    HIPCHK(hipEventRecord(c->ev_fork2, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->side2, c->ev_fork2, 0));
+  }
+  if (s1) SWEEP(pb::SW_BT1, th) else SWEEP(pb::SW_BT, th);
+  SWEEP(pb::SW_CT, th);
+  SWEEP(pb::SW_DT, th);
+  if (!c->linear) HIPCHK(hipEventRecord(c->ev_join2, c->side2));
   SWEEP(pb::SW_BV, c->stream);
   SWEEP(pb::SW_CNU, c->stream);
   SWEEP(pb::SW_DNU, c->stream);
-  HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
+  if (!c->linear) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
   SWEEP(pb::SW_E, c->stream);
   SWEEP(pb::SW_F, c->stream);
   SWEEP(pb::SW_G, c->stream);
@@ -717,12 +729,14 @@ static int launch_wavespace(pb_ctx *c, int s) {
     // The one-thread wave00_kernel also uses the scratch columns, which the
     // sweeps use too, so it runs after them.
     if (c->mean_lds) {
-      HIPCHK(hipEventRecord(c->ev_fork, c->stream));
-      HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-      hipLaunchKernelGGL(pb::mean_kernel, dim3(3), dim3(pb::MEAN_THREADS), c->mean_lds, c->side, P, A, c->scal,
-                         c->mean_db);
+      if (!c->linear) {
+        HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+        HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+      }
+      hipLaunchKernelGGL(pb::mean_kernel, dim3(3), dim3(pb::MEAN_THREADS), c->mean_lds, c->linear ? c->stream : c->side,
+                         P, A, c->scal, c->mean_db);
       HIPCHK(hipGetLastError());
-      HIPCHK(hipEventRecord(c->ev_join, c->side));
+      if (!c->linear) HIPCHK(hipEventRecord(c->ev_join, c->side));
     }
     int r = PB_OK;
     switch (c->bw) {
@@ -731,7 +745,7 @@ static int launch_wavespace(pb_ctx *c, int s) {
       case 5: r = launch_sweeps<5>(c, P, A, s); break;
     }
     if (c->mean_lds) {
-      HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+      if (!c->linear) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     } else {
       hipLaunchKernelGGL(pb::wave00_kernel, dim3(1), dim3(64), 0, c->stream, P, A, c->scal);
       HIPCHK(hipGetLastError());
@@ -802,9 +816,14 @@ static int upload_dt(pb_ctx *c, double dt) {
   return PB_OK;
 }
 
+static int parse_outputs(pb_ctx *c, int s0, int s1, pb_substep_out *out, double *dt_inout);
 static int read_outputs(pb_ctx *c, int s0, int s1, pb_substep_out *out, double *dt_inout) {
   HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SCAL_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return parse_outputs(c, s0, s1, out, dt_inout);
+}
+// outputs of substeps s0..s1 from the host mirror of the device scalars
+static int parse_outputs(pb_ctx *c, int s0, int s1, pb_substep_out *out, double *dt_inout) {
   const double *h = c->hscal;
   *dt_inout = h[pb::SCAL_DT];
   if (out)
@@ -919,6 +938,303 @@ int pb_step(pb_ctx *c, double *dt_inout, pb_substep_out *out) {
   for (int s = 1; s <= c->cfg.rk_step; ++s)
     if ((r = substep_async(c, s))) return r;
   return read_outputs(c, 1, c->cfg.rk_step, out, dt_inout);
+}
+
+// ---- captured step ----------------------------------------------------------
+// pb_step's device work -- dt into the device scalars, the rk_step substeps,
+// the scalars back to the pinned host mirror -- enqueued on the context's
+// stream (the wall-normal stage forks onto the two side streams and joins
+// them).  The kernels read dt from device memory, so the same work serves
+// every dt.  No memset node: the device scalars are overwritten by copies.
+static int enqueue_step(pb_ctx *c) {
+  HIPCHK(hipMemcpyAsync(c->scal, c->hscal, 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  for (int s = 1; s <= c->cfg.rk_step; ++s) {
+    const int r = substep_async(c, s);
+    if (r) return r;
+  }
+  HIPCHK(hipMemcpyAsync(c->hscal, c->scal, SCAL_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return PB_OK;
+}
+
+static void drop_graph(pb_ctx *c) {
+  if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
+  if (c->step_graph) (void)hipGraphDestroy(c->step_graph);
+  c->step_exec = nullptr;
+  c->step_graph = nullptr;
+}
+
+// Capture `work` on stream `st` into *g and instantiate it into *x; the
+// capture is always ended, also when `work` fails.
+extern "C++" template <class W>
+static int capture(hipStream_t st, W &&work, hipGraph_t *g, hipGraphExec_t *x) {
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int r = work();
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &graph);
+  if (r) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r;
+  }
+  if (e != hipSuccess || !graph) return fail(PB_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(x, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return fail(PB_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+  }
+  *g = graph;
+  return PB_OK;
+}
+
+static int step_checks(pb_ctx *c, const double *dt_inout) {
+  if (!c || !dt_inout) return fail(PB_E_ARG, "null argument");
+  if (!c->have_state) return fail(PB_E_STATE, "step before pb_set_state");
+  if (c->timing) return fail(PB_E_STATE, "a captured step cannot record the stage timing events; pb_timing(ctx, 0)");
+  const int r = check_dt(*dt_inout);
+  if (r && c->cfg.dt_factor == 1.0) return r;
+  return PB_OK;
+}
+
+int pb_step_graph(pb_ctx *c, double *dt_inout, pb_substep_out *out) {
+  int r;
+  if ((r = step_checks(c, dt_inout))) return r;
+  if (!c->step_exec && (r = capture(c->stream, [&] { return enqueue_step(c); }, &c->step_graph, &c->step_exec)))
+    return r;
+  c->hscal[pb::SCAL_DT] = *dt_inout;
+  c->hscal[pb::SCAL_NONFINITE] = 0.0;
+  HIPCHK(hipGraphLaunch(c->step_exec, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return parse_outputs(c, 1, c->cfg.rk_step, out, dt_inout);
+}
+
+// ---- ensemble of independent channels ---------------------------------------
+struct pb_ensemble {
+  std::vector<pb_ctx *> ch;
+  std::vector<int> order;  // capture / launch order of the channels (tuner)
+  // 0: one graph holding every channel's step; 1: each channel's own step
+  // captured without side-stream forks and launched on its own stream; 2: each
+  // channel's step enqueued directly (no graph) on its own streams
+  int mode = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_join;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static void ens_drop_graph(pb_ensemble *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+}
+
+int pb_ensemble_destroy(pb_ensemble *e) {
+  if (!e) return PB_OK;
+  ens_drop_graph(e);
+  for (pb_ctx *c : e->ch) pb_destroy(c);
+  for (hipEvent_t v : e->ev_join)
+    if (v) (void)hipEventDestroy(v);
+  if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return PB_OK;
+}
+
+int pb_ensemble_create(int32_t n, const pb_config *cfgs, pb_ensemble **out) {
+  if (!cfgs || !out || n < 1) return fail(PB_E_ARG, "pb_ensemble_create: %d channels", n);
+  *out = nullptr;
+  pb_ensemble *e = new pb_ensemble;
+  e->ch.assign(n, nullptr);
+  e->ev_join.assign(n, nullptr);
+  for (int i = 0; i < n; ++i) {
+    e->order.push_back(i);
+    const int r = pb_create(&cfgs[i], &e->ch[i]);
+    if (r) {
+      pb_ensemble_destroy(e);
+      return r;
+    }
+  }
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess) {
+    pb_ensemble_destroy(e);
+    return fail(PB_E_HIP, "ensemble stream / event");
+  }
+  for (int i = 0; i < n; ++i)
+    if (hipEventCreateWithFlags(&e->ev_join[i], hipEventDisableTiming) != hipSuccess) {
+      pb_ensemble_destroy(e);
+      return fail(PB_E_HIP, "ensemble event");
+    }
+  *out = e;
+  return PB_OK;
+}
+
+int pb_ensemble_channel(pb_ensemble *e, int32_t i, pb_ctx **ctx) {
+  if (!e || !ctx || i < 0 || i >= (int)e->ch.size()) return fail(PB_E_ARG, "pb_ensemble_channel: index %d", i);
+  *ctx = e->ch[i];
+  return PB_OK;
+}
+
+// every channel's step forked from the ensemble stream onto the channel's
+// own streams and joined back: one graph, one launch per ensemble step
+static int ens_enqueue(pb_ensemble *e) {
+  HIPCHK(hipEventRecord(e->ev_fork, e->stream));
+  for (int i : e->order) {
+    pb_ctx *c = e->ch[i];
+    HIPCHK(hipStreamWaitEvent(c->stream, e->ev_fork, 0));
+    const int r = enqueue_step(c);
+    if (r) return r;
+    HIPCHK(hipEventRecord(e->ev_join[i], c->stream));
+    HIPCHK(hipStreamWaitEvent(e->stream, e->ev_join[i], 0));
+  }
+  return PB_OK;
+}
+
+// capture what the mode needs (once)
+static int ens_prepare(pb_ensemble *e) {
+  int r;
+  if (e->mode == 0) {
+    if (!e->exec && (r = capture(e->stream, [&] { return ens_enqueue(e); }, &e->graph, &e->exec))) return r;
+  } else if (e->mode == 1) {
+    for (pb_ctx *c : e->ch)
+      if (!c->step_exec) {
+        c->linear = true;
+        r = capture(c->stream, [&] { return enqueue_step(c); }, &c->step_graph, &c->step_exec);
+        c->linear = false;
+        if (r) return r;
+      }
+  }
+  return PB_OK;
+}
+// launch one ensemble step and wait for it
+static int ens_launch(pb_ensemble *e) {
+  if (e->mode == 0) {
+    HIPCHK(hipGraphLaunch(e->exec, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+  } else if (e->mode == 1) {
+    for (int i : e->order) HIPCHK(hipGraphLaunch(e->ch[i]->step_exec, e->ch[i]->stream));
+    for (int i : e->order) HIPCHK(hipStreamSynchronize(e->ch[i]->stream));
+  } else {
+    for (int i : e->order) {
+      const int r = enqueue_step(e->ch[i]);
+      if (r) return r;
+    }
+    for (int i : e->order) HIPCHK(hipStreamSynchronize(e->ch[i]->stream));
+  }
+  return PB_OK;
+}
+
+int pb_ensemble_step(pb_ensemble *e, double *dt, pb_substep_out *out) {
+  if (!e || !dt) return fail(PB_E_ARG, "null argument");
+  int r;
+  for (size_t i = 0; i < e->ch.size(); ++i)
+    if ((r = step_checks(e->ch[i], &dt[i]))) return r;
+  if ((r = ens_prepare(e))) return r;
+  for (size_t i = 0; i < e->ch.size(); ++i) {
+    e->ch[i]->hscal[pb::SCAL_DT] = dt[i];
+    e->ch[i]->hscal[pb::SCAL_NONFINITE] = 0.0;
+  }
+  if ((r = ens_launch(e))) return r;
+  int first = PB_OK;
+  size_t off = 0;
+  for (size_t i = 0; i < e->ch.size(); ++i) {
+    pb_ctx *c = e->ch[i];
+    r = parse_outputs(c, 1, c->cfg.rk_step, out ? out + off : nullptr, &dt[i]);
+    off += c->cfg.rk_step;
+    if (r && !first) first = r;
+  }
+  return first;
+}
+
+// The tuner: per channel, the z-transform tile width (columns per block of
+// the per-block z kernels: fewer, longer rows per block or more blocks); for
+// the ensemble, the capture order of the channels.  Candidates
+// are timed as captured steps with HIP events, `reps` launches each, on the
+// fields the channels hold (the zero fields of a fresh ensemble: call it
+// before pb_set_state and pb_init_stores, which it leaves to the caller).
+// None of the knobs changes any arithmetic.  ms_out[i] (may be NULL)
+// receives the tuned step time of channel i, ms_out[n] that of the ensemble.
+static int time_graph(hipGraphExec_t x, hipStream_t st, int reps, float *ms) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    if (a) (void)hipEventDestroy(a);
+    return fail(PB_E_HIP, "tuner events");
+  }
+  hipError_t err = hipGraphLaunch(x, st);  // warm-up
+  if (err == hipSuccess) err = hipEventRecord(a, st);
+  for (int k = 0; k < reps && err == hipSuccess; ++k) err = hipGraphLaunch(x, st);
+  if (err == hipSuccess) err = hipEventRecord(b, st);
+  if (err == hipSuccess) err = hipEventSynchronize(b);
+  if (err == hipSuccess) err = hipEventElapsedTime(ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (err != hipSuccess) return fail(PB_E_HIP, "tuner timing: %s", hipGetErrorString(err));
+  *ms /= (float)reps;
+  return PB_OK;
+}
+
+int pb_ensemble_tune(pb_ensemble *e, int32_t reps, double *ms_out) {
+  if (!e || reps < 1) return fail(PB_E_ARG, "pb_ensemble_tune: reps %d", reps);
+  const int n = (int)e->ch.size();
+  int r;
+  for (int i = 0; i < n; ++i) {
+    pb_ctx *c = e->ch[i];
+    if (c->timing) return fail(PB_E_STATE, "pb_ensemble_tune: stage timing is on");
+    HIPCHK(hipMemsetAsync(c->U, 0, 9 * c->field * sizeof(double2), c->stream));
+    HIPCHK(hipMemsetAsync(c->RK, 0, 3 * c->field * sizeof(double2), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    c->hscal[pb::SCAL_DT] = 1e-3;
+    c->hscal[pb::SCAL_NONFINITE] = 0.0;
+    // candidate tile widths: the default and the halvings down to 1 (the
+    // streamed z transforms of 3nz/2 = 1152 have their own fixed tiles)
+    std::vector<int> bzs;
+    for (int b = c->Bz; b >= 1 && bzs.size() < 3; b /= 2) bzs.push_back(b);
+    if (c->ct_z_ok) bzs.assign(1, c->Bz);
+    float best = 0.f;
+    int best_bz = c->Bz;
+    for (int bz : bzs) {
+      drop_graph(c);
+      c->Bz = bz;
+      if ((r = capture(c->stream, [&] { return enqueue_step(c); }, &c->step_graph, &c->step_exec))) return r;
+      float ms = 0.f;
+      if ((r = time_graph(c->step_exec, c->stream, reps, &ms))) return r;
+      if (best == 0.f || ms < best) { best = ms; best_bz = bz; }
+    }
+    drop_graph(c);
+    c->Bz = best_bz;
+    if (ms_out) ms_out[i] = best;
+  }
+  // capture order: as given, largest channel first, smallest first
+  std::vector<int> by_size(n);
+  for (int i = 0; i < n; ++i) by_size[i] = i;
+  std::stable_sort(by_size.begin(), by_size.end(),
+                   [&](int a, int b) { return e->ch[a]->field > e->ch[b]->field; });
+  std::vector<std::vector<int>> orders = {e->order, by_size, std::vector<int>(by_size.rbegin(), by_size.rend())};
+  float best = 0.f;
+  std::vector<int> best_order = e->order;
+  int best_mode = 0;
+  for (int mode = 0; mode < 3; ++mode)
+    for (const auto &o : orders) {
+      ens_drop_graph(e);
+      for (pb_ctx *c : e->ch) drop_graph(c);
+      e->mode = mode;
+      e->order = o;
+      if ((r = ens_prepare(e))) return r;
+      if ((r = ens_launch(e))) return r;  // warm-up
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int k = 0; k < reps; ++k)
+        if ((r = ens_launch(e))) return r;
+      const float ms =
+          std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count() / (float)reps;
+      if (best == 0.f || ms < best) { best = ms; best_order = o; best_mode = mode; }
+    }
+  ens_drop_graph(e);
+  for (pb_ctx *c : e->ch) drop_graph(c);
+  e->order = best_order;
+  e->mode = best_mode;
+  for (pb_ctx *c : e->ch) c->have_state = false;
+  if (ms_out) ms_out[n] = best;
+  return PB_OK;
 }
 
 int pb_sync(pb_ctx *c) {

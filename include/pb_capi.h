@@ -171,8 +171,42 @@ int pb_statistics(pb_ctx *ctx, double *out);
 int pb_spectra(pb_ctx *ctx, double *ex, double *ez);
 /* rk_step substeps; out may be NULL or point to rk_step records. */
 int pb_step(pb_ctx *ctx, double *dt_inout, pb_substep_out *out);
+/* pb_step with the step's device work captured in a hipGraph on the first
+ * call (dt upload, the rk_step substeps with their side-stream forks and
+ * joins, the scalars back) and replayed by every later call: one launch and
+ * one synchronisation per step.  Same results as pb_step, bit for bit.  Not
+ * while stage timing is on (PB_E_STATE).  No counterpart in the reference
+ * (its time_march is host-driven). */
+int pb_step_graph(pb_ctx *ctx, double *dt_inout, pb_substep_out *out);
 /* Wait for the context's stream. */
 int pb_sync(pb_ctx *ctx);
+
+/* ---- ensembles of independent channels ---------------------------------- */
+/* n channels of uneven size and parameters (one pb_config each) advanced
+ * together: every channel's step is a captured graph (as pb_step_graph),
+ * either all of them forked from one stream and joined back inside one
+ * graph (one launch per ensemble step) or each channel's graph launched on
+ * its own stream, whichever pb_ensemble_tune found faster; one wait per
+ * ensemble step.  Each channel gives bit-identical
+ * results to stepping it alone.  Channels are pb_ctx handles owned by the
+ * ensemble (pb_ensemble_channel) for the state exchange; do not destroy
+ * them.  No counterpart in the reference (one channel per run). */
+typedef struct pb_ensemble pb_ensemble;
+int pb_ensemble_create(int32_t n, const pb_config *cfgs, pb_ensemble **out);
+int pb_ensemble_channel(pb_ensemble *e, int32_t i, pb_ctx **ctx);
+/* dt[n] in/out (one per channel); out: rk_step records per channel, channel
+ * after channel (or NULL).  The first error of any channel is returned. */
+int pb_ensemble_step(pb_ensemble *e, double *dt, pb_substep_out *out);
+/* The autotuner: times candidate schedules as captured steps (reps launches
+ * each) and keeps the fastest -- per channel the tile width of the per-block
+ * z transforms; for the ensemble one graph holding every channel versus each
+ * channel's own graph on its own stream, each with the channels as given,
+ * largest first or smallest first.  No candidate changes any
+ * arithmetic.  Call it before pb_set_state: it steps the channels on zeroed
+ * fields and leaves them without a state.  ms_out[n+1] (or NULL): the tuned
+ * step time of each channel, then of the ensemble. */
+int pb_ensemble_tune(pb_ensemble *e, int32_t reps, double *ms_out);
+int pb_ensemble_destroy(pb_ensemble *e);
 
 /* Per-stage device timing with HIP events recorded on the context's stream
  * around every kernel launch (no counterpart in the reference, whose phase

@@ -147,6 +147,47 @@ module poongback_gpu
        real(c_double), intent(out) :: ex(*), ez(*)
      end function pb_spectra
 
+     ! pb_step as a captured hipGraph (captured on the first call)
+     integer(c_int) function pb_step_graph(ctx, dt, out) bind(C, name="pb_step_graph")
+       import :: c_int, c_double, c_ptr, pb_substep_out
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: dt
+       type(pb_substep_out), intent(out) :: out(*)
+     end function pb_step_graph
+
+     ! ensembles: n channels in one captured graph per step
+     integer(c_int) function pb_ensemble_create(n, cfgs, ens) bind(C, name="pb_ensemble_create")
+       import :: c_int, c_int32_t, c_ptr, pb_config
+       integer(c_int32_t), value :: n
+       type(pb_config), intent(in) :: cfgs(*)
+       type(c_ptr), intent(out) :: ens
+     end function pb_ensemble_create
+
+     integer(c_int) function pb_ensemble_channel(ens, i, ctx) bind(C, name="pb_ensemble_channel")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ens
+       integer(c_int32_t), value :: i
+       type(c_ptr), intent(out) :: ctx
+     end function pb_ensemble_channel
+
+     integer(c_int) function pb_ensemble_step(ens, dt, out) bind(C, name="pb_ensemble_step")
+       import :: c_int, c_double, c_ptr, pb_substep_out
+       type(c_ptr), value :: ens
+       real(c_double), intent(inout) :: dt(*)
+       type(pb_substep_out), intent(out) :: out(*)
+     end function pb_ensemble_step
+
+     integer(c_int) function pb_ensemble_tune(ens, reps, ms_out) bind(C, name="pb_ensemble_tune")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ens, ms_out
+       integer(c_int32_t), value :: reps
+     end function pb_ensemble_tune
+
+     integer(c_int) function pb_ensemble_destroy(ens) bind(C, name="pb_ensemble_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ens
+     end function pb_ensemble_destroy
+
      integer(c_int) function pb_sync(ctx) bind(C, name="pb_sync")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
