@@ -69,6 +69,9 @@ SIGNATURES = {
     "pb_labs_mat_mul": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32, _dp, _dp]),
     "pb_labs_bs_noalloc": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
     "pb_step_graph": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(PbSubstepOut)]),
+    "pb_set_state_vg": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 6),
+    "pb_set_state_gphi": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 6),
+    "pb_get_checkpoint": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 9),
     "pb_ensemble_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(PbConfig), ctypes.POINTER(ctypes.c_void_p)]),
     "pb_ensemble_channel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     "pb_ensemble_step": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(PbSubstepOut)]),
@@ -162,6 +165,34 @@ class Channel:
 
     def init_stores(self):
         _check(load().pb_init_stores(self.h))
+
+    def _field(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        assert a.shape == self.shape(1)[1:]
+        return a
+
+    def set_state_vg(self, v, g, theta, u_mean, w_mean, t_mean):
+        """Restart from v, g, theta (nz, nx/2, ny) and the mean lines (ny)."""
+        f = [self._field(x) for x in (v, g, theta)]
+        m = [np.ascontiguousarray(x, dtype=np.float64) for x in (u_mean, w_mean, t_mean)]
+        _check(load().pb_set_state_vg(self.h, *[_ptr(x) for x in f], *[x.ctypes.data_as(_dp) for x in m]))
+
+    def set_state_gphi(self, g, phi, theta, u_mean, w_mean, t_mean):
+        """Restart from g, phi, theta (nz, nx/2, ny) and the mean lines (ny)."""
+        f = [self._field(x) for x in (g, phi, theta)]
+        m = [np.ascontiguousarray(x, dtype=np.float64) for x in (u_mean, w_mean, t_mean)]
+        _check(load().pb_set_state_gphi(self.h, *[_ptr(x) for x in f], *[x.ctypes.data_as(_dp) for x in m]))
+
+    def checkpoint(self):
+        """data_store's datasets: dict of u, v, w, theta, g, phi (nz, nx/2, ny)
+        and U_mean, W_mean, T_mean (ny)."""
+        shp = self.shape(1)[1:]
+        f = {k: np.zeros(shp, np.complex128) for k in ("u", "v", "w", "theta", "g", "phi")}
+        m = {k: np.zeros(self.cfg.ny) for k in ("U_mean", "W_mean", "T_mean")}
+        _check(load().pb_get_checkpoint(self.h, *[_ptr(f[k]) for k in ("u", "v", "w", "theta", "g", "phi")],
+                                        *[m[k].ctypes.data_as(_dp) for k in ("U_mean", "W_mean", "T_mean")]))
+        f.update(m)
+        return f
 
     def substep(self, s: int, dt: float):
         d = ctypes.c_double(dt)

@@ -520,6 +520,79 @@ int pb_init_stores(pb_ctx *c) {
   return PB_OK;
 }
 
+// ---- restart paths and the checkpoint (initial_step.f90:20-45, 237-513;
+// data_store.f90:92-118) --------------------------------------------------
+static const int kSlot0[1] = {0}, kSlot1[1] = {1}, kSlot2[1] = {2}, kSlot6[1] = {6};
+
+// the three mean lines (ny doubles each) into the (0,0) line of RK 0, 1, 2,
+// u_mean shifted by -u_gal (data_read_vg / data_read_gphi)
+static int upload_means(pb_ctx *c, const double *u_mean, const double *w_mean, const double *t_mean) {
+  std::vector<double2> col(c->ny);
+  const double *src[3] = {u_mean, w_mean, t_mean};
+  for (int f = 0; f < 3; ++f) {
+    for (int i = 0; i < c->ny; ++i) col[i] = make_double2(src[f][i] - (f == 0 ? c->cfg.u_gal : 0.0), 0.0);
+    HIPCHK(hipMemcpy2D(c->RK + (size_t)f * c->field, c->plane * sizeof(double2), col.data(), sizeof(double2),
+                       sizeof(double2), c->ny, hipMemcpyHostToDevice));
+  }
+  return PB_OK;
+}
+
+int pb_set_state_vg(pb_ctx *c, const double *v, const double *g, const double *theta, const double *u_mean,
+                    const double *w_mean, const double *t_mean) {
+  if (!c || !v || !g || !theta || !u_mean || !w_mean || !t_mean) return fail(PB_E_ARG, "null argument");
+  int r;
+  if ((r = upload_ref(c, v, 1, kSlot1, c->U))) return r;
+  if ((r = upload_ref(c, g, 1, kSlot0, c->RK))) return r;
+  if ((r = upload_ref(c, theta, 1, kSlot2, c->RK))) return r;
+  if ((r = upload_means(c, u_mean, w_mean, t_mean))) return r;
+  pb::LineParams P = c->line_params(1, 0.0);
+  hipLaunchKernelGGL(pb::init_vg_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P, (pb::cplx *)c->U,
+                     (pb::cplx *)c->RK, (pb::cplx *)c->buf, c->plane, c->nsys, c->nxh, c->nh, c->lmfxr, c->lmfzr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_state = true;
+  return PB_OK;
+}
+
+int pb_set_state_gphi(pb_ctx *c, const double *g, const double *phi, const double *theta, const double *u_mean,
+                      const double *w_mean, const double *t_mean) {
+  if (!c || !g || !phi || !theta || !u_mean || !w_mean || !t_mean) return fail(PB_E_ARG, "null argument");
+  int r;
+  if ((r = upload_ref(c, g, 1, kSlot0, c->RK))) return r;
+  if ((r = upload_ref(c, phi, 1, kSlot1, c->RK))) return r;
+  if ((r = upload_ref(c, theta, 1, kSlot2, c->RK))) return r;
+  if ((r = upload_means(c, u_mean, w_mean, t_mean))) return r;
+  pb::LineParams P = c->line_params(1, 0.0);
+  hipLaunchKernelGGL(pb::init_gphi_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P,
+                     (pb::cplx *)c->U, (const pb::cplx *)c->RK, (pb::cplx *)c->buf, c->dbuf, c->plane, c->nsys, c->nxh,
+                     c->nh, c->lmfxr, c->lmfzr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_state = true;
+  return PB_OK;
+}
+
+int pb_get_checkpoint(pb_ctx *c, double *u, double *v, double *w, double *theta, double *g, double *phi,
+                      double *u_mean, double *w_mean, double *t_mean) {
+  if (!c) return fail(PB_E_ARG, "null context");
+  if (!c->have_state) return fail(PB_E_STATE, "no state set");
+  int r;
+  double *fld[6] = {u, v, w, theta, g, phi};
+  const int *map[6] = {kSlot0, kSlot1, kSlot2, kSlot6, kSlot0, kSlot1};
+  for (int q = 0; q < 6; ++q)
+    if (fld[q] && (r = download_ref(c, fld[q], 1, map[q], q < 4 ? c->U : c->RK))) return r;
+  std::vector<double2> col(c->ny);
+  double *dst[3] = {u_mean, w_mean, t_mean};
+  const int slot[3] = {0, 2, 6};
+  for (int f = 0; f < 3; ++f) {
+    if (!dst[f]) continue;
+    HIPCHK(hipMemcpy2D(col.data(), sizeof(double2), c->U + (size_t)slot[f] * c->field, c->plane * sizeof(double2),
+                       sizeof(double2), c->ny, hipMemcpyDeviceToHost));
+    for (int i = 0; i < c->ny; ++i) dst[f][i] = col[i].x + (f == 0 ? c->cfg.u_gal : 0.0);
+  }
+  return PB_OK;
+}
+
 // ---- stage launchers ------------------------------------------------------
 static int launch_oppvd(pb_ctx *c, const double2 *state) {
   StageTimer tm(c, PB_STAGE_OPPVD);

@@ -1380,6 +1380,79 @@ __global__ void __launch_bounds__(64) init_stores_kernel(LineParams P, cplx *U, 
   for (int i = 0; i < n; ++i) r3[i] = th[i];
 }
 
+// Restart from v, g, theta and the mean lines (set_initial_data_vg,
+// initial_step.f90:301-353): v in slot 1, g, theta (and the means on the
+// (0,0) line) in RK 0 and 2, W mean in RK 1 at (0,0); u, w, theta -> slots
+// 0, 2, 6, phi -> RK 1.
+__global__ void __launch_bounds__(64) init_vg_kernel(LineParams P, cplx *U, cplx *RK, cplx *buf, size_t plane,
+                                                     size_t nsys, int nxh, int nh, double lmfxr, double lmfzr) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const int n = P.g.n;
+  const size_t field = plane * n;
+  const int k = (int)(sys % nxh), j = (int)(sys / nxh);
+  const int kzi = (j <= nh) ? j : j + 1 - 2 * nh - 1;
+  SV u{U + sys, plane}, v{U + field + sys, plane}, w{U + 2 * field + sys, plane}, th{U + 6 * field + sys, plane};
+  SV r1{RK + sys, plane}, r2{RK + field + sys, plane}, r3{RK + 2 * field + sys, plane};
+  SV d{buf + sys, plane}, f{buf + field + sys, plane};
+  if (k == 0 && kzi == 0) {
+    for (int i = 0; i < n; ++i) { u[i] = r1[i]; v[i] = mk(0.0, 0.0); w[i] = r2[i]; th[i] = r3[i]; }
+    return;
+  }
+  for (int i = 0; i < n; ++i) d[i] = v[i];
+  band_bs(P.g, P.D0LU, d);
+  band_mv(P.g, P.D1, d, f);
+  band_mv(P.g, P.D2, d, r2);
+  const double kxp = k * lmfxr, kzp = kzi * lmfzr;
+  const double k2 = kxp * kxp + kzp * kzp;
+  for (int i = 0; i < n; ++i) r2[i] = csub(r2[i], cscale(v[i], k2));
+  for (int i = 0; i < n; ++i) { u[i] = r1[i]; w[i] = f[i]; }
+  solve_u_w(n, u, w, kxp, kzp);
+  for (int i = 0; i < n; ++i) th[i] = r3[i];
+}
+
+// Restart from g, phi, theta and the mean lines (set_initial_data_gphi,
+// initial_step.f90:430-513): v from [D2 - k^2 D0] c = phi (the operator's own
+// diagonal on the wall rows, zero wall values), v = D0 c, f = D1 c, u and w
+// from g and f.  Db: per-system band scratch (W * ny doubles, stride plane).
+__global__ void __launch_bounds__(64) init_gphi_kernel(LineParams P, cplx *U, const cplx *RK, cplx *buf, double *dbuf,
+                                                       size_t plane, size_t nsys, int nxh, int nh, double lmfxr,
+                                                       double lmfzr) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const int n = P.g.n, W = P.g.W(), m = P.g.bw;
+  const size_t field = plane * n;
+  const int k = (int)(sys % nxh), j = (int)(sys / nxh);
+  const int kzi = (j <= nh) ? j : j + 1 - 2 * nh - 1;
+  SV u{U + sys, plane}, v{U + field + sys, plane}, w{U + 2 * field + sys, plane}, th{U + 6 * field + sys, plane};
+  const SV r1{const_cast<cplx *>(RK) + sys, plane}, r2{const_cast<cplx *>(RK) + field + sys, plane},
+      r3{const_cast<cplx *>(RK) + 2 * field + sys, plane};
+  SV cc{buf + sys, plane}, f{buf + field + sys, plane};
+  if (k == 0 && kzi == 0) {
+    for (int i = 0; i < n; ++i) { u[i] = r1[i]; v[i] = mk(0.0, 0.0); w[i] = r2[i]; th[i] = r3[i]; }
+    return;
+  }
+  const double kxp = k * lmfxr, kzp = kzi * lmfzr;
+  const double k2 = kzp * kzp + kxp * kxp;
+  const OwnBand Db{dbuf + sys, plane, W};
+  for (int q = 0; q < W; ++q) Db.at(0, q) = 0.0;
+  Db.at(0, m) = -k2 * P.D0(0, m) + P.D2(0, m);
+  for (int r = 1; r < n - 1; ++r)
+    for (int q = 0; q < W; ++q) Db.at(r, q) = -k2 * P.D0(r, q) + P.D2(r, q);
+  for (int q = 0; q < W; ++q) Db.at(n - 1, q) = 0.0;
+  Db.at(n - 1, m) = -k2 * P.D0(n - 1, m) + P.D2(n - 1, m);
+  band_lu(P.g, Db);
+  for (int i = 0; i < n; ++i) cc[i] = r2[i];
+  cc[0] = mk(0.0, 0.0);
+  cc[n - 1] = mk(0.0, 0.0);
+  band_bs(P.g, Db, cc);
+  band_mv(P.g, P.D0, cc, v);
+  band_mv(P.g, P.D1, cc, f);
+  for (int i = 0; i < n; ++i) { u[i] = r1[i]; w[i] = f[i]; }
+  solve_u_w(n, u, w, kxp, kzp);
+  for (int i = 0; i < n; ++i) th[i] = r3[i];
+}
+
 // ------------------------------------------- LABS band operations, batched
 // Packed bands (2bw+1, n) one after the other, Fortran order (slot fastest);
 // one thread per matrix / right-hand side.  Same functions as the solver.

@@ -128,6 +128,24 @@ int pb_get_state_slots(pb_ctx *ctx, double *u, double *v, double *w, double *the
 /* Fill the RK stores from u, v, w, theta already set by pb_set_state with
  * rk == NULL (set_initial_data_uvw, initial_step.f90:139-179). */
 int pb_init_stores(pb_ctx *ctx);
+/* The reference's other two restart kinds (initial_step.f90:20-45): fields
+ * (ny, nx/2, nz) complex as its restart file holds them, the mean lines as ny
+ * doubles with u_mean in the stored frame (u_gal is subtracted, as data_read_vg
+ * / data_read_gphi do).  From v, g, theta: u and w from continuity and g,
+ * the phi store = (D2 LU(D0)^-1 - k^2) v (set_initial_data_vg, 301-353).
+ * From g, phi, theta: v from [D2 - k^2 D0] c = phi with zero wall values,
+ * then u, w (set_initial_data_gphi, 430-513).  Both leave the stores set. */
+int pb_set_state_vg(pb_ctx *ctx, const double *v, const double *g, const double *theta, const double *u_mean,
+                    const double *w_mean, const double *t_mean);
+int pb_set_state_gphi(pb_ctx *ctx, const double *g, const double *phi, const double *theta,
+                      const double *u_mean, const double *w_mean, const double *t_mean);
+/* What data_store writes (data_store.f90:92-118): u, v, w, theta, g and phi
+ * (the stores: after a full step they hold g and phi of the state) as
+ * (ny, nx/2, nz) complex, and the mean lines U (+ u_gal), W, Theta as ny
+ * doubles.  Any pointer may be NULL.  The HDF5 writing stays with the caller
+ * (layout 0 of vel_step_*.h5 is this array order). */
+int pb_get_checkpoint(pb_ctx *ctx, double *u, double *v, double *w, double *theta, double *g, double *phi,
+                      double *u_mean, double *w_mean, double *t_mean);
 
 /* ---- the time step (device resident; no host<->device copies) ---------- */
 

@@ -96,6 +96,30 @@ module poongback_gpu
        type(c_ptr), value :: ctx
      end function pb_init_stores
 
+     ! restart kinds v/g and g/phi (initial_step.f90:20-45); means: ny doubles
+     integer(c_int) function pb_set_state_vg(ctx, v, g, theta, u_mean, w_mean, t_mean) &
+          bind(C, name="pb_set_state_vg")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: v(*), g(*), theta(*)
+       real(c_double), intent(in) :: u_mean(*), w_mean(*), t_mean(*)
+     end function pb_set_state_vg
+
+     integer(c_int) function pb_set_state_gphi(ctx, g, phi, theta, u_mean, w_mean, t_mean) &
+          bind(C, name="pb_set_state_gphi")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: g(*), phi(*), theta(*)
+       real(c_double), intent(in) :: u_mean(*), w_mean(*), t_mean(*)
+     end function pb_set_state_gphi
+
+     ! what data_store writes; every argument c_loc(...) or c_null_ptr
+     integer(c_int) function pb_get_checkpoint(ctx, u, v, w, theta, g, phi, u_mean, w_mean, t_mean) &
+          bind(C, name="pb_get_checkpoint")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, u, v, w, theta, g, phi, u_mean, w_mean, t_mean
+     end function pb_get_checkpoint
+
      integer(c_int) function pb_substep(ctx, s, dt, out) bind(C, name="pb_substep")
        import :: c_int, c_int32_t, c_double, c_ptr, pb_substep_out
        type(c_ptr), value :: ctx

@@ -1147,6 +1147,82 @@ int orc_init_stores(void *h, const double *Ud, double *RKd) {
   return 0;
 }
 
+// Restart from v, g, theta and the mean lines (set_initial_data_vg,
+// initial_step.f90:301-353).  In: S(ny,nxh,nz,4) slot 1 = v; RK slot 0 = g
+// (mean U - u_gal on kx = kz = 0), slot 1 = W mean on kx = kz = 0, slot 2 =
+// theta (mean Theta on kx = kz = 0).  Out: S slots 0, 2, 3 = u, w, theta,
+// RK slot 1 = phi = (D2 LU(D0)^-1 - k^2) v off the mean line.
+int orc_init_from_vg(void *h, double *Sd, double *RKd) {
+  Ctx *c = static_cast<Ctx *>(h);
+  cd *S = reinterpret_cast<cd *>(Sd);
+  cd *RK = reinterpret_cast<cd *>(RKd);
+  const int n = c->ny;
+#pragma omp parallel for schedule(static)
+  for (int l = 0; l < c->nxh * c->nz; ++l) {
+    const int k = l % c->nxh, j = l / c->nxh;
+    cd *u = S + ix(*c, 0, j, k, 0), *v = S + ix(*c, 1, j, k, 0), *w = S + ix(*c, 2, j, k, 0),
+       *th = S + ix(*c, 3, j, k, 0);
+    cd *r1 = RK + ix(*c, 0, j, k, 0), *r2 = RK + ix(*c, 1, j, k, 0), *r3 = RK + ix(*c, 2, j, k, 0);
+    if (c->kx[k] == 0 && c->kz[j] == 0) {
+      for (int i = 0; i < n; ++i) { u[i] = r1[i]; v[i] = C(0.0, 0.0); w[i] = r2[i]; th[i] = r3[i]; }
+      continue;
+    }
+    std::vector<cd> d(v, v + n), f(n);
+    labs_bs(c->d0lu.data(), n, c->bw, d.data());
+    labs_mv(c->d1.data(), n, c->bw, d.data(), f.data());
+    labs_mv(c->d2.data(), n, c->bw, d.data(), r2);
+    const double kxp = c->kx[k] * c->lmfxr, kzp = c->kz[j] * c->lmfzr;
+    const double k2 = kxp * kxp + kzp * kzp;
+    for (int i = 0; i < n; ++i) r2[i] = sub(r2[i], mulr(v[i], k2));
+    for (int i = 0; i < n; ++i) { u[i] = r1[i]; w[i] = f[i]; }
+    solve_u_w(n, u, w, kxp, kzp);  // g -> u, f -> w
+    for (int i = 0; i < n; ++i) th[i] = r3[i];
+  }
+  return 0;
+}
+
+// Restart from g, phi, theta and the mean lines (set_initial_data_gphi,
+// initial_step.f90:430-513).  In: RK slots 0, 1, 2 = g, phi, theta (the mean
+// lines on kx = kz = 0 as in orc_init_from_vg).  Out: S = u, v, w, theta:
+// v from [D2 - k^2 D0] c = phi with the operator's own diagonal on the wall
+// rows and zero wall values, v = D0 c, f = D1 c, u and w from g and f.
+int orc_init_from_gphi(void *h, double *Sd, double *RKd) {
+  Ctx *c = static_cast<Ctx *>(h);
+  cd *S = reinterpret_cast<cd *>(Sd);
+  const cd *RK = reinterpret_cast<const cd *>(RKd);
+  const int n = c->ny, W = c->W, m = c->bw;
+#pragma omp parallel for schedule(static)
+  for (int l = 0; l < c->nxh * c->nz; ++l) {
+    const int k = l % c->nxh, j = l / c->nxh;
+    cd *u = S + ix(*c, 0, j, k, 0), *v = S + ix(*c, 1, j, k, 0), *w = S + ix(*c, 2, j, k, 0),
+       *th = S + ix(*c, 3, j, k, 0);
+    const cd *r1 = RK + ix(*c, 0, j, k, 0), *r2 = RK + ix(*c, 1, j, k, 0), *r3 = RK + ix(*c, 2, j, k, 0);
+    if (c->kx[k] == 0 && c->kz[j] == 0) {
+      for (int i = 0; i < n; ++i) { u[i] = r1[i]; v[i] = C(0.0, 0.0); w[i] = r2[i]; th[i] = r3[i]; }
+      continue;
+    }
+    const double kxp = c->kx[k] * c->lmfxr, kzp = c->kz[j] * c->lmfzr;
+    const double k2 = kzp * kzp + kxp * kxp;
+    std::vector<double> Db((size_t)n * W, 0.0);
+    Db[m] = -k2 * c->d0[m] + c->d2[m];
+    for (int r = 1; r < n - 1; ++r)
+      for (int q = 0; q < W; ++q) Db[(size_t)r * W + q] = -k2 * c->d0[(size_t)r * W + q] + c->d2[(size_t)r * W + q];
+    const size_t e = (size_t)(n - 1) * W;
+    Db[e + m] = -k2 * c->d0[e + m] + c->d2[e + m];
+    labs_lu(Db.data(), n, c->bw);
+    std::vector<cd> cc(r2, r2 + n), f(n);
+    cc[0] = C(0.0, 0.0);
+    cc[n - 1] = C(0.0, 0.0);
+    labs_bs(Db.data(), n, c->bw, cc.data());
+    labs_mv(c->d0.data(), n, c->bw, cc.data(), v);
+    labs_mv(c->d1.data(), n, c->bw, cc.data(), f.data());
+    for (int i = 0; i < n; ++i) { u[i] = r1[i]; w[i] = f[i]; }
+    solve_u_w(n, u, w, kxp, kzp);
+    for (int i = 0; i < n; ++i) th[i] = r3[i];
+  }
+  return 0;
+}
+
 // Strided view of a field set: element (slot, kz index j, kx index k, y).
 struct View {
   cd *p;

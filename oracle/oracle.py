@@ -56,6 +56,8 @@ def lib():
         L.orc_band_matvec.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_fft.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
         L.orc_hvhg.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_init_from_vg.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_init_from_gphi.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_set_mutant.argtypes = [ctypes.c_int]
         L.orc_set_mutant.restype = None
         _lib = L
@@ -170,6 +172,32 @@ class Oracle:
         RK = np.zeros((3, cfg.nz, cfg.nxh, cfg.ny), dtype=np.complex128)
         lib().orc_init_stores(self.h, S.ctypes.data_as(_dp), RK.ctypes.data_as(_dp))
         return RK
+
+    def init_from_vg(self, v, g, theta, u_mean, w_mean, t_mean):
+        """Restart from v, g, theta and the mean lines (set_initial_data_vg):
+        returns (S, RK).  u_mean is the stored U (with u_gal added)."""
+        cfg = self.cfg
+        S = np.zeros((4, cfg.nz, cfg.nxh, cfg.ny), np.complex128)
+        RK = np.zeros((3, cfg.nz, cfg.nxh, cfg.ny), np.complex128)
+        S[1], RK[0], RK[2] = v, g, theta
+        RK[0, 0, 0] = np.asarray(u_mean) - cfg.u_gal
+        RK[1, 0, 0] = w_mean
+        RK[2, 0, 0] = t_mean
+        lib().orc_init_from_vg(self.h, S.ctypes.data_as(_dp), RK.ctypes.data_as(_dp))
+        return S, RK
+
+    def init_from_gphi(self, g, phi, theta, u_mean, w_mean, t_mean):
+        """Restart from g, phi, theta and the mean lines
+        (set_initial_data_gphi): returns (S, RK)."""
+        cfg = self.cfg
+        S = np.zeros((4, cfg.nz, cfg.nxh, cfg.ny), np.complex128)
+        RK = np.zeros((3, cfg.nz, cfg.nxh, cfg.ny), np.complex128)
+        RK[0], RK[1], RK[2] = g, phi, theta
+        RK[0, 0, 0] = np.asarray(u_mean) - cfg.u_gal
+        RK[1, 0, 0] = w_mean
+        RK[2, 0, 0] = t_mean
+        lib().orc_init_from_gphi(self.h, S.ctypes.data_as(_dp), RK.ctypes.data_as(_dp))
+        return S, RK
 
     def nonlinear(self, S: np.ndarray, want_vv: bool = False, y0: int = 0, y1: int | None = None):
         cfg = self.cfg
