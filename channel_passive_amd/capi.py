@@ -70,6 +70,8 @@ SIGNATURES = {
     "pb_labs_bs_noalloc": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
     "pb_step_graph": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(PbSubstepOut)]),
     "pb_set_state_vg": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 6),
+    "pb_substep_stats2": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.POINTER(PbSubstepOut), _dp,
+                                         _dp, _dp]),
     "pb_set_state_gphi": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 6),
     "pb_get_checkpoint": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 9),
     "pb_ensemble_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(PbConfig), ctypes.POINTER(ctypes.c_void_p)]),
@@ -210,6 +212,22 @@ class Channel:
         _check(load().pb_substep_stats(self.h, s, ctypes.byref(d), ctypes.byref(o), _ptr(p),
                                        vv.ctypes.data_as(_dp)))
         return o.as_dict(), p, vv
+
+    STAT2_NAMES = ("uuu", "vvv", "www", "uuuu", "vvvv", "wwww", "u6", "v6", "w6", "u8", "v8", "w8", "u10", "v10",
+                   "w10", "-duuvdy", "-dvvvdy", "-dvwwdy", "-duvvdy", "up", "-dpudy", "vp", "-2dpvdy", "wp", "pp",
+                   "2pdudx", "2pdvdy", "2pdwdz", "pdudy+pdvdx")
+
+    def substep_stats2(self, dt: float):
+        """Substep 1 with get_statistics_1's pressure and get_statistics_2:
+        returns (out, p, <vv>, dict name -> ny line)."""
+        d = ctypes.c_double(dt)
+        o = PbSubstepOut()
+        p = np.zeros(self.shape(1)[1:], np.complex128)
+        vv = np.zeros(self.cfg.ny)
+        st = np.zeros((29, self.cfg.ny))
+        _check(load().pb_substep_stats2(self.h, 1, ctypes.byref(d), ctypes.byref(o), _ptr(p),
+                                        vv.ctypes.data_as(_dp), st.ctypes.data_as(_dp)))
+        return o.as_dict(), p, vv, {k: st[i] for i, k in enumerate(self.STAT2_NAMES)}
 
     STAT_NAMES = ("U", "W", "Theta", "dUdy", "dWdy", "dTdy", "uu", "vv", "ww", "tt", "uv", "uw", "vw", "ut",
                   "wt", "vt")

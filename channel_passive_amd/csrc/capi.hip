@@ -87,6 +87,13 @@ struct pb_ctx {
   // must run beside other graphs (a graph's own branches share the
   // runtime's internal streams with every other graph's)
   bool linear = false;
+  // get_statistics_2 (allocated on its first use): copies of u, v, w of the
+  // statistics substep's input, the moment partials of the x pass (set in
+  // mom_active while that pass runs), their sums, the Parseval sums, the 29
+  // output lines and two ny-long complex scratch columns
+  double2 *stat_uvw = nullptr;
+  double *dmom = nullptr, *dmsum = nullptr, *dpv = nullptr, *dstat2 = nullptr, *mom_active = nullptr;
+  double2 *dstat2_tmp = nullptr;
   double *dbuf = nullptr, *dbuf2 = nullptr, *rbuf = nullptr, *vpart = nullptr, *scal = nullptr;
   double2 *ppe_bc = nullptr;  // statistics time: pressure wall values [2][kz][kx]
   double *dcenter = nullptr;  // B_k(0): centre-line values of the basis (get_statistics.f90:566-570)
@@ -153,7 +160,8 @@ static int ctx_free(pb_ctx *c) {
   if (!c) return 0;
   void *ptrs[] = {c->U, c->RK, c->pad, c->buf, c->vd, c->dbuf, c->dbuf2, c->rbuf, c->vpart, c->scal, c->dD0, c->dD1, c->dD2,
                   c->dD0LU, c->dinteg, c->ddyr, c->drows, c->twz, c->twx, c->slotmap, c->mean_db,
-                  c->ppe_bc, c->dvv, c->dcenter, c->dstat, c->usrc};
+                  c->ppe_bc, c->dvv, c->dcenter, c->dstat, c->usrc, c->stat_uvw, c->dmom, c->dmsum, c->dpv,
+                  c->dstat2, c->dstat2_tmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -625,7 +633,7 @@ static int launch_xphys_t(pb_ctx *c, int want_vv) {
   const size_t ldsx = F::lds_elems(c->nx32, 2) * sizeof(double2);
   StageTimer tm(c, PB_STAGE_XPHYS);
   hipLaunchKernelGGL(pb::xphys_kernel<F>, dim3(c->nz32, c->ny), dim3(F::T), ldsx, c->stream, c->pad, c->px, c->twx,
-                     c->nxh, c->nz32, c->pfield, c->ddyr, c->dxr, c->dzr, c->vpart, want_vv ? 3 : 2);
+                     c->nxh, c->nz32, c->pfield, c->ddyr, c->dxr, c->dzr, c->vpart, want_vv ? 3 : 2, c->mom_active);
   c->vpart_n = (size_t)c->ny * c->nz32 * ((F::T + 63) / 64);
   HIPCHK(hipGetLastError());
   return PB_OK;
@@ -948,6 +956,69 @@ int pb_substep_stats(pb_ctx *c, int32_t s, double *dt_inout, pb_substep_out *out
     HIPCHK(hipMemcpyAsync(vv_out, c->dvv, c->ny * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
+  if ((r = launch_wavespace(c, s))) return r;
+  return read_outputs(c, s, s, out, dt_inout);
+}
+
+static constexpr int kMaxWavesPerLine = 16;  // the x pass's blocks have at most 16 waves
+
+int pb_substep_stats2(pb_ctx *c, int32_t s, double *dt_inout, pb_substep_out *out, double *p_out, double *vv_out,
+                      double *stat2) {
+  if (!stat2) return pb_substep_stats(c, s, dt_inout, out, p_out, vv_out);
+  if (!c || !dt_inout) return fail(PB_E_ARG, "null argument");
+  if (s != 1) return fail(PB_E_ARG, "pb_substep_stats2: statistics are taken on substep 1, not %d", s);
+  if (!c->have_state) return fail(PB_E_STATE, "pb_substep_stats2 before pb_set_state");
+  if (c->bw < 3 || c->bw > 5) return fail(PB_E_ARG, "pb_substep_stats2: bandwidth %d outside 3..5", c->bw);
+  int r = check_dt(*dt_inout);
+  if (r && c->cfg.dt_factor == 1.0) return r;
+  if (!c->stat_uvw) {
+    auto alloc = [&](void **p, size_t bytes) -> int {
+      const hipError_t e = hipMalloc(p, bytes);
+      if (e != hipSuccess) return fail(PB_E_HIP, "hipMalloc(%zu) for statistics: %s", bytes, hipGetErrorString(e));
+      c->bytes += bytes;
+      return PB_OK;
+    };
+    if ((r = alloc((void **)&c->stat_uvw, 3 * c->field * sizeof(double2))) ||
+        (r = alloc((void **)&c->dmom, (size_t)pb::NMOM * c->ny * c->nz32 * kMaxWavesPerLine * sizeof(double))) ||
+        (r = alloc((void **)&c->dmsum, (size_t)pb::NMOM * c->ny * sizeof(double))) ||
+        (r = alloc((void **)&c->dpv, (size_t)8 * c->ny * sizeof(double))) ||
+        (r = alloc((void **)&c->dstat2, (size_t)PB_NSTAT2_LINES * c->ny * sizeof(double))) ||
+        (r = alloc((void **)&c->dstat2_tmp, (size_t)2 * c->ny * sizeof(double2))))
+      return r;
+  }
+  if ((r = upload_dt(c, *dt_inout))) return r;
+  // the substep's input u, v, w, kept for the correlations with p
+  HIPCHK(hipMemcpyAsync(c->stat_uvw, c->U, 3 * c->field * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+  c->mom_active = c->dmom;
+  r = substep_front(c, s, true);
+  c->mom_active = nullptr;
+  if (r) return r;
+  if (c->vpart_n > (size_t)c->ny * c->nz32 * kMaxWavesPerLine) return fail(PB_E_ARG, "x pass: too many waves");
+  const pb::LineParams P = c->line_params(1, 0.0);
+  const dim3 b64(64), grid(blocks_for(c->plane, 64));
+  pb::cplx *buf = (pb::cplx *)c->buf;
+  switch (c->bw) {
+    case 3: hipLaunchKernelGGL(pb::stats2_deriv_kernel<3>, grid, b64, 0, c->stream, P, (const pb::cplx *)c->stat_uvw, buf, c->field, c->plane); break;
+    case 4: hipLaunchKernelGGL(pb::stats2_deriv_kernel<4>, grid, b64, 0, c->stream, P, (const pb::cplx *)c->stat_uvw, buf, c->field, c->plane); break;
+    case 5: hipLaunchKernelGGL(pb::stats2_deriv_kernel<5>, grid, b64, 0, c->stream, P, (const pb::cplx *)c->stat_uvw, buf, c->field, c->plane); break;
+  }
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(pb::stats2_mom_kernel, dim3(c->ny, pb::NMOM), dim3(256), 0, c->stream, (const double *)c->dmom,
+                     c->vpart_n / c->ny, c->dmsum);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(pb::stats2_pv_kernel, dim3(blocks_for(c->ny, 64)), dim3(64), 0, c->stream,
+                     (const pb::cplx *)c->stat_uvw, (const pb::cplx *)(c->U + 5 * c->field), (const pb::cplx *)buf,
+                     c->plane, c->ny, c->nxh, c->nh, c->lmfxr, c->lmfzr, c->dpv);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(pb::stats2_final_kernel, dim3(1), dim3(64), 0, c->stream, P, (const double *)c->dmsum,
+                     (const double *)c->dpv, c->rescale, (pb::cplx *)c->dstat2_tmp, (pb::cplx *)c->dstat2_tmp + c->ny,
+                     c->dstat2);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(stat2, c->dstat2, (size_t)PB_NSTAT2_LINES * c->ny * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  if (p_out && (r = download_ref(c, p_out, 1, kSlot5, c->U))) return r;
+  if (vv_out) HIPCHK(hipMemcpyAsync(vv_out, c->dvv, c->ny * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   if ((r = launch_wavespace(c, s))) return r;
   return read_outputs(c, s, s, out, dt_inout);
 }

@@ -504,6 +504,22 @@ __global__ void __launch_bounds__(F::T, F::MINW) zinv_kernel(const double2 *__re
   }
 }
 
+// Statistics-time moments of the physical velocities at one point
+// (dft_x_realspace.f90:128-152), the order of its accumulators 3..21:
+// u^3 v^3 w^3, u^4 v^4 w^4, -u^2 v, -v w^2, -v^3, -u v^2, u^6 v^6 w^6,
+// u^8 v^8 w^8, u^10 v^10 w^10.
+constexpr int NMOM = 19;
+__device__ __forceinline__ void add_moments(double (&acc)[NMOM], double u, double v, double w) {
+  const double u2 = u * u, v2 = v * v, w2 = w * w;
+  acc[0] += u * u * u; acc[1] += v * v * v; acc[2] += w * w * w;
+  acc[3] += u * u * u * u; acc[4] += v * v * v * v; acc[5] += w * w * w * w;
+  acc[6] -= u * u * v; acc[7] -= v * w * w; acc[8] -= v * v * v; acc[9] -= u * v * v;
+  const double u4 = u2 * u2, v4 = v2 * v2, w4 = w2 * w2;
+  acc[10] += u4 * u2; acc[11] += v4 * v2; acc[12] += w4 * w2;
+  acc[13] += u4 * u4; acc[14] += v4 * v4; acc[15] += w4 * w4;
+  acc[16] += u4 * u4 * u2; acc[17] += v4 * v4 * v2; acc[18] += w4 * w4 * w2;
+}
+
 // ---------------------------------------------------------------- x physical
 // grid: (nz32, ny): one x line per block.  pad fields 0..3 = u,v,w,theta in,
 // 0..7 = spectra of u^2-v^2, uv, w^2-v^2, vw, uw, u theta, v theta, w theta out.
@@ -512,7 +528,7 @@ __global__ void __launch_bounds__(F::T, F::MINW) xphys_kernel(double2 *__restric
                                                             const double2 *__restrict__ tw, int nxh, int nz32,
                                                             size_t pfield, const double *__restrict__ dyr,
                                                             double dxr, double dzr, double *__restrict__ vpart,
-                                                            int nround) {
+                                                            int nround, double *__restrict__ mom) {
   extern __shared__ double2 lds[];  // 2 lines of N, layout F::at
   const int z = blockIdx.x, y = blockIdx.y, N = F::N ? F::N : P.N;
   double2 *line = pad + (size_t)y * nz32 * nxh + (size_t)z * nxh;
@@ -558,6 +574,24 @@ __global__ void __launch_bounds__(F::T, F::MINW) xphys_kernel(double2 *__restric
   for (int d = 32; d > 0; d >>= 1) vmax = nan_max(vmax, __shfl_xor(vmax, d, 64));
   constexpr int NW = (F::T + 63) / 64;
   if ((threadIdx.x & 63) == 0) vpart[((size_t)y * nz32 + z) * NW + threadIdx.x / 64] = vmax;
+  // statistics time: the 19 physical-space moments of this line, one partial
+  // sum per wave: mom[((q * ny + y) * nz32 + z) * NW + wave]
+  if (mom) {
+    double acc[NMOM];
+#pragma unroll
+    for (int k = 0; k < NMOM; ++k) acc[k] = 0.0;
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q)
+      if (threadIdx.x + q * F::T < N) add_moments(acc, pu[q], pv[q], pw[q]);
+#pragma unroll
+    for (int k = 0; k < NMOM; ++k) {
+      double a = acc[k];
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) a += __shfl_xor(a, d, 64);
+      if ((threadIdx.x & 63) == 0)
+        mom[(((size_t)k * gridDim.y + y) * nz32 + z) * NW + threadIdx.x / 64] = a;
+    }
+  }
   // two pairs of products per round: (P1 + i P2, P3 + i P4), (P5 + i P7, P8 + i P9);
   // a third round (statistics time) transforms v*v alone into pad field 8
   for (int round = 0; round < nround; ++round) {
@@ -1218,6 +1252,111 @@ __global__ void __launch_bounds__(64) deriv_kernel(LineParams P, cplx *U, cplx *
   deriv_line<BW>(P, SV{U + sys, plane}, SV{U + 2 * field + sys, plane}, SV{buf + sys, plane},
                  SV{buf + field + sys, plane});
 }
+// ------------------------------------------------- get_statistics_2
+// moment sums per (q, y): grid (ny, NMOM), the partials of all lines and waves
+__global__ void __launch_bounds__(256) stats2_mom_kernel(const double *mom, size_t per_y, double *msum) {
+  __shared__ double red[256];
+  const int y = blockIdx.x, q = blockIdx.y;
+  const double *src = mom + ((size_t)q * gridDim.x + y) * per_y;
+  double a = 0.0;
+  for (size_t i = threadIdx.x; i < per_y; i += 256) a += src[i];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) msum[(size_t)q * gridDim.x + y] = red[0];
+}
+// du/dy and dv/dy of the statistics copies of u and v (get_turb_transport_PI1_ij's
+// labs_bs + labs_mat_mul) into the scratch columns
+template <int BW>
+__global__ void __launch_bounds__(64) stats2_deriv_kernel(LineParams P, const cplx *uvw, cplx *buf, size_t field,
+                                                          size_t plane) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= plane) return;
+  deriv_line<BW>(P, SV{const_cast<cplx *>(uvw) + sys, plane}, SV{const_cast<cplx *>(uvw) + field + sys, plane},
+                 SV{buf + sys, plane}, SV{buf + field + sys, plane});
+}
+// Parseval sums per y, serial over the (kx, kz) lines in the reference's order
+// (get_uv, get_statistics.f90:982-1017; get_turb_transport_PI1_ij, 817-902):
+// out[0..7][y] = up, vp, wp, pp, 2pdudx, 2pdvdy, 2pdwdz, pdudy+pdvdx
+__global__ void __launch_bounds__(64) stats2_pv_kernel(const cplx *uvw, const cplx *p, const cplx *d, size_t plane,
+                                                       int ny, int nxh, int nh, double lmfxr, double lmfzr,
+                                                       double *out) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= ny) return;
+  const size_t field = plane * ny, row = (size_t)y * plane;
+  const cplx *u = uvw + row, *v = uvw + field + row, *w = uvw + 2 * field + row, *pr = p + row;
+  const cplx *du = d + row, *dv = d + field + row;
+  out[0 * ny + y] = mean_uv_row(u, pr, nxh, plane);
+  out[1 * ny + y] = mean_uv_row(v, pr, nxh, plane);
+  out[2 * ny + y] = mean_uv_row(w, pr, nxh, plane);
+  out[3 * ny + y] = mean_uv_row(pr, pr, nxh, plane);
+  double puu = 0.0, pvv = 0.0, pww = 0.0, puv = 0.0;
+  for (size_t l = 0; l < plane; ++l) {
+    const int k = (int)(l % nxh), j = (int)(l / nxh);
+    const int kzi = (j <= nh) ? j : j + 1 - 2 * nh - 1;
+    const double kxw = lmfxr * k, kzw = lmfzr * kzi;
+    const cplx a = u[l], b = v[l], c = w[l], q = pr[l];
+    if (k == 0) {
+      puu = puu + 2.0 * kxw * (a.re * q.im - a.im * q.re);
+      puv = puv + kxw * (b.re * q.im - b.im * q.re);
+      pww = pww + 2.0 * kzw * (c.re * q.im - c.im * q.re);
+    } else {
+      puu = puu + 4.0 * kxw * (a.re * q.im - a.im * q.re);
+      puv = puv + 2.0 * kxw * (b.re * q.im - b.im * q.re);
+      pww = pww + 4.0 * kzw * (c.re * q.im - c.im * q.re);
+    }
+  }
+  for (size_t l = 0; l < plane; ++l) {
+    const cplx a = du[l], b = dv[l], q = pr[l];
+    if (l % nxh == 0) {
+      pvv = pvv + 2.0 * (b.re * q.re + b.im * q.im);
+      puv = puv + (a.re * q.re + a.im * q.im);
+    } else {
+      pvv = pvv + 4.0 * (b.re * q.re + b.im * q.im);
+      puv = puv + 2.0 * (a.re * q.re + a.im * q.im);
+    }
+  }
+  out[4 * ny + y] = puu;
+  out[5 * ny + y] = pvv;
+  out[6 * ny + y] = pww;
+  out[7 * ny + y] = puv;
+}
+// the 29 datasets of get_statistics_2 from the moment sums and the Parseval
+// sums (one thread: a few ny-long band solves)
+__global__ void stats2_final_kernel(LineParams P, const double *msum, const double *pv, double rescale, cplx *tmp,
+                                    cplx *tmp2, double *out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int n = P.g.n;
+  const int mom_of[15] = {0, 1, 2, 3, 4, 5, 10, 11, 12, 13, 14, 15, 16, 17, 18};
+  for (int q = 0; q < 15; ++q) {
+    double *o = out + (size_t)q * n;
+    o[0] = 0.0;
+    for (int i = 1; i < n - 1; ++i) o[i] = rescale * msum[(size_t)mom_of[q] * n + i];
+    o[n - 1] = 0.0;
+  }
+  const SV a{tmp, 1}, b{tmp2, 1};
+  auto ddy = [&](const double *x, double scale_in, double *o, double scale_out) {
+    for (int i = 0; i < n; ++i) a[i] = mk(x[i] * scale_in, 0.0);
+    band_bs(P.g, P.D0LU, a);
+    band_mv(P.g, P.D1, a, b);
+    for (int i = 0; i < n; ++i) o[i] = scale_out * b[i].re;
+  };
+  const int tri[4] = {6, 8, 7, 9};
+  for (int q = 0; q < 4; ++q) ddy(msum + (size_t)tri[q] * n, rescale, out + (size_t)(15 + q) * n, 1.0);
+  for (int i = 0; i < n; ++i) {
+    out[(size_t)19 * n + i] = pv[i];
+    out[(size_t)21 * n + i] = pv[(size_t)1 * n + i];
+    out[(size_t)23 * n + i] = pv[(size_t)2 * n + i];
+    out[(size_t)24 * n + i] = pv[(size_t)3 * n + i];
+    for (int q = 0; q < 4; ++q) out[(size_t)(25 + q) * n + i] = pv[(size_t)(4 + q) * n + i];
+  }
+  ddy(pv, 1.0, out + (size_t)20 * n, -1.0);
+  ddy(pv + n, 1.0, out + (size_t)22 * n, -2.0);
+}
+
 // vorticity into slots 3, 4, 5 (get_vorticity_field, get_statistics.f90:912-948):
 // ox = dw/dy - (0,kz) v, oy = (0,kz) u - (0,kx) w, oz = (0,kx) v - du/dy with (0,k) = dcmplx(0d0,k)
 __global__ void vort_kernel(cplx *U, const cplx *buf, size_t field, size_t plane, int nxh, int nh, double lmfxr,

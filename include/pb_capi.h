@@ -165,6 +165,19 @@ int pb_substep(pb_ctx *ctx, int32_t s, double *dt_inout, pb_substep_out *out);
  * exactly as with pb_substep.  For s != 1 it is pb_substep. */
 int pb_substep_stats(pb_ctx *ctx, int32_t s, double *dt_inout, pb_substep_out *out, double *p_out,
                      double *vv_out);
+/* pb_substep_stats plus get_statistics_2 (get_statistics.f90:376-539): stat2
+ * receives PB_NSTAT2_LINES lines of ny doubles, the reference's datasets in
+ * its order: uuu vvv www, uuuu vvvv wwww, u6 v6 w6, u8 v8 w8, u10 v10 w10
+ * (physical-space averages of the raw powers over the 3/2-padded grid,
+ * zero on the wall rows; dft_x_realspace.f90:128-484), -duuvdy -dvvvdy
+ * -dvwwdy -duvvdy (D1 LU(D0)^-1 of the averages of -uuv, -vvv, -vww, -uvv),
+ * up, -dpudy, vp, -2dpvdy, wp, pp (get_uv with the pressure), 2pdudx,
+ * 2pdvdy, 2pdwdz, pdudy+pdvdx (get_turb_transport_PI1_ij, 817-902).
+ * Substep 1 only; stat2 == NULL makes it pb_substep_stats.  Allocates its
+ * buffers (3 fields) on the first call. */
+#define PB_NSTAT2_LINES 29
+int pb_substep_stats2(pb_ctx *ctx, int32_t s, double *dt_inout, pb_substep_out *out, double *p_out,
+                      double *vv_out, double *stat2);
 /* Profiles of get_statistics_1 (get_statistics.f90:54-155) from the state
  * held between steps: out receives 16 lines of ny doubles, in this order:
  * U (+u_gal), W, Theta (get_u, 1024-1048); dU/dy, dW/dy, dTheta/dy

@@ -138,6 +138,16 @@ module poongback_gpu
        type(c_ptr), value :: p, vv
      end function pb_substep_stats
 
+     ! pb_substep_stats + get_statistics_2 (29 lines of ny); p, vv, stat2: c_loc or c_null_ptr
+     integer(c_int) function pb_substep_stats2(ctx, s, dt, out, p, vv, stat2) bind(C, name="pb_substep_stats2")
+       import :: c_int, c_int32_t, c_double, c_ptr, pb_substep_out
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: s
+       real(c_double), intent(inout) :: dt
+       type(pb_substep_out), intent(out) :: out
+       type(c_ptr), value :: p, vv, stat2
+     end function pb_substep_stats2
+
      ! the three substeps of main.f90:134-156; out(rk_step)
      integer(c_int) function pb_step(ctx, dt, out) bind(C, name="pb_step")
        import :: c_int, c_double, c_ptr, pb_substep_out

@@ -56,6 +56,7 @@ namespace {
 //   7 Helmholtz operator (1 - b k^2) D0 - b D2
 //   8 wave00: + c d(uv)/dy in place of - c d(uv)/dy in the mean-U right-hand side
 //   9 mixed-type scalar source with the wrong sign, -(nu/pr) u / bulk_u
+//  10 statistics: <2p du/dx> with the wrong sign on kx > 0
 static int g_mutant = 0;
 
 // ----------------------------------------------------------------- complex
@@ -1232,8 +1233,24 @@ struct View {
 
 // Nonlinear part of the substep for one y plane: y_to_x (x4),
 // dft_x_realspace and x_to_y (9 or 10 fields) and rescale_data.
+// Statistics-time moments of the physical velocities at one x point
+// (dft_x_realspace.f90:128-152), in the order of its accumulators 3..21:
+// u^3 v^3 w^3, u^4 v^4 w^4, -u^2 v, -v w^2, -v^3, -u v^2, u^6 v^6 w^6,
+// u^8 v^8 w^8, u^10 v^10 w^10 (raw moments, the means included).
+constexpr int kNMom = 19;
+static inline void add_moments(double *acc, double u, double v, double w) {
+  const double u2 = u * u, v2 = v * v, w2 = w * w;
+  acc[0] += u * u * u; acc[1] += v * v * v; acc[2] += w * w * w;
+  acc[3] += u * u * u * u; acc[4] += v * v * v * v; acc[5] += w * w * w * w;
+  acc[6] -= u * u * v; acc[7] -= v * w * w; acc[8] -= v * v * v; acc[9] -= u * v * v;
+  const double u4 = u2 * u2, v4 = v2 * v2, w4 = w2 * w2;
+  acc[10] += u4 * u2; acc[11] += v4 * v2; acc[12] += w4 * w2;
+  acc[13] += u4 * u4; acc[14] += v4 * v4; acc[15] += w4 * w4;
+  acc[16] += u4 * u4 * u2; acc[17] += v4 * v4 * v2; acc[18] += w4 * w4 * w2;
+}
+
 static double nonlinear_plane(const Ctx *c, const View &S, const View &P, int want_vv, int y,
-                              std::vector<cd> &G, std::vector<cd> &H) {
+                              std::vector<cd> &G, std::vector<cd> &H, double *mom = nullptr) {
   const int nxh = c->nxh, nz32 = c->nz32, nx32 = c->nx32, nh = c->nh;
   std::vector<cd> col(nz32), colo(nz32), xs(nx32), xo(nx32);
   std::vector<double> phys((size_t)4 * nx32), out((size_t)10 * nx32);
@@ -1277,6 +1294,7 @@ static double nonlinear_plane(const Ctx *c, const View &S, const View &P, int wa
       out[(size_t)5 * nx32 + m] = bv2;
       const double vel = (std::fabs(t1) * c->dxr) + (std::fabs(t2) * c->dyr[y]) + (std::fabs(t3) * c->dzr);
       vmax = std::max(vel, vmax);
+      if (mom) add_moments(mom, t1, t2, t3);
     }
     // r2c, keep 0..nxh-1
     for (int f = 0; f < 10; ++f) {
@@ -1315,6 +1333,140 @@ int orc_nonlinear(void *h, const double *Sd, double *Pd, int want_vv, int y0, in
     for (int y = y0; y < y1; ++y) vmax = std::max(vmax, nonlinear_plane(c, S, P, want_vv, y, G, H));
   }
   if (velmax) *velmax = vmax;
+  return 0;
+}
+
+// orc_nonlinear with the statistics-time moments: mom(19, ny) receives the
+// per-plane sums of add_moments over the padded physical grid (unscaled).
+int orc_nonlinear_mom(void *h, const double *Sd, double *Pd, int want_vv, double *velmax, double *mom) {
+  Ctx *c = static_cast<Ctx *>(h);
+  const size_t ny = c->ny, nxh = c->nxh, nz = c->nz;
+  View S{reinterpret_cast<cd *>(const_cast<double *>(Sd)), nz * nxh * ny, nxh * ny, ny, 1};
+  View P{reinterpret_cast<cd *>(Pd), nz * nxh * ny, nxh * ny, ny, 1};
+  double vmax = 0.0;
+#pragma omp parallel reduction(max : vmax)
+  {
+    std::vector<cd> G((size_t)4 * c->nz32 * nxh), H((size_t)10 * c->nz32 * nxh);
+    std::vector<double> acc(kNMom);
+#pragma omp for schedule(dynamic, 1)
+    for (int y = 0; y < (int)ny; ++y) {
+      std::fill(acc.begin(), acc.end(), 0.0);
+      vmax = std::max(vmax, nonlinear_plane(c, S, P, want_vv, y, G, H, acc.data()));
+      for (int q = 0; q < kNMom; ++q) mom[(size_t)q * ny + y] = acc[q];
+    }
+  }
+  if (velmax) *velmax = vmax;
+  return 0;
+}
+
+// get_statistics_2 (get_statistics.f90:376-539 with the moments of
+// dft_x_realspace.f90:128-484).  S(ny,nxh,nz,4): the substep's input state;
+// P(ny,nxh,nz,10): its products with p in slot 5 (after orc_pressure); mom:
+// orc_nonlinear_mom's sums.  out(29, ny), the datasets in the reference's
+// order: uuu vvv www, uuuu vvvv wwww, u6 v6 w6, u8 v8 w8, u10 v10 w10
+// (rescale * sum, zero on the wall rows), -duuvdy -dvvvdy -dvwwdy -duvvdy
+// (D1 LU(D0)^-1 of rescale * sum of -uuv, -vvv, -vww, -uvv), up, -dpudy,
+// vp, -2dpvdy, wp, pp, 2pdudx, 2pdvdy, 2pdwdz, pdudy+pdvdx.
+int orc_statistics2(void *h, const double *Sd, const double *Pd, const double *mom, double *out) {
+  Ctx *c = static_cast<Ctx *>(h);
+  const cd *S = reinterpret_cast<const cd *>(Sd);
+  const cd *P = reinterpret_cast<const cd *>(Pd);
+  const int n = c->ny;
+  auto line = [&](int q) { return out + (size_t)q * n; };
+  // dataset order of the first 15 lines -> accumulator index (3..21 - 3)
+  const int mom_of[15] = {0, 1, 2, 3, 4, 5, 10, 11, 12, 13, 14, 15, 16, 17, 18};
+  for (int q = 0; q < 15; ++q) {
+    double *o = line(q);
+    o[0] = 0.0;
+    for (int i = 1; i < n - 1; ++i) o[i] = c->rescale * mom[(size_t)mom_of[q] * n + i];
+    o[n - 1] = 0.0;
+  }
+  // D1 LU(D0)^-1 of a real line (dcmplx(x, 0), real part)
+  auto ddy = [&](const double *x, double scale_in, double *o, double scale_out) {
+    std::vector<cd> a(n), b(n);
+    for (int i = 0; i < n; ++i) a[i] = C(x[i] * scale_in, 0.0);
+    labs_bs(c->d0lu.data(), n, c->bw, a.data());
+    labs_mv(c->d1.data(), n, c->bw, a.data(), b.data());
+    for (int i = 0; i < n; ++i) o[i] = scale_out * b[i].re;
+  };
+  // -uuv, -vvv, -vww, -uvv are accumulators 6, 8, 7, 9 (the reference's
+  // dataset order -duuvdy, -dvvvdy, -dvwwdy, -duvvdy)
+  const int tri[4] = {6, 8, 7, 9};
+  for (int q = 0; q < 4; ++q) ddy(mom + (size_t)tri[q] * n, c->rescale, line(15 + q), 1.0);
+  // Parseval sums over all modes in the reference's line order (kx fastest):
+  // get_uv (982-1017) and get_turb_transport_PI1_ij (817-902)
+  std::vector<double> up(n, 0.0), vp(n, 0.0), wp(n, 0.0), pp(n, 0.0), puu(n, 0.0), pvv(n, 0.0), pww(n, 0.0),
+      puv(n, 0.0);
+  const int L = c->nxh * c->nz;
+  auto uv_sum = [&](std::vector<double> &acc, int fa, int fb, bool fb_is_p) {
+    for (int l = 0; l < L; ++l) {
+      const int k = l % c->nxh, j = l / c->nxh;
+      const cd *a = S + ix(*c, fa, j, k, 0);
+      const cd *b = fb_is_p ? P + ix(*c, 5, j, k, 0) : S + ix(*c, fb, j, k, 0);
+      for (int i = 0; i < n; ++i) {
+        if (k == 0) acc[i] = acc[i] + a[i].re * b[i].re + a[i].im * b[i].im;
+        else acc[i] = acc[i] + 2.0 * (a[i].re * b[i].re + a[i].im * b[i].im);
+      }
+    }
+  };
+  uv_sum(up, 0, 0, true);
+  uv_sum(vp, 1, 0, true);
+  uv_sum(wp, 2, 0, true);
+  for (int l = 0; l < L; ++l) {  // <pp>
+    const int k = l % c->nxh, j = l / c->nxh;
+    const cd *a = P + ix(*c, 5, j, k, 0);
+    for (int i = 0; i < n; ++i) {
+      if (k == 0) pp[i] = pp[i] + a[i].re * a[i].re + a[i].im * a[i].im;
+      else pp[i] = pp[i] + 2.0 * (a[i].re * a[i].re + a[i].im * a[i].im);
+    }
+  }
+  for (int l = 0; l < L; ++l) {  // PI1, first loop: x and z derivatives
+    const int k = l % c->nxh, j = l / c->nxh;
+    const double kxw = c->lmfxr * c->kx[k], kzw = c->lmfzr * c->kz[j];
+    const cd *u = S + ix(*c, 0, j, k, 0), *v = S + ix(*c, 1, j, k, 0), *w = S + ix(*c, 2, j, k, 0);
+    const cd *p = P + ix(*c, 5, j, k, 0);
+    for (int i = 0; i < n; ++i) {
+      if (k == 0) {
+        puu[i] = puu[i] + 2.0 * kxw * (u[i].re * p[i].im - u[i].im * p[i].re);
+        puv[i] = puv[i] + kxw * (v[i].re * p[i].im - v[i].im * p[i].re);
+        pww[i] = pww[i] + 2.0 * kzw * (w[i].re * p[i].im - w[i].im * p[i].re);
+      } else {
+        puu[i] = puu[i] + (g_mutant == 10 ? -4.0 : 4.0) * kxw * (u[i].re * p[i].im - u[i].im * p[i].re);
+        puv[i] = puv[i] + 2.0 * kxw * (v[i].re * p[i].im - v[i].im * p[i].re);
+        pww[i] = pww[i] + 4.0 * kzw * (w[i].re * p[i].im - w[i].im * p[i].re);
+      }
+    }
+  }
+  std::vector<cd> du(n), dv(n), t(n);
+  for (int l = 0; l < L; ++l) {  // PI1, second loop: y derivatives
+    const int k = l % c->nxh, j = l / c->nxh;
+    const cd *u = S + ix(*c, 0, j, k, 0), *v = S + ix(*c, 1, j, k, 0), *p = P + ix(*c, 5, j, k, 0);
+    for (int i = 0; i < n; ++i) t[i] = u[i];
+    labs_bs(c->d0lu.data(), n, c->bw, t.data());
+    labs_mv(c->d1.data(), n, c->bw, t.data(), du.data());
+    for (int i = 0; i < n; ++i) t[i] = v[i];
+    labs_bs(c->d0lu.data(), n, c->bw, t.data());
+    labs_mv(c->d1.data(), n, c->bw, t.data(), dv.data());
+    for (int i = 0; i < n; ++i) {
+      if (k == 0) {
+        pvv[i] = pvv[i] + 2.0 * (dv[i].re * p[i].re + dv[i].im * p[i].im);
+        puv[i] = puv[i] + (du[i].re * p[i].re + du[i].im * p[i].im);
+      } else {
+        pvv[i] = pvv[i] + 4.0 * (dv[i].re * p[i].re + dv[i].im * p[i].im);
+        puv[i] = puv[i] + 2.0 * (du[i].re * p[i].re + du[i].im * p[i].im);
+      }
+    }
+  }
+  std::copy(up.begin(), up.end(), line(19));
+  ddy(up.data(), 1.0, line(20), -1.0);
+  std::copy(vp.begin(), vp.end(), line(21));
+  ddy(vp.data(), 1.0, line(22), -2.0);
+  std::copy(wp.begin(), wp.end(), line(23));
+  std::copy(pp.begin(), pp.end(), line(24));
+  std::copy(puu.begin(), puu.end(), line(25));
+  std::copy(pvv.begin(), pvv.end(), line(26));
+  std::copy(pww.begin(), pww.end(), line(27));
+  std::copy(puv.begin(), puv.end(), line(28));
   return 0;
 }
 

@@ -57,6 +57,8 @@ def lib():
         L.orc_fft.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
         L.orc_hvhg.argtypes = [ctypes.c_void_p, _dp]
         L.orc_init_from_vg.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_nonlinear_mom.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_statistics2.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_init_from_gphi.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_set_mutant.argtypes = [ctypes.c_int]
         L.orc_set_mutant.restype = None
@@ -206,6 +208,26 @@ class Oracle:
         lib().orc_nonlinear(self.h, S.ctypes.data_as(_dp), P.ctypes.data_as(_dp), int(want_vv), y0,
                             cfg.ny if y1 is None else y1, ctypes.byref(vm))
         return P, vm.value
+
+    STAT2_NAMES = ("uuu", "vvv", "www", "uuuu", "vvvv", "wwww", "u6", "v6", "w6", "u8", "v8", "w8", "u10", "v10",
+                   "w10", "-duuvdy", "-dvvvdy", "-dvwwdy", "-duvvdy", "up", "-dpudy", "vp", "-2dpvdy", "wp", "pp",
+                   "2pdudx", "2pdvdy", "2pdwdz", "pdudy+pdvdx")
+
+    def statistics2(self, S: np.ndarray):
+        """get_statistics_2's datasets for the state S (4 slots) at a
+        statistics substep: products with the moments, the pressure, then
+        the correlations.  Returns (dict name -> ny line, P with p in slot 5,
+        the raw moment sums (19, ny))."""
+        cfg = self.cfg
+        P = np.zeros((10, cfg.nz, cfg.nxh, cfg.ny), dtype=np.complex128)
+        mom = np.zeros((19, cfg.ny))
+        vm = ctypes.c_double(0.0)
+        Sc = np.ascontiguousarray(S)
+        lib().orc_nonlinear_mom(self.h, Sc.ctypes.data_as(_dp), P.ctypes.data_as(_dp), 1, ctypes.byref(vm), _p(mom))
+        self.pressure(Sc, P)
+        out = np.zeros((29, cfg.ny))
+        lib().orc_statistics2(self.h, Sc.ctypes.data_as(_dp), P.ctypes.data_as(_dp), _p(mom), _p(out))
+        return {k: out[i] for i, k in enumerate(self.STAT2_NAMES)}, P, mom
 
     def opp_planes(self, S: np.ndarray) -> np.ndarray:
         cfg = self.cfg

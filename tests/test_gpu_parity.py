@@ -513,3 +513,32 @@ def test_mixed_scalar_laminar_on_device(lib, oracle_lib):
     assert np.abs(Sg[3, 0, 0] - S[3, 0, 0]).max() <= 1e-12
     assert all(o["phi_t"] == 0.0 for o in outs)
     assert abs(outs[-1]["q_w"] - kappa) <= 1e-11 * kappa
+
+
+@pytest.mark.parametrize("name", ["re180_64", "re1000_thin"])
+def test_statistics2(lib, oracle_lib, name):
+    """get_statistics_2 on the device against the oracle: the moment lines
+    and the derivatives of the triple products to 1e-12 of each line's
+    maximum (sums in another order), the pressure correlations to 1e-10
+    (the pressure itself is held to 1e-11); the state then advances exactly
+    as with pb_substep."""
+    cfg = CFGS[name]
+    orc = oracle_lib.Oracle(cfg)
+    S, RK = developed(cfg, orc)
+    ref, P, _ = orc.statistics2(S)
+    ch = lib.Channel(cfg)
+    ch.set_state(S, RK)
+    out, p, vv, st = ch.substep_stats2(cfg.dt)
+    for i, k in enumerate(ch.STAT2_NAMES):
+        scale = max(np.abs(ref[k]).max(), 1e-300)
+        e = np.abs(st[k] - ref[k]).max() / scale
+        print(f"{name} {k}: {e:.2e}", flush=True)
+        assert e <= (1e-12 if i < 19 else 1e-10), (k, e)
+    S1, RK1 = ch.get_state()
+    ch.close()
+    ch = lib.Channel(cfg)
+    ch.set_state(S, RK)
+    ch.substep(1, cfg.dt)
+    S2, RK2 = ch.get_state()
+    ch.close()
+    assert np.array_equal(S1, S2) and np.array_equal(RK1, RK2)
