@@ -79,6 +79,14 @@ SIGNATURES = {
     "pb_ensemble_step": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(PbSubstepOut)]),
     "pb_ensemble_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp]),
     "pb_ensemble_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "pb_set_control": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _dp]),
+    "pb_adj_wavespace_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp, _dp,
+                                             _dp, _dp, ctypes.c_double, ctypes.c_double]),
+    "pb_adj_nonlinear_host": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "pb_adj_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp,
+                                      ctypes.c_double, ctypes.c_double, _dp, _dp]),
+    "pb_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                   _dp, _dp, _dp]),
 }
 
 _lib = None
@@ -325,6 +333,55 @@ class Channel:
                                         None if v_old is None else _ptr(np.ascontiguousarray(v_old)),
                                         ctypes.byref(o)))
         return o.as_dict()
+
+    # ---- opposition gain, wall actuation, discrete adjoint (pb_capi.h)
+    def set_control(self, alpha: float, act: np.ndarray | None = None):
+        """Opposition gain and wall actuation act (nz, nx/2) complex (or None)."""
+        a = None if act is None else np.ascontiguousarray(act, dtype=np.complex128)
+        if a is not None:
+            assert a.shape == (self.cfg.nz, self.cfg.nxh)
+        _check(load().pb_set_control(self.h, float(alpha), _ptr(a)))
+        self.cfg.opp_alpha = float(alpha)
+
+    def adj_wavespace_host(self, s: int, dt: float, AU: np.ndarray, ARK: np.ndarray, vd: np.ndarray | None = None,
+                           tau_bar: float = 0.0, q_bar: float = 0.0):
+        """Transpose of the wall-normal stage: AU (9, nz, nx/2, ny) and ARK
+        (3, ...) in place; returns (avd (2, nz, nx/2), act_bar (nz, nx/2),
+        alpha terms (nz, nx/2))."""
+        c = self.cfg
+        avd = np.zeros((2, c.nz, c.nxh), np.complex128)
+        act = np.zeros((c.nz, c.nxh), np.complex128)
+        al = np.zeros((c.nz, c.nxh))
+        _check(load().pb_adj_wavespace_host(self.h, s, dt, _ptr(AU), _ptr(ARK),
+                                            None if vd is None else _ptr(np.ascontiguousarray(vd)), _ptr(avd),
+                                            _ptr(act), al.ctypes.data_as(_dp), tau_bar, q_bar))
+        return avd, act, al
+
+    def adj_nonlinear_host(self, S: np.ndarray, aprod: np.ndarray) -> np.ndarray:
+        """Transpose of the nonlinear stage at state S: aprod (9 or 10, ...)
+        -> adjoint of S (4, nz, nx/2, ny)."""
+        out = np.zeros(self.shape(4), np.complex128)
+        _check(load().pb_adj_nonlinear_host(self.h, _ptr(np.ascontiguousarray(S)),
+                                            _ptr(np.ascontiguousarray(aprod[:9])), _ptr(out)))
+        return out
+
+    def adj_substep(self, s: int, dt: float, S: np.ndarray, aS: np.ndarray, aRK: np.ndarray, tau_bar=0.0,
+                    q_bar=0.0):
+        """Transpose of substep s around input state S: aS, aRK in place (out
+        -> in); returns (d/d alpha, act_bar (nz, nx/2))."""
+        al = ctypes.c_double(0.0)
+        act = np.zeros((self.cfg.nz, self.cfg.nxh), np.complex128)
+        _check(load().pb_adj_substep(self.h, s, dt, _ptr(np.ascontiguousarray(S)), _ptr(aS), _ptr(aRK), tau_bar,
+                                     q_bar, ctypes.byref(al), _ptr(act)))
+        return al.value, act
+
+    def gradient(self, nsteps: int, dt: float, ckpt_every: int = 20, wq: float = 0.5):
+        """J = mean over nsteps of tau_w + wq q_w and its gradient: (J,
+        dJ/dalpha, dJ/da (nz, nx/2) complex)."""
+        J, da = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        g = np.zeros((self.cfg.nz, self.cfg.nxh), np.complex128)
+        _check(load().pb_gradient(self.h, nsteps, ckpt_every, dt, wq, ctypes.byref(J), ctypes.byref(da), _ptr(g)))
+        return J.value, da.value, g
 
 
 # ---- LABS band operations (module_labs.f90:462, 499, 570) on the device

@@ -94,6 +94,19 @@ struct pb_ctx {
   double2 *stat_uvw = nullptr;
   double *dmom = nullptr, *dmsum = nullptr, *dpv = nullptr, *dstat2 = nullptr, *mom_active = nullptr;
   double2 *dstat2_tmp = nullptr;
+  // wall actuation (pb_set_control): one complex value per system, added to
+  // the wall v of both walls; used when act_on
+  double2 *act = nullptr;
+  bool act_on = false;
+  // discrete adjoint (allocated on first use): adjoints of the 9 slots, of
+  // the stores, 6 scratch columns per system, the padded product / state
+  // adjoints (8 fields), of vd (2 planes), of the actuation (accumulated)
+  // and per-system d/d alpha terms (accumulated); adj_states: saved states
+  // of the gradient driver (checkpoints and one segment's substep inputs)
+  double2 *adj_U = nullptr, *adj_RK = nullptr, *adj_buf = nullptr, *adj_pad = nullptr, *adj_vd = nullptr,
+          *adj_act = nullptr, *adj_states = nullptr;
+  double *adj_alpha = nullptr;
+  size_t adj_nstates = 0;
   double *dbuf = nullptr, *dbuf2 = nullptr, *rbuf = nullptr, *vpart = nullptr, *scal = nullptr;
   double2 *ppe_bc = nullptr;  // statistics time: pressure wall values [2][kz][kx]
   double *dcenter = nullptr;  // B_k(0): centre-line values of the basis (get_statistics.f90:566-570)
@@ -161,7 +174,8 @@ static int ctx_free(pb_ctx *c) {
   void *ptrs[] = {c->U, c->RK, c->pad, c->buf, c->vd, c->dbuf, c->dbuf2, c->rbuf, c->vpart, c->scal, c->dD0, c->dD1, c->dD2,
                   c->dD0LU, c->dinteg, c->ddyr, c->drows, c->twz, c->twx, c->slotmap, c->mean_db,
                   c->ppe_bc, c->dvv, c->dcenter, c->dstat, c->usrc, c->stat_uvw, c->dmom, c->dmsum, c->dpv,
-                  c->dstat2, c->dstat2_tmp};
+                  c->dstat2, c->dstat2_tmp, c->act, c->adj_U, c->adj_RK, c->adj_buf, c->adj_pad, c->adj_vd,
+                  c->adj_act, c->adj_states, c->adj_alpha};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -237,17 +251,23 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
     ctx_free(c);
     return fail(PB_E_ARG, "scalar_bc %d is neither PB_SCALAR_SOURCE nor PB_SCALAR_MIXED", cfg->scalar_bc);
   }
-  if (cfg->opp_alpha != 0.0) {
-    // detection rows r with v(y_d) = r . v (= B(y_d) . LU(D0)^-1 v)
+  // detection rows r with v(y_d) = r . v (= B(y_d) . LU(D0)^-1 v): needed
+  // with control; also set up without it when the plane lies inside the
+  // channel (pb_set_control may switch control on, the adjoint's d/d alpha
+  // needs v(y_d))
+  if (cfg->opp_alpha != 0.0 || (cfg->opp_ydist > 0.0 && cfg->opp_ydist < 1.0)) {
     std::vector<double> lo, hi;
     if (cfg->opp_ydist <= 0.0 || cfg->opp_ydist >= 1.0 ||
         pb::detection_row(c->col, c->d0lu, c->bw, -1.0 + cfg->opp_ydist, lo) ||
         pb::detection_row(c->col, c->d0lu, c->bw, 1.0 - cfg->opp_ydist, hi)) {
-      ctx_free(c);
-      return fail(PB_E_ARG, "opposition-control plane %g outside the channel", cfg->opp_ydist);
+      if (cfg->opp_alpha != 0.0) {
+        ctx_free(c);
+        return fail(PB_E_ARG, "opposition-control plane %g outside the channel", cfg->opp_ydist);
+      }
+    } else {
+      c->opp_rows = lo;
+      c->opp_rows.insert(c->opp_rows.end(), hi.begin(), hi.end());
     }
-    c->opp_rows = lo;
-    c->opp_rows.insert(c->opp_rows.end(), hi.begin(), hi.end());
   }
   c->pz = make_fft_plan(c->nz32);
   c->px = make_fft_plan(c->nx32);
@@ -624,7 +644,7 @@ static int launch_zinv_t(pb_ctx *c) {
   const size_t ldsz = (size_t)c->nz32 * c->Bz * sizeof(double2);
   StageTimer tm(c, PB_STAGE_ZINV);
   hipLaunchKernelGGL(pb::zinv_kernel<F>, dim3(c->nxh / c->Bz, c->ny, 4), dim3(F::T), ldsz, c->stream, c->U,
-                     c->pad, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield);
+                     c->pad, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield, pb::MAP_STATE, 1.0);
   HIPCHK(hipGetLastError());
   return PB_OK;
 }
@@ -643,7 +663,8 @@ static int launch_zfwd_t(pb_ctx *c, int want_vv, double scale) {
   const size_t ldsz = (size_t)c->nz32 * c->Bz * sizeof(double2);
   StageTimer tm(c, PB_STAGE_ZFWD);
   hipLaunchKernelGGL(pb::zfwd_kernel<F>, dim3(c->nxh / c->Bz, c->ny, want_vv ? 9 : 8), dim3(F::T), ldsz, c->stream,
-                     c->pad, c->U, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield, scale);
+                     c->pad, c->U, c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield, scale,
+                     pb::MAP_PROD);
   HIPCHK(hipGetLastError());
   return PB_OK;
 }
@@ -788,6 +809,7 @@ static pb::WsArgs ws_args(pb_ctx *c, int s) {
   A.vd = c->cfg.opp_alpha != 0.0 ? (const pb::cplx *)c->vd : nullptr;
   A.usrc = c->cfg.scalar_bc == PB_SCALAR_MIXED ? (const pb::cplx *)c->usrc : nullptr;
   A.src_scale = c->cfg.nu / c->cfg.pr / c->cfg.bulk_u;
+  A.act = c->act_on ? (const pb::cplx *)c->act : nullptr;
   A.plane = c->plane;
   A.field = c->field;
   A.nxh = c->nxh;
@@ -1607,6 +1629,318 @@ int pb_labs_bs_noalloc(int32_t n, int32_t bw, const double *A_lu, int32_t nrhs, 
                      (pb::cplx *)(ptr + off_b));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(b, ptr + off_b, nb * sizeof(double), hipMemcpyDeviceToHost));
+  return PB_OK;
+}
+
+// ---- wall actuation and discrete adjoint --------------------------------------
+int pb_set_control(pb_ctx *c, double alpha, const double *act) {
+  if (!c) return fail(PB_E_ARG, "null context");
+  if (alpha != 0.0 && c->opp_rows.empty())
+    return fail(PB_E_ARG, "opposition control needs a detection plane inside the channel (opp_ydist in (0,1))");
+  if (act) {
+    if (!c->act) {
+      const hipError_t e = hipMalloc((void **)&c->act, c->plane * sizeof(double2));
+      if (e != hipSuccess) return fail(PB_E_HIP, "hipMalloc for the actuation: %s", hipGetErrorString(e));
+      c->bytes += c->plane * sizeof(double2);
+    }
+    HIPCHK(hipMemcpy(c->act, act, c->plane * sizeof(double2), hipMemcpyHostToDevice));
+  }
+  c->act_on = act != nullptr;
+  c->cfg.opp_alpha = alpha;
+  drop_graph(c);  // the captured step holds the old control
+  return PB_OK;
+}
+
+static int ensure_adj(pb_ctx *c) {
+  if (c->adj_U) return PB_OK;
+  auto alloc = [&](void **p, size_t bytes) -> int {
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(PB_E_HIP, "hipMalloc(%zu) for the adjoint: %s", bytes, hipGetErrorString(e));
+    c->bytes += bytes;
+    return PB_OK;
+  };
+  int r;
+  if ((r = alloc((void **)&c->adj_U, 9 * c->field * sizeof(double2))) ||
+      (r = alloc((void **)&c->adj_RK, 3 * c->field * sizeof(double2))) ||
+      (r = alloc((void **)&c->adj_buf, 6 * c->field * sizeof(double2))) ||
+      (r = alloc((void **)&c->adj_pad, 8 * c->pfield * sizeof(double2))) ||
+      (r = alloc((void **)&c->adj_vd, 2 * c->plane * sizeof(double2))) ||
+      (r = alloc((void **)&c->adj_act, c->plane * sizeof(double2))) ||
+      (r = alloc((void **)&c->adj_alpha, c->plane * sizeof(double))))
+    return r;
+  return PB_OK;
+}
+
+extern "C++" {
+template <class F>
+static int launch_zmap_inv_t(pb_ctx *c, const double2 *src, double2 *dst, int nf, uint64_t map, double scale) {
+  const size_t ldsz = (size_t)c->nz32 * c->Bz * sizeof(double2);
+  hipLaunchKernelGGL(pb::zinv_kernel<F>, dim3(c->nxh / c->Bz, c->ny, nf), dim3(F::T), ldsz, c->stream, src, dst,
+                     c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield, map, scale);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+template <class F>
+static int launch_zmap_fwd_t(pb_ctx *c, const double2 *src, double2 *dst, int nf, uint64_t map, double scale) {
+  const size_t ldsz = (size_t)c->nz32 * c->Bz * sizeof(double2);
+  hipLaunchKernelGGL(pb::zfwd_kernel<F>, dim3(c->nxh / c->Bz, c->ny, nf), dim3(F::T), ldsz, c->stream, src, dst,
+                     c->pz, c->twz, c->nxh, c->nz, c->ny, c->Bz, c->field, c->pfield, scale, map);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+template <class F>
+static int launch_xadj_t(pb_ctx *c) {
+  const size_t ldsx = F::lds_elems(c->nx32, 2) * sizeof(double2);
+  hipLaunchKernelGGL(pb::xadj_kernel<F>, dim3(c->nz32, c->ny), dim3(F::T), ldsx, c->stream, c->pad, c->adj_pad,
+                     c->px, c->twx, c->nxh, c->nz32, c->pfield);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+}  // extern "C++"
+
+// spectral slots -> padded fields (z inverse, times scale) / padded fields ->
+// spectral slots (z forward, truncation, times scale)
+static int launch_zmap_inv(pb_ctx *c, const double2 *src, double2 *dst, int nf, uint64_t map, double scale) {
+  int r = PB_OK;
+  const int ez = pick_ept((size_t)c->nz32 * c->Bz);
+  if (ct_z(c)) r = launch_zmap_inv_t<pb::FftZ1152>(c, src, dst, nf, map, scale);
+  else PB_EPT_DISPATCH(ez, r = launch_zmap_inv_t<F>(c, src, dst, nf, map, scale));
+  return r;
+}
+static int launch_zmap_fwd(pb_ctx *c, const double2 *src, double2 *dst, int nf, uint64_t map, double scale) {
+  int r = PB_OK;
+  const int ez = pick_ept((size_t)c->nz32 * c->Bz);
+  if (ct_z(c)) r = launch_zmap_fwd_t<pb::FftZ1152>(c, src, dst, nf, map, scale);
+  else PB_EPT_DISPATCH(ez, r = launch_zmap_fwd_t<F>(c, src, dst, nf, map, scale));
+  return r;
+}
+static int launch_xadj(pb_ctx *c) {
+  int r = PB_OK;
+  const int ex = pick_ept((size_t)c->nx32 * 2);
+  if (ct_x(c)) r = launch_xadj_t<pb::FftX1536>(c);
+  else PB_EPT_DISPATCH(ex, r = launch_xadj_t<F>(c));
+  return r;
+}
+
+// Transpose of the nonlinear stage for the state in U slots 0,1,2,6: the
+// product adjoints in adj_U slots 0-4, 6-8 -> the state adjoints in adj_U
+// slots 0,1,2,6 (the z inverse of the state is left in pad fields 0-3).
+static int adj_nonlinear_dev(pb_ctx *c) {
+  int r;
+  if ((r = launch_zinv_any(c))) return r;
+  if ((r = launch_zmap_inv(c, c->adj_U, c->adj_pad, 8, pb::MAP_PROD & 0xFFFFFFFFull, c->rescale))) return r;
+  if ((r = launch_xadj(c))) return r;
+  return launch_zmap_fwd(c, c->adj_pad, c->adj_U, 4, pb::MAP_STATE, 1.0);
+}
+
+static pb::AdjArgs adj_args(pb_ctx *c, const double2 *vd, double tau_bar, double q_bar) {
+  pb::AdjArgs A;
+  A.AU = (pb::cplx *)c->adj_U;
+  A.ARK = (pb::cplx *)c->adj_RK;
+  A.abuf = (pb::cplx *)c->adj_buf;
+  A.dbuf = c->dbuf;
+  A.vd = (const pb::cplx *)vd;
+  A.avd = (pb::cplx *)c->adj_vd;
+  A.aact = (pb::cplx *)c->adj_act;
+  A.aalpha = c->adj_alpha;
+  A.mixed = c->cfg.scalar_bc == PB_SCALAR_MIXED;
+  A.ks = c->cfg.nu / c->cfg.pr / c->cfg.bulk_u;
+  A.tau_bar = tau_bar;
+  A.q_bar = q_bar;
+  A.plane = c->plane;
+  A.field = c->field;
+  A.nxh = c->nxh;
+  A.nz = c->nz;
+  A.nh = c->nh;
+  A.lmfxr = c->lmfxr;
+  A.lmfzr = c->lmfzr;
+  A.opp_alpha = c->cfg.opp_alpha;
+  return A;
+}
+
+// Transpose of substep s (fixed dt) at the input state in U slots 0,1,2,6:
+// adj_U slots 0,1,2,6 and adj_RK hold the adjoints of the substep's output
+// state and stores on entry, of its input state and stores on return;
+// adj_act and adj_alpha accumulate.  tau_bar, q_bar: adjoints of tau_w, q_w
+// (used on the last substep).
+static int adj_substep_dev(pb_ctx *c, int s, double dt, double tau_bar, double q_bar) {
+  int r;
+  const bool rows = !c->opp_rows.empty();
+  if (rows && (r = launch_oppvd(c, c->U))) return r;
+  pb::LineParams P = c->line_params(s, dt);
+  const pb::AdjArgs A = adj_args(c, rows ? c->vd : nullptr, tau_bar, q_bar);
+  hipLaunchKernelGGL(pb::adj_ws_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P, A, c->nsys);
+  HIPCHK(hipGetLastError());
+  if ((r = adj_nonlinear_dev(c))) return r;
+  const bool opp = c->cfg.opp_alpha != 0.0, mixed = c->cfg.scalar_bc == PB_SCALAR_MIXED;
+  if (opp || mixed) {
+    hipLaunchKernelGGL(pb::adj_finish_kernel, dim3(blocks_for(c->field, 256)), dim3(256), 0, c->stream,
+                       (pb::cplx *)c->adj_U, opp ? (const pb::cplx *)c->adj_vd : nullptr, c->drows, c->plane,
+                       c->field, c->ny, mixed ? 1 : 0);
+    HIPCHK(hipGetLastError());
+  }
+  return PB_OK;
+}
+
+static int adj_clear_controls(pb_ctx *c, const double *act_bar) {
+  HIPCHK(hipMemsetAsync(c->adj_alpha, 0, c->plane * sizeof(double), c->stream));
+  if (act_bar) HIPCHK(hipMemcpyAsync(c->adj_act, act_bar, c->plane * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  else HIPCHK(hipMemsetAsync(c->adj_act, 0, c->plane * sizeof(double2), c->stream));
+  return PB_OK;
+}
+// sum of the per-system d/d alpha terms (ascending system order) and the
+// actuation adjoint, to the host
+static int adj_read_controls(pb_ctx *c, double *alpha_sum, double *alpha_plane, double *act_bar) {
+  std::vector<double> a(c->plane);
+  HIPCHK(hipMemcpyAsync(a.data(), c->adj_alpha, c->plane * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (act_bar) HIPCHK(hipMemcpyAsync(act_bar, c->adj_act, c->plane * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (alpha_plane) std::memcpy(alpha_plane, a.data(), c->plane * sizeof(double));
+  if (alpha_sum) {
+    double t = 0.0;
+    for (double x : a) t += x;
+    *alpha_sum += t;
+  }
+  return PB_OK;
+}
+
+int pb_adj_wavespace_host(pb_ctx *c, int32_t s, double dt, double *AU, double *ark, const double *vd, double *avd,
+                          double *act_bar, double *alpha_bar, double tau_bar, double q_bar) {
+  if (!c || !AU || !ark) return fail(PB_E_ARG, "null argument");
+  if (s < 1 || s > c->cfg.rk_step) return fail(PB_E_ARG, "substep %d out of range", s);
+  int r;
+  if ((r = check_dt(dt)) || (r = ensure_adj(c))) return r;
+  static const int map9[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+  if ((r = upload_ref(c, AU, 9, map9, c->adj_U)) || (r = upload_ref(c, ark, 3, kRKMap, c->adj_RK))) return r;
+  if (vd) HIPCHK(hipMemcpyAsync(c->vd, vd, 2 * c->plane * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  if ((r = adj_clear_controls(c, act_bar))) return r;
+  pb::LineParams P = c->line_params(s, dt);
+  const pb::AdjArgs A = adj_args(c, vd ? c->vd : nullptr, tau_bar, q_bar);
+  hipLaunchKernelGGL(pb::adj_ws_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P, A, c->nsys);
+  HIPCHK(hipGetLastError());
+  if ((r = download_ref(c, AU, 9, map9, c->adj_U)) || (r = download_ref(c, ark, 3, kRKMap, c->adj_RK))) return r;
+  if (avd) HIPCHK(hipMemcpy(avd, c->adj_vd, 2 * c->plane * sizeof(double2), hipMemcpyDeviceToHost));
+  return adj_read_controls(c, nullptr, alpha_bar, act_bar);
+}
+
+int pb_adj_nonlinear_host(pb_ctx *c, const double *U, const double *aprod, double *astate) {
+  if (!c || !U || !aprod || !astate) return fail(PB_E_ARG, "null argument");
+  int r;
+  if ((r = ensure_adj(c))) return r;
+  static const int map9[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+  if ((r = upload_ref(c, U, 4, kStateMap, c->U)) || (r = upload_ref(c, aprod, 9, map9, c->adj_U))) return r;
+  c->have_state = true;
+  if ((r = adj_nonlinear_dev(c))) return r;
+  return download_ref(c, astate, 4, kStateMap, c->adj_U);
+}
+
+int pb_adj_substep(pb_ctx *c, int32_t s, double dt, const double *state, double *astate, double *ark,
+                   double tau_bar, double q_bar, double *alpha_bar, double *act_bar) {
+  if (!c || !state || !astate || !ark) return fail(PB_E_ARG, "null argument");
+  if (s < 1 || s > c->cfg.rk_step) return fail(PB_E_ARG, "substep %d out of range", s);
+  int r;
+  if ((r = check_dt(dt)) || (r = ensure_adj(c))) return r;
+  if ((r = upload_ref(c, state, 4, kStateMap, c->U))) return r;
+  c->have_state = true;
+  if ((r = upload_ref(c, astate, 4, kStateMap, c->adj_U)) || (r = upload_ref(c, ark, 3, kRKMap, c->adj_RK))) return r;
+  if ((r = adj_clear_controls(c, act_bar))) return r;
+  if ((r = adj_substep_dev(c, s, dt, tau_bar, q_bar))) return r;
+  if ((r = download_ref(c, astate, 4, kStateMap, c->adj_U)) || (r = download_ref(c, ark, 3, kRKMap, c->adj_RK)))
+    return r;
+  return adj_read_controls(c, alpha_bar, nullptr, act_bar);
+}
+
+// saved states of the gradient driver: checkpoint q = U slots 0,1,2,6 and the
+// three stores (7 fields), substep inputs = U slots 0,1,2,6 (4 fields)
+static int copy_slots(pb_ctx *c, double2 *dst, const double2 *src, const int *slots, int n, bool to_saved) {
+  for (int i = 0; i < n; ++i) {
+    double2 *d = to_saved ? dst + (size_t)i * c->field : dst + (size_t)slots[i] * c->field;
+    const double2 *s = to_saved ? src + (size_t)slots[i] * c->field : src + (size_t)i * c->field;
+    HIPCHK(hipMemcpyAsync(d, s, c->field * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+  }
+  return PB_OK;
+}
+static int save_state(pb_ctx *c, double2 *dst, bool with_rk) {
+  static const int st[4] = {0, 1, 2, 6}, rk[3] = {0, 1, 2};
+  int r;
+  if ((r = copy_slots(c, dst, c->U, st, 4, true))) return r;
+  return with_rk ? copy_slots(c, dst + 4 * c->field, c->RK, rk, 3, true) : PB_OK;
+}
+static int restore_state(pb_ctx *c, const double2 *src, bool with_rk) {
+  static const int st[4] = {0, 1, 2, 6}, rk[3] = {0, 1, 2};
+  int r;
+  if ((r = copy_slots(c, c->U, src, st, 4, false))) return r;
+  return with_rk ? copy_slots(c, c->RK, src + 4 * c->field, rk, 3, false) : PB_OK;
+}
+
+int pb_gradient(pb_ctx *c, int32_t nsteps, int32_t ckpt_every, double dt, double wq, double *J, double *dJ_dalpha,
+                double *dJ_da) {
+  if (!c) return fail(PB_E_ARG, "null context");
+  if (!c->have_state) return fail(PB_E_STATE, "pb_gradient before pb_set_state");
+  if (nsteps < 1 || ckpt_every < 1) return fail(PB_E_ARG, "nsteps %d, ckpt_every %d", nsteps, ckpt_every);
+  if (c->cfg.dt_factor != 1.0) return fail(PB_E_ARG, "the adjoint needs a fixed dt (dt_factor = 1)");
+  int r;
+  if ((r = check_dt(dt)) || (r = ensure_adj(c))) return r;
+  const int S = c->cfg.rk_step, C = ckpt_every, nck = (nsteps + C - 1) / C;
+  // checkpoints (7 fields each), the final state (7), one segment's substep inputs (4 each)
+  const size_t need = (size_t)(nck + 1) * 7 + (size_t)C * S * 4;
+  if (c->adj_nstates < need) {
+    if (c->adj_states) {
+      (void)hipFree(c->adj_states);
+      c->bytes -= c->adj_nstates * c->field * sizeof(double2);
+      c->adj_states = nullptr;
+      c->adj_nstates = 0;
+    }
+    const hipError_t e = hipMalloc((void **)&c->adj_states, need * c->field * sizeof(double2));
+    if (e != hipSuccess)
+      return fail(PB_E_HIP, "hipMalloc(%zu) for %zu saved fields: %s", need * c->field * sizeof(double2), need,
+                  hipGetErrorString(e));
+    c->adj_nstates = need;
+    c->bytes += need * c->field * sizeof(double2);
+  }
+  double2 *ckpt = c->adj_states, *fin = ckpt + (size_t)nck * 7 * c->field, *seg = fin + 7 * c->field;
+  std::vector<pb_substep_out> out(S);
+  // forward: J and the checkpoints
+  double jsum = 0.0;
+  for (int k = 0; k < nsteps; ++k) {
+    if (k % C == 0 && (r = save_state(c, ckpt + (size_t)(k / C) * 7 * c->field, true))) return r;
+    double dtv = dt;
+    if ((r = upload_dt(c, dt))) return r;
+    for (int s = 1; s <= S; ++s)
+      if ((r = substep_async(c, s))) return r;
+    if ((r = read_outputs(c, 1, S, out.data(), &dtv))) return r;
+    jsum += out[S - 1].tau_w + wq * out[S - 1].q_w;
+  }
+  if ((r = save_state(c, fin, true))) return r;
+  if (J) *J = jsum / nsteps;
+  // reverse: segments last to first, each recomputed from its checkpoint
+  HIPCHK(hipMemsetAsync(c->adj_U, 0, 9 * c->field * sizeof(double2), c->stream));
+  HIPCHK(hipMemsetAsync(c->adj_RK, 0, 3 * c->field * sizeof(double2), c->stream));
+  if ((r = adj_clear_controls(c, nullptr))) return r;
+  const double tb = 1.0 / nsteps, qb = wq / nsteps;
+  for (int q = nck - 1; q >= 0; --q) {
+    const int k0 = q * C, k1 = std::min(nsteps, k0 + C);
+    if ((r = restore_state(c, ckpt + (size_t)q * 7 * c->field, true))) return r;
+    for (int k = k0; k < k1; ++k) {
+      if ((r = upload_dt(c, dt))) return r;
+      for (int s = 1; s <= S; ++s) {
+        if ((r = save_state(c, seg + (size_t)((k - k0) * S + s - 1) * 4 * c->field, false))) return r;
+        if (k == k1 - 1 && s == S) break;  // the last substep's output is not needed
+        if ((r = substep_async(c, s))) return r;
+      }
+    }
+    for (int k = k1 - 1; k >= k0; --k)
+      for (int s = S; s >= 1; --s) {
+        if ((r = restore_state(c, seg + (size_t)((k - k0) * S + s - 1) * 4 * c->field, false))) return r;
+        if ((r = adj_substep_dev(c, s, dt, s == S ? tb : 0.0, s == S ? qb : 0.0))) return r;
+      }
+  }
+  double asum = 0.0;
+  if ((r = adj_read_controls(c, &asum, nullptr, dJ_da))) return r;
+  if (dJ_dalpha) *dJ_dalpha = asum;
+  // leave the context at the state after the window, as a forward run would
+  if ((r = restore_state(c, fin, true))) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));
   return PB_OK;
 }
 

@@ -16,6 +16,7 @@
 #include "pb_common.h"
 #include "solver.h"
 #include "solver_fused.h"
+#include "solver_adj.h"
 
 namespace pb {
 
@@ -475,17 +476,23 @@ using FftZ1152 = PB_FFTZ1152;
 using FftX1536 = PB_FFTX1536;
 
 // ------------------------------------------------------------------ z inverse
-// grid: (nxh/B, ny, nfield); the 4 state fields u, v, w, theta live in U
-// slots 0,1,2,6.  Output pad[f][y][z][kx].
+// Slot maps: field f of a pad <-> slot (map >> 4f) & 15 of U.
+constexpr uint64_t MAP_STATE = 0x6210;          // u, v, w, theta in slots 0, 1, 2, 6
+constexpr uint64_t MAP_PROD = 0x587643210ull;   // 8 products (+ v*v) in slots 0-4, 6-8 (5)
+__device__ __forceinline__ int map_slot(uint64_t map, int f) { return (int)((map >> (4 * f)) & 15); }
+
+// grid: (nxh/B, ny, nfield); field f comes from U slot map_slot(map, f)
+// (MAP_STATE: the 4 state fields).  Output pad[f][y][z][kx], times scale.
 template <class F>
 __global__ void __launch_bounds__(F::T, F::MINW) zinv_kernel(const double2 *__restrict__ U, double2 *__restrict__ pad,
                                                            FftPlan P, const double2 *__restrict__ tw, int nxh, int nz,
-                                                           int ny, int B_, size_t field, size_t pfield) {
+                                                           int ny, int B_, size_t field, size_t pfield, uint64_t map,
+                                                           double scale) {
   extern __shared__ double2 lds[];
   const int B = F::B ? F::B : B_;
   const Blk bk = xcd_block();
   const int kx0 = bk.x * B, y = bk.y, f = bk.z;
-  const int slot = (f == 3) ? 6 : f;
+  const int slot = map_slot(map, f);
   const int N = F::N ? F::N : P.N, nh = nz / 2;
   const double2 *src = U + (size_t)slot * field + (size_t)y * nz * nxh + kx0;
   for (int e = threadIdx.x; e < N * B; e += F::T) {
@@ -500,7 +507,8 @@ __global__ void __launch_bounds__(F::T, F::MINW) zinv_kernel(const double2 *__re
   double2 *dst = pad + (size_t)f * pfield + (size_t)y * N * nxh + kx0;
   for (int e = threadIdx.x; e < N * B; e += F::T) {
     const int z = e / B, b = e % B;
-    dst[(size_t)z * nxh + b] = lds[F::at(z, b, B)];
+    const double2 v = lds[F::at(z, b, B)];
+    dst[(size_t)z * nxh + b] = scale == 1.0 ? v : c2(v.x * scale, v.y * scale);
   }
 }
 
@@ -636,19 +644,147 @@ __global__ void __launch_bounds__(F::T, F::MINW) xphys_kernel(double2 *__restric
   }
 }
 
+// ------------------------------------------------------- adjoint x pass
+// The transpose of xphys_kernel's work on one x line (grid (nz32, ny)),
+// for a fixed state: fwd = pad fields 0..3 of that state (u, v, w, theta,
+// z-inverted), adj = pad fields 0..7 of the product adjoints in (the
+// layout xphys_kernel writes its products in), fields 0..3 of the state
+// adjoints out.  Each step of the forward is transposed: the unpack of two
+// real spectra from one complex transform, the transform of sign -1 (its
+// transpose is the transform of sign +1), the products (u^2-v^2, uv, w^2-v^2,
+// vw, uw, u theta, v theta, w theta) at the point, the transform of sign +1
+// and the Hermitian completion of u + i v and w + i theta.
+template <class F>
+__global__ void __launch_bounds__(F::T, F::MINW) xadj_kernel(const double2 *__restrict__ fwd, double2 *__restrict__ adj,
+                                                           FftPlan P, const double2 *__restrict__ tw, int nxh,
+                                                           int nz32, size_t pfield) {
+  extern __shared__ double2 lds[];  // 2 lines of N, layout F::at
+  const int z = blockIdx.x, y = blockIdx.y, N = F::N ? F::N : P.N;
+  const size_t off = (size_t)y * nz32 * nxh + (size_t)z * nxh;
+  const double2 *fl = fwd + off;
+  double2 *al = adj + off;
+  for (int m = threadIdx.x; m < N; m += F::T) {
+    double2 z1 = c2(0.0, 0.0), z2 = c2(0.0, 0.0);
+    if (m == 0) {
+      z1 = c2(fl[0].x, fl[pfield].x);
+      z2 = c2(fl[2 * pfield].x, fl[3 * pfield].x);
+    } else if (m < nxh) {
+      const double2 a = fl[m], b = fl[pfield + m], c = fl[2 * pfield + m], d = fl[3 * pfield + m];
+      z1 = c2(a.x - b.y, a.y + b.x);
+      z2 = c2(c.x - d.y, c.y + d.x);
+    } else if (m > N - nxh) {
+      const int q = N - m;
+      const double2 a = fl[q], b = fl[pfield + q], c = fl[2 * pfield + q], d = fl[3 * pfield + q];
+      z1 = c2(a.x + b.y, -a.y + b.x);
+      z2 = c2(c.x + d.y, -c.y + d.x);
+    }
+    lds[F::at(m, 0, 2)] = z1;
+    lds[F::at(m, 1, 2)] = z2;
+  }
+  __syncthreads();
+  F::run(lds, 2, P, tw, +1);
+  constexpr int MAXP = (F::ELEMS + 1) / 2;
+  double pu[MAXP], pv[MAXP], pw[MAXP], pt[MAXP], au[MAXP], av[MAXP], aw[MAXP], at[MAXP];
+#pragma unroll
+  for (int q = 0; q < MAXP; ++q) {
+    const int m = threadIdx.x + q * F::T;
+    au[q] = av[q] = aw[q] = at[q] = 0.0;
+    pu[q] = pv[q] = pw[q] = pt[q] = 0.0;
+    if (m < N) {
+      const double2 a = lds[F::at(m, 0, 2)], c = lds[F::at(m, 1, 2)];
+      pu[q] = a.x; pv[q] = a.y; pw[q] = c.x; pt[q] = c.y;
+    }
+  }
+  for (int round = 0; round < 2; ++round) {
+    __syncthreads();
+    // transpose of the unpack: Z[m] gets A_m/2 + (i/2) B_m (m < nxh) and
+    // conj(A_q)/2 + (i/2) conj(B_q) for q = N-m < nxh (q = 0 for m = 0)
+    for (int m = threadIdx.x; m < N; m += F::T) {
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int fa = round * 4 + 2 * b;
+        double zr = 0.0, zi = 0.0;
+        if (m < nxh) {
+          const double2 A = al[(size_t)fa * pfield + m], B = al[(size_t)(fa + 1) * pfield + m];
+          zr += 0.5 * A.x - 0.5 * B.y;
+          zi += 0.5 * A.y + 0.5 * B.x;
+        }
+        const int q = (m == 0) ? 0 : N - m;
+        if (q < nxh) {
+          const double2 A = al[(size_t)fa * pfield + q], B = al[(size_t)(fa + 1) * pfield + q];
+          zr += 0.5 * A.x + 0.5 * B.y;
+          zi += -0.5 * A.y + 0.5 * B.x;
+        }
+        lds[F::at(m, b, 2)] = c2(zr, zi);
+      }
+    }
+    __syncthreads();
+    F::run(lds, 2, P, tw, +1);
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q) {
+      const int m = threadIdx.x + q * F::T;
+      if (m < N) {
+        const double2 Z0 = lds[F::at(m, 0, 2)], Z1 = lds[F::at(m, 1, 2)];
+        const double u = pu[q], v = pv[q], w = pw[q], t = pt[q];
+        if (round == 0) {  // P1 = u^2 - v^2, P2 = uv, P3 = w^2 - v^2, P4 = vw
+          au[q] += 2.0 * u * Z0.x + v * Z0.y;
+          av[q] += -2.0 * v * Z0.x + u * Z0.y - 2.0 * v * Z1.x + w * Z1.y;
+          aw[q] += 2.0 * w * Z1.x + v * Z1.y;
+        } else {  // P5 = uw, P7 = u theta, P8 = v theta, P9 = w theta
+          au[q] += w * Z0.x + t * Z0.y;
+          aw[q] += u * Z0.x + t * Z1.y;
+          av[q] += t * Z1.x;
+          at[q] += u * Z0.y + v * Z1.x + w * Z1.y;
+        }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXP; ++q) {
+    const int m = threadIdx.x + q * F::T;
+    if (m < N) {
+      lds[F::at(m, 0, 2)] = c2(au[q], av[q]);
+      lds[F::at(m, 1, 2)] = c2(aw[q], at[q]);
+    }
+  }
+  __syncthreads();
+  F::run(lds, 2, P, tw, -1);
+  __syncthreads();
+  // transpose of the Hermitian completion: kx = q < nxh gathers Z[q] and
+  // conj(Z[N-q]) (a + i b at q, conj(a) + i conj(b) at N-q; Re only at 0)
+  for (int q = threadIdx.x; q < nxh; q += F::T) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const double2 Z = lds[F::at(q, b, 2)];
+      double2 A, B;
+      if (q == 0) {
+        A = c2(Z.x, 0.0);
+        B = c2(Z.y, 0.0);
+      } else {
+        const double2 W = lds[F::at(N - q, b, 2)];
+        A = c2(Z.x + W.x, Z.y - W.y);
+        B = c2(Z.y + W.y, W.x - Z.x);
+      }
+      al[(size_t)(2 * b) * pfield + q] = A;
+      al[(size_t)(2 * b + 1) * pfield + q] = B;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ z forward
-// grid: (nxh/B, ny, 8 or 9).  pad field g -> U slot (0,1,2,3,4,6,7,8,5)[g],
-// with truncation and rescale.
+// grid: (nxh/B, ny, nfield).  pad field g -> U slot map_slot(map, g)
+// (MAP_PROD: (0,1,2,3,4,6,7,8,5)[g]), with truncation and rescale.
 template <class F>
 __global__ void __launch_bounds__(F::T, F::MINW) zfwd_kernel(const double2 *__restrict__ pad, double2 *__restrict__ U,
                                                            FftPlan P, const double2 *__restrict__ tw, int nxh, int nz,
                                                            int ny, int B_, size_t field, size_t pfield,
-                                                           double rescale) {
+                                                           double rescale, uint64_t map) {
   extern __shared__ double2 lds[];
   const int B = F::B ? F::B : B_;
   const Blk bk = xcd_block();
   const int kx0 = bk.x * B, y = bk.y, g = bk.z;
-  const int slot = g < 5 ? g : (g < 8 ? g + 1 : 5);
+  const int slot = map_slot(map, g);
   const int N = F::N ? F::N : P.N, nh = nz / 2;
   const double2 *src = pad + (size_t)g * pfield + (size_t)y * N * nxh + kx0;
   for (int e = threadIdx.x; e < N * B; e += F::T) {
@@ -1083,6 +1219,31 @@ __global__ void __launch_bounds__(64) wavespace_kernel(LineParams P, WsArgs A, s
   LineParams Q = P;
   Q.dt = scal[0];
   wavespace_line(Q, A, sys);
+}
+
+// Transpose of the wall-normal stage, one thread per system (solver_adj.h);
+// P.dt is the fixed dt of the substep.
+__global__ void __launch_bounds__(64) adj_ws_kernel(LineParams P, AdjArgs A, size_t nsys) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  adj_wavespace_line(P, A, sys);
+}
+
+// The last transposes of a substep, per element of the [y][kz][kx] field:
+// the detection planes (vd = r_lo . v, r_hi . v over y; rows: r_lo then r_hi)
+// send avd back into the adjoint of v (slot 1), and the mixed-type source
+// (u of the input state) sends slot 5 into the adjoint of u (slot 0).
+__global__ void adj_finish_kernel(cplx *__restrict__ AU, const cplx *__restrict__ avd, const double *__restrict__ rows,
+                                  size_t plane, size_t field, int ny, int mixed) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= field) return;
+  const size_t sys = e % plane;
+  const int y = (int)(e / plane);
+  if (avd) {
+    const cplx lo = avd[sys], hi = avd[plane + sys];
+    AU[field + e] = cadd(AU[field + e], cadd(cscale(lo, rows[y]), cscale(hi, rows[ny + y])));
+  }
+  if (mixed) AU[e] = cadd(AU[e], AU[5 * field + e]);
 }
 
 // The (kx,kz) = (0,0) system: mean flow, mean spanwise velocity and mean

@@ -562,6 +562,9 @@ struct WsArgs {
   // the reference's held-bulk uniform source
   const cplx *usrc;
   double src_scale;
+  // wall actuation: one complex value per system added to the wall v of
+  // both walls ((kx,kz) != (0,0)), or nullptr
+  const cplx *act;
   size_t plane;   // nz*nxh
   size_t field;   // ny*plane
   int nxh, nz, nh;
@@ -592,6 +595,10 @@ PB_HD void wavespace_line(const LineParams &P, const WsArgs &A, size_t sys) {
   if (A.opp_alpha != 0.0 && A.vd) {
     vlo = cscale(A.vd[sys], -A.opp_alpha);
     vhi = cscale(A.vd[A.plane + sys], -A.opp_alpha);
+  }
+  if (A.act) {
+    vlo = cadd(vlo, A.act[sys]);
+    vhi = cadd(vhi, A.act[sys]);
   }
   solve_v_f(P, Db, col(A.U, 1), col(A.U, 2), col(A.U, 4), col(A.RK, 1), col(A.U, 3), b1, b2, kx, kz, vlo, vhi);
   solve_u_w(P.g.n, col(A.U, 0), col(A.U, 2), kx, kz);

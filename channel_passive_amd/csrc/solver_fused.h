@@ -750,7 +750,8 @@ PB_HD void sweep_Dt(const LineParams &P, const WsArgs &A, const SysCtx &c, doubl
 }
 
 // E: LU of the Poisson operator fused with the forward substitution of
-// v_p (wall values -alpha v(y_d) with opposition control, else 0) and v_h
+// v_p (wall values -alpha v(y_d) with opposition control, else 0, plus the
+// wall actuation) and v_h
 // (solve_v_f.f90:205-262), for the pair of systems sharing the operator
 template <int BW, bool TWO>
 PB_HD void sweep_E_t(const LineParams &P, const WsArgs &A, const PairCtx &pc) {
@@ -776,6 +777,10 @@ PB_HD void sweep_E_t(const LineParams &P, const WsArgs &A, const PairCtx &pc) {
       const size_t off = (r == 0 ? 0 : A.plane);
       bp = opp ? cscale(A.vd[off + c.sys], -A.opp_alpha) : mk(0.0, 0.0);
       if (TWO) bq = opp ? cscale(A.vd[off + d.sys], -A.opp_alpha) : mk(0.0, 0.0);
+      if (A.act) {  // wall actuation, the same value at both walls
+        bp = cadd(bp, A.act[c.sys]);
+        if (TWO) bq = cadd(bq, A.act[d.sys]);
+      }
       bh = 0.0;
     }
     const cplx yp = fwd_lrow<BW, MID>(g, r, L, wp, s0, bp);

@@ -311,6 +311,48 @@ int pb_x_to_y_host(pb_ctx *ctx, int32_t nfield, const double *Upad, double *U, d
 int pb_pressure_host(pb_ctx *ctx, const double *U, const double *P, double *p_out, double *vv_out);
 
 /* ---- LABS band operations (module_labs.f90) on host arrays -------------- */
+/* ---- opposition gain, wall actuation and the discrete adjoint ----------------
+ * Beside the drop-in: the reference has one opposition gain read at start-up
+ * (opp_alpha) and no actuation or adjoint.  pb_set_control sets the gain and
+ * a wall-actuation field act (nx/2 * nz complex, [kz][kx], the same value
+ * added to the wall v of both walls for (kx,kz) != (0,0): v_wall = -alpha
+ * v(y_d) + act), or none (act = NULL); a captured step is re-captured.  A
+ * non-zero gain needs the detection plane (opp_ydist in (0,1)). */
+int pb_set_control(pb_ctx *ctx, double alpha, const double *act);
+/* Transpose of the wall-normal stage of substep s at a fixed dt (the
+ * transpose of pb_wavespace_host's map, a linear map for fixed inputs):
+ * AU (9 fields, layout of pb_wavespace_host's U) holds the adjoints of u, v,
+ * w, theta (slots 0, 1, 2, 6) on entry and those of the products (0-4, 6-8)
+ * and of the mixed-type source u (5) on return; rk (3 fields) the adjoints of
+ * the new stores -> of the old ones.  vd: the forward's detection-plane v
+ * (2 planes [kz][kx] complex) or NULL; avd (2 planes) receives its adjoint,
+ * act_bar (1 plane) accumulates the actuation's, alpha_bar (nx/2 * nz
+ * doubles) receives the per-system terms of d/d alpha (all may be NULL).
+ * tau_bar, q_bar: adjoints of tau_w, q_w (last substep). */
+int pb_adj_wavespace_host(pb_ctx *ctx, int32_t s, double dt, double *AU, double *rk, const double *vd, double *avd,
+                          double *act_bar, double *alpha_bar, double tau_bar, double q_bar);
+/* Transpose of the nonlinear stage (y_to_x, products, x_to_y) at state U
+ * (4 fields, pb_set_state's layout): aprod (9 fields, pb_nonlinear_host's
+ * layout; slots 0-4, 6-8 read) -> astate (4 fields). */
+int pb_adj_nonlinear_host(pb_ctx *ctx, const double *U, const double *aprod, double *astate);
+/* Transpose of one substep at a fixed dt around the input state `state` (4
+ * fields): astate (4 fields) and rk (3 fields) hold the adjoints of the
+ * substep's output state and stores on entry, of its input state and stores
+ * on return; tau_bar, q_bar as above; *alpha_bar and act_bar (1 plane)
+ * accumulate (may be NULL).  The context's u, v, w, theta become `state`. */
+int pb_adj_substep(pb_ctx *ctx, int32_t s, double dt, const double *state, double *astate, double *rk,
+                   double tau_bar, double q_bar, double *alpha_bar, double *act_bar);
+/* Gradient of J = (1/nsteps) sum over steps of (tau_w + wq q_w) (the values
+ * after each step's last substep) over nsteps steps of fixed dt from the
+ * context's state, with respect to the opposition gain (*dJ_dalpha) and the
+ * actuation field (dJ_da, 1 plane complex): forward with a checkpoint every
+ * ckpt_every steps, then each segment recomputed from its checkpoint and
+ * reversed substep by substep.  *J receives J; the context ends at the
+ * state after the window.  Needs dt_factor = 1; allocates its saved states
+ * ((nsteps/ckpt_every + 1) * 7 + 12 ckpt_every fields) on first use. */
+int pb_gradient(pb_ctx *ctx, int32_t nsteps, int32_t ckpt_every, double dt, double wq, double *J, double *dJ_dalpha,
+                double *dJ_da);
+
 /* Packed bands A(2bw+1, n) in the reference's near-boundary format
  * (LABS_full_to_band, module_labs.f90:301-328), one after the other; vectors
  * are n complex values, one after the other.  They run the device solver's

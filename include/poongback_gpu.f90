@@ -307,6 +307,54 @@ module poongback_gpu
        complex(c_double_complex), intent(inout) :: b(*)
      end function pb_labs_bs_noalloc
 
+     ! opposition gain and wall actuation (act: nx/2*nz complex, or c_null_ptr)
+     integer(c_int) function pb_set_control(ctx, alpha, act) bind(C, name="pb_set_control")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), value :: alpha
+       type(c_ptr), value :: act
+     end function pb_set_control
+
+     ! discrete adjoint: the transposed wall-normal stage, nonlinear stage and substep
+     integer(c_int) function pb_adj_wavespace_host(ctx, s, dt, AU, rk, vd, avd, act_bar, alpha_bar, tau_bar, q_bar) &
+         bind(C, name="pb_adj_wavespace_host")
+       import :: c_int, c_int32_t, c_double, c_double_complex, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: s
+       real(c_double), value :: dt, tau_bar, q_bar
+       complex(c_double_complex), intent(inout) :: AU(*), rk(*)
+       type(c_ptr), value :: vd, avd, act_bar, alpha_bar
+     end function pb_adj_wavespace_host
+
+     integer(c_int) function pb_adj_nonlinear_host(ctx, U, aprod, astate) bind(C, name="pb_adj_nonlinear_host")
+       import :: c_int, c_double_complex, c_ptr
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: U(*), aprod(*)
+       complex(c_double_complex), intent(out) :: astate(*)
+     end function pb_adj_nonlinear_host
+
+     integer(c_int) function pb_adj_substep(ctx, s, dt, state, astate, rk, tau_bar, q_bar, alpha_bar, act_bar) &
+         bind(C, name="pb_adj_substep")
+       import :: c_int, c_int32_t, c_double, c_double_complex, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: s
+       real(c_double), value :: dt, tau_bar, q_bar
+       complex(c_double_complex), intent(in) :: state(*)
+       complex(c_double_complex), intent(inout) :: astate(*), rk(*)
+       type(c_ptr), value :: alpha_bar, act_bar
+     end function pb_adj_substep
+
+     ! gradient of the window-averaged tau_w + wq q_w w.r.t. alpha and the actuation
+     integer(c_int) function pb_gradient(ctx, nsteps, ckpt_every, dt, wq, J, dJ_dalpha, dJ_da) &
+         bind(C, name="pb_gradient")
+       import :: c_int, c_int32_t, c_double, c_double_complex, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nsteps, ckpt_every
+       real(c_double), value :: dt, wq
+       real(c_double), intent(out) :: J, dJ_dalpha
+       complex(c_double_complex), intent(out) :: dJ_da(*)
+     end function pb_gradient
+
      integer(c_int) function pb_labs_mat_mul(n, bw, A, nrhs, x, b) bind(C, name="pb_labs_mat_mul")
        import :: c_int, c_int32_t, c_double, c_double_complex
        integer(c_int32_t), value :: n, bw, nrhs

@@ -8,6 +8,7 @@
 
 #include "../../channel_passive_amd/csrc/bspline.h"
 #include "../../channel_passive_amd/csrc/solver.h"
+#include "../../channel_passive_amd/csrc/solver_adj.h"
 #include "../../include/pb_capi.h"
 
 namespace {
@@ -65,9 +66,9 @@ int emu_setup(const pb_config *cfg, int *bw, double *y, double *d0, double *d1, 
 
 // U: 9 slots, RK: 3 slots, device layout [slot][y][kz][kx]; vd: 2 planes or
 // NULL; usrc: u of the substep's input state (one field) under the mixed-type
-// scalar condition, else NULL
-int emu_wavespace(const pb_config *cfg, int s, double dt, double *U, double *RK, const double *vd,
-                  const double *usrc, double *mean4) {
+// scalar condition, else NULL; act: the wall actuation (one plane) or NULL
+int emu_wavespace_act(const pb_config *cfg, int s, double dt, double *U, double *RK, const double *vd,
+                      const double *usrc, const double *act, double *mean4) {
   Setup S;
   if (make_setup(cfg, S)) return -1;
   pb::LineParams P = params(cfg, S, s, dt);
@@ -81,7 +82,39 @@ int emu_wavespace(const pb_config *cfg, int s, double dt, double *U, double *RK,
   A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha; A.mean_out = mean4;
   A.usrc = cfg->scalar_bc == PB_SCALAR_MIXED ? (const pb::cplx *)usrc : nullptr;
   A.src_scale = cfg->nu / cfg->pr / cfg->bulk_u;
+  A.act = (const pb::cplx *)act;
   for (size_t sys = 0; sys < plane; ++sys) pb::wavespace_line(P, A, sys);
+  return 0;
+}
+
+int emu_wavespace(const pb_config *cfg, int s, double dt, double *U, double *RK, const double *vd,
+                  const double *usrc, double *mean4) {
+  return emu_wavespace_act(cfg, s, dt, U, RK, vd, usrc, nullptr, mean4);
+}
+
+// Adjoint of the wall-normal stage (solver_adj.h), device layout.  On entry
+// AU slots 0,1,2,6 hold the adjoints of u, v, w, theta out and ARK those of
+// the new stores; on return AU slots 0-4, 6-8 the adjoints of the products,
+// slot 5 that of the mixed-type source u, ARK those of the old stores.
+// vd: the forward's detection-plane v (2 planes) or NULL; avd (2 planes),
+// aact (1 plane, accumulated) and aalpha (1 plane) may be NULL.
+int emu_adj_wavespace(const pb_config *cfg, int s, double dt, double *AU, double *ARK, const double *vd, double *avd,
+                      double *aact, double *aalpha, double tau_bar, double q_bar) {
+  Setup S;
+  if (make_setup(cfg, S)) return -1;
+  pb::LineParams P = params(cfg, S, s, dt);
+  const int nxh = (int)cfg->nx / 2, nz = (int)cfg->nz;
+  const size_t plane = (size_t)nz * nxh, field = plane * S.n;
+  std::vector<pb::cplx> abuf(6 * field);
+  std::vector<double> dbuf((size_t)S.W * field);
+  pb::AdjArgs A;
+  A.AU = (pb::cplx *)AU; A.ARK = (pb::cplx *)ARK; A.abuf = abuf.data(); A.dbuf = dbuf.data();
+  A.vd = (const pb::cplx *)vd; A.avd = (pb::cplx *)avd; A.aact = (pb::cplx *)aact; A.aalpha = aalpha;
+  A.mixed = cfg->scalar_bc == PB_SCALAR_MIXED; A.ks = cfg->nu / cfg->pr / cfg->bulk_u;
+  A.tau_bar = tau_bar; A.q_bar = q_bar;
+  A.plane = plane; A.field = field; A.nxh = nxh; A.nz = nz; A.nh = nz / 2;
+  A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha;
+  for (size_t sys = 0; sys < plane; ++sys) pb::adj_wavespace_line(P, A, sys);
   return 0;
 }
 }
@@ -107,8 +140,8 @@ static void run_id(int id, const pb::LineParams &P, const pb::WsArgs &A, double 
   }
 }
 
-extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, double *U, double *RK, const double *vd,
-                                   const double *usrc, double *mean4) {
+extern "C" int emu_wavespace_fused_act(const pb_config *cfg, int s, double dt, double *U, double *RK,
+                                       const double *vd, const double *usrc, const double *act, double *mean4) {
   Setup S;
   if (make_setup(cfg, S)) return -1;
   pb::LineParams P = params(cfg, S, s, dt);
@@ -122,6 +155,7 @@ extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, doubl
   A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha; A.mean_out = mean4;
   A.usrc = cfg->scalar_bc == PB_SCALAR_MIXED ? (const pb::cplx *)usrc : nullptr;
   A.src_scale = cfg->nu / cfg->pr / cfg->bulk_u;
+  A.act = (const pb::cplx *)act;
   if (S.bw < 3 || S.bw > 5) return -2;
   pb::wavespace_line(P, A, 0);  // (0,0): mean modes
   // sweep-major, as the device runs them: each sweep over all systems
@@ -134,6 +168,10 @@ extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, doubl
       if (S.bw == 5) run_id<5>(ids[q], P, A, dbuf2.data(), idx);
     }
   return 0;
+}
+extern "C" int emu_wavespace_fused(const pb_config *cfg, int s, double dt, double *U, double *RK, const double *vd,
+                                   const double *usrc, double *mean4) {
+  return emu_wavespace_fused_act(cfg, s, dt, U, RK, vd, usrc, nullptr, mean4);
 }
 
 // Opposition-control detection planes with the device's per-system code.
