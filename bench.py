@@ -164,6 +164,54 @@ def ensemble_leg(steps, warmup, reps=3):
             "tuned_channel_ms": [round(float(x), 4) for x in tuned[:-1]], "tuned_ensemble_ms": float(tuned[-1])}
 
 
+def adjoint_leg(steps, ckpt_every=20, warmup=5):
+    """The differentiation workload (Re_tau = 180, 192x129x160, opposition
+    gain 0.8 at y+ = 10, a seeded Gaussian wall actuation of std 1e-3, the
+    mixed-type scalar condition): J = window mean of tau_w + 0.5 q_w over
+    `steps` steps and its gradient with respect to the gain and the
+    actuation field (pb_gradient: forward with checkpoints every ckpt_every
+    steps, recomputation, the transposed substeps), timed against the same
+    window stepped forward alone."""
+    import dataclasses
+    import numpy as np
+    from channel_passive_amd import capi
+    from channel_passive_amd.config import preset
+    from channel_passive_amd.initial import make_state
+
+    cfg = dataclasses.replace(preset("re180_192"), opp_alpha=0.8, opp_yplus=10.0)
+    ch = capi.Channel(cfg)
+    rng = np.random.default_rng(0xC0FFEE + 4)
+    act = 1e-3 * (rng.standard_normal((cfg.nz, cfg.nxh)) + 1j * rng.standard_normal((cfg.nz, cfg.nxh)))
+    act[0, 0] = 0.0
+    ch.set_control(cfg.opp_alpha, act)
+    ch.set_state(make_state(cfg, seed=0xC0FFEE + 4))
+    ch.init_stores()
+    for _ in range(warmup):
+        ch.step(cfg.dt)
+    S0, RK0 = ch.get_state()
+    ch.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ch.step(cfg.dt)
+    ch.sync()
+    t_fwd = time.perf_counter() - t0
+    ch.set_state(S0, RK0)
+    ch.gradient(2, cfg.dt, ckpt_every=1)  # allocations and first launches outside the timing
+    ch.set_state(S0, RK0)
+    ch.sync()
+    t0 = time.perf_counter()
+    J, dJa, g = ch.gradient(steps, cfg.dt, ckpt_every=ckpt_every)
+    t_grad = time.perf_counter() - t0
+    ch.close()
+    print(f"[bench] adjoint: forward {1e3 * t_fwd / steps:.3f} ms/step, gradient {1e3 * t_grad / steps:.3f} ms/step",
+          file=sys.stderr, flush=True)
+    return {"workload": f"Re_tau=180 {cfg.nx}x{cfg.ny}x{cfg.nz}, alpha={cfg.opp_alpha} at y+={cfg.opp_yplus:g}, "
+                        "wall actuation std 1e-3, mixed-type scalar condition",
+            "window_steps": steps, "ckpt_every": ckpt_every, "dt": cfg.dt,
+            "forward_ms_per_step": 1e3 * t_fwd / steps, "gradient_ms_per_step": 1e3 * t_grad / steps,
+            "cost_ratio": t_grad / t_fwd, "J": J, "dJ_dalpha": dJa, "dJ_da_max_abs": float(np.abs(g).max())}
+
+
 ENSEMBLE_HW_QUEUES = "16"
 
 
@@ -196,6 +244,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 24-channel ensemble comparison")
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (discrete adjoint) leg")
+    ap.add_argument("--adjoint-steps", type=int, default=400)
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the gradient leg and print its JSON")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the ensemble comparison and print its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -207,6 +258,9 @@ def main():
         return
     if args.ensemble_only:
         print(json.dumps(ensemble_leg(args.ensemble_steps, 10)))
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(args.adjoint_steps)))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the library runs on one GPU "
@@ -312,6 +366,11 @@ def main():
         "stages_roofline_frac": stage_frac,
         "cpu_baseline": cpu,
     }
+    if not args.no_adjoint:
+        try:
+            out["adjoint"] = adjoint_leg(args.adjoint_steps)
+        except Exception as exc:  # reported, not fatal: the headline stands
+            out["adjoint"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not args.no_ensemble:
         out["ensemble"] = ensemble_subprocess(args.ensemble_steps)
     print(json.dumps(out))

@@ -101,12 +101,17 @@ struct pb_ctx {
   // discrete adjoint (allocated on first use): adjoints of the 9 slots, of
   // the stores, 6 scratch columns per system, the padded product / state
   // adjoints (8 fields), of vd (2 planes), of the actuation (accumulated)
-  // and per-system d/d alpha terms (accumulated); adj_states: saved states
-  // of the gradient driver (checkpoints and one segment's substep inputs)
+  // and per-system d/d alpha terms (accumulated); adj_states: what the
+  // gradient driver saves (checkpoints, one segment's substep inputs),
+  // adj_nstates double2 elements
   double2 *adj_U = nullptr, *adj_RK = nullptr, *adj_buf = nullptr, *adj_pad = nullptr, *adj_vd = nullptr,
           *adj_act = nullptr, *adj_states = nullptr;
   double *adj_alpha = nullptr;
   size_t adj_nstates = 0;
+  // the transposed substeps' matrices for s = 1..3 at dt adj_fac_dt:
+  // [luh_s, lut_s (W fields each), hom_s (4 fields)] x 3, then lup (W fields)
+  double *adj_fac = nullptr;
+  double adj_fac_dt = 0.0;
   double *dbuf = nullptr, *dbuf2 = nullptr, *rbuf = nullptr, *vpart = nullptr, *scal = nullptr;
   double2 *ppe_bc = nullptr;  // statistics time: pressure wall values [2][kz][kx]
   double *dcenter = nullptr;  // B_k(0): centre-line values of the basis (get_statistics.f90:566-570)
@@ -175,7 +180,7 @@ static int ctx_free(pb_ctx *c) {
                   c->dD0LU, c->dinteg, c->ddyr, c->drows, c->twz, c->twx, c->slotmap, c->mean_db,
                   c->ppe_bc, c->dvv, c->dcenter, c->dstat, c->usrc, c->stat_uvw, c->dmom, c->dmsum, c->dpv,
                   c->dstat2, c->dstat2_tmp, c->act, c->adj_U, c->adj_RK, c->adj_buf, c->adj_pad, c->adj_vd,
-                  c->adj_act, c->adj_states, c->adj_alpha};
+                  c->adj_act, c->adj_states, c->adj_alpha, c->adj_fac};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -1662,7 +1667,8 @@ static int ensure_adj(pb_ctx *c) {
   int r;
   if ((r = alloc((void **)&c->adj_U, 9 * c->field * sizeof(double2))) ||
       (r = alloc((void **)&c->adj_RK, 3 * c->field * sizeof(double2))) ||
-      (r = alloc((void **)&c->adj_buf, 6 * c->field * sizeof(double2))) ||
+      (r = alloc((void **)&c->adj_buf, pb::ADJ_SCRATCH * c->field * sizeof(double2))) ||
+      (r = alloc((void **)&c->adj_fac, (size_t)(3 * (2 * c->W + 4) + c->W) * c->field * sizeof(double))) ||
       (r = alloc((void **)&c->adj_pad, 8 * c->pfield * sizeof(double2))) ||
       (r = alloc((void **)&c->adj_vd, 2 * c->plane * sizeof(double2))) ||
       (r = alloc((void **)&c->adj_act, c->plane * sizeof(double2))) ||
@@ -1689,9 +1695,9 @@ static int launch_zmap_fwd_t(pb_ctx *c, const double2 *src, double2 *dst, int nf
   return PB_OK;
 }
 template <class F>
-static int launch_xadj_t(pb_ctx *c) {
+static int launch_xadj_t(pb_ctx *c, const double2 *fpad) {
   const size_t ldsx = F::lds_elems(c->nx32, 2) * sizeof(double2);
-  hipLaunchKernelGGL(pb::xadj_kernel<F>, dim3(c->nz32, c->ny), dim3(F::T), ldsx, c->stream, c->pad, c->adj_pad,
+  hipLaunchKernelGGL(pb::xadj_kernel<F>, dim3(c->nz32, c->ny), dim3(F::T), ldsx, c->stream, fpad, c->adj_pad,
                      c->px, c->twx, c->nxh, c->nz32, c->pfield);
   HIPCHK(hipGetLastError());
   return PB_OK;
@@ -1714,31 +1720,75 @@ static int launch_zmap_fwd(pb_ctx *c, const double2 *src, double2 *dst, int nf, 
   else PB_EPT_DISPATCH(ez, r = launch_zmap_fwd_t<F>(c, src, dst, nf, map, scale));
   return r;
 }
-static int launch_xadj(pb_ctx *c) {
+static int launch_xadj(pb_ctx *c, const double2 *fpad) {
   int r = PB_OK;
   const int ex = pick_ept((size_t)c->nx32 * 2);
-  if (ct_x(c)) r = launch_xadj_t<pb::FftX1536>(c);
-  else PB_EPT_DISPATCH(ex, r = launch_xadj_t<F>(c));
+  if (ct_x(c)) r = launch_xadj_t<pb::FftX1536>(c, fpad);
+  else PB_EPT_DISPATCH(ex, r = launch_xadj_t<F>(c, fpad));
   return r;
 }
 
-// Transpose of the nonlinear stage for the state in U slots 0,1,2,6: the
-// product adjoints in adj_U slots 0-4, 6-8 -> the state adjoints in adj_U
-// slots 0,1,2,6 (the z inverse of the state is left in pad fields 0-3).
-static int adj_nonlinear_dev(pb_ctx *c) {
+// Transpose of the nonlinear stage around a state: the product adjoints in
+// adj_U slots 0-4, 6-8 -> the state adjoints in adj_U slots 0,1,2,6.  fpad:
+// the state's z inverse (4 padded fields) saved by the forward, or null:
+// then it is formed here from U slots 0,1,2,6 into pad fields 0-3.
+static int adj_nonlinear_dev(pb_ctx *c, const double2 *fpad) {
   int r;
-  if ((r = launch_zinv_any(c))) return r;
+  if (!fpad) {
+    if ((r = launch_zinv_any(c))) return r;
+    fpad = c->pad;
+  }
   if ((r = launch_zmap_inv(c, c->adj_U, c->adj_pad, 8, pb::MAP_PROD & 0xFFFFFFFFull, c->rescale))) return r;
-  if ((r = launch_xadj(c))) return r;
+  if ((r = launch_xadj(c, fpad))) return r;
   return launch_zmap_fwd(c, c->adj_pad, c->adj_U, 4, pb::MAP_STATE, 1.0);
 }
 
-static pb::AdjArgs adj_args(pb_ctx *c, const double2 *vd, double tau_bar, double q_bar) {
+static pb::AdjFactors adj_factors(pb_ctx *c, int s) {
+  pb::AdjFactors F;
+  const size_t f = c->field, per = (size_t)(2 * c->W + 4) * f;
+  F.luh = c->adj_fac + (size_t)(s - 1) * per;
+  F.lut = F.luh + (size_t)c->W * f;
+  F.hom = F.lut + (size_t)c->W * f;
+  F.lup = c->adj_fac + 3 * per;
+  F.plane = c->plane;
+  F.field = f;
+  F.W = c->W;
+  return F;
+}
+// the factors of the three substeps at dt (formed again when dt changes)
+static int adj_prepare(pb_ctx *c, double dt) {
+  if (c->adj_fac_dt == dt) return PB_OK;
+  for (int s = 1; s <= c->cfg.rk_step; ++s) {
+    const pb::LineParams P = c->line_params(s, dt);
+    hipLaunchKernelGGL(pb::adj_factor_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P,
+                       adj_factors(c, s), c->nsys, c->nxh, c->nh, c->lmfxr, c->lmfzr, (pb::cplx *)c->adj_buf);
+    HIPCHK(hipGetLastError());
+  }
+  c->adj_fac_dt = dt;
+  return PB_OK;
+}
+// The transposed wall-normal stage: the three chains of every system, the
+// four parts of the transposed hvhg, their elementwise combination.  Built
+// for bandwidth 4 (the order-7 B-splines of every benchmark configuration).
+static int launch_adj_ws(pb_ctx *c, const pb::LineParams &P, const pb::AdjArgs &A) {
+  if (c->bw != 4) return fail(PB_E_ARG, "the adjoint is built for bandwidth 4 (order-7 B-splines), not %d", c->bw);
+  const unsigned nb = (unsigned)blocks_for(c->nsys, 64);
+  hipLaunchKernelGGL(pb::adj_chain_kernel<4>, dim3(nb, 3), dim3(64), 0, c->stream, P, A, c->nsys);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(pb::adj_hvhg_part_kernel<4>, dim3(nb, 4), dim3(64), 0, c->stream, P, A, c->nsys);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(pb::adj_hvhg_combine_kernel, dim3(blocks_for(c->field, 256)), dim3(256), 0, c->stream, A,
+                     c->field);
+  HIPCHK(hipGetLastError());
+  return PB_OK;
+}
+
+static pb::AdjArgs adj_args(pb_ctx *c, int s, const double2 *vd, double tau_bar, double q_bar) {
   pb::AdjArgs A;
   A.AU = (pb::cplx *)c->adj_U;
   A.ARK = (pb::cplx *)c->adj_RK;
   A.abuf = (pb::cplx *)c->adj_buf;
-  A.dbuf = c->dbuf;
+  A.F = adj_factors(c, s);
   A.vd = (const pb::cplx *)vd;
   A.avd = (pb::cplx *)c->adj_vd;
   A.aact = (pb::cplx *)c->adj_act;
@@ -1763,15 +1813,20 @@ static pb::AdjArgs adj_args(pb_ctx *c, const double2 *vd, double tau_bar, double
 // state and stores on entry, of its input state and stores on return;
 // adj_act and adj_alpha accumulate.  tau_bar, q_bar: adjoints of tau_w, q_w
 // (used on the last substep).
-static int adj_substep_dev(pb_ctx *c, int s, double dt, double tau_bar, double q_bar) {
+// fpad, fvd: the input state's z inverse and detection planes saved by the
+// forward, or null: then they are formed from U slots 0,1,2,6.
+static int adj_substep_dev(pb_ctx *c, int s, double dt, double tau_bar, double q_bar, const double2 *fpad = nullptr,
+                           const double2 *fvd = nullptr) {
   int r;
   const bool rows = !c->opp_rows.empty();
-  if (rows && (r = launch_oppvd(c, c->U))) return r;
-  pb::LineParams P = c->line_params(s, dt);
-  const pb::AdjArgs A = adj_args(c, rows ? c->vd : nullptr, tau_bar, q_bar);
-  hipLaunchKernelGGL(pb::adj_ws_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P, A, c->nsys);
-  HIPCHK(hipGetLastError());
-  if ((r = adj_nonlinear_dev(c))) return r;
+  if (rows && !fvd) {
+    if ((r = launch_oppvd(c, c->U))) return r;
+    fvd = c->vd;
+  }
+  if ((r = adj_prepare(c, dt))) return r;
+  const pb::LineParams P = c->line_params(s, dt);
+  if ((r = launch_adj_ws(c, P, adj_args(c, s, rows ? fvd : nullptr, tau_bar, q_bar)))) return r;
+  if ((r = adj_nonlinear_dev(c, fpad))) return r;
   const bool opp = c->cfg.opp_alpha != 0.0, mixed = c->cfg.scalar_bc == PB_SCALAR_MIXED;
   if (opp || mixed) {
     hipLaunchKernelGGL(pb::adj_finish_kernel, dim3(blocks_for(c->field, 256)), dim3(256), 0, c->stream,
@@ -1813,11 +1868,9 @@ int pb_adj_wavespace_host(pb_ctx *c, int32_t s, double dt, double *AU, double *a
   static const int map9[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
   if ((r = upload_ref(c, AU, 9, map9, c->adj_U)) || (r = upload_ref(c, ark, 3, kRKMap, c->adj_RK))) return r;
   if (vd) HIPCHK(hipMemcpyAsync(c->vd, vd, 2 * c->plane * sizeof(double2), hipMemcpyHostToDevice, c->stream));
-  if ((r = adj_clear_controls(c, act_bar))) return r;
-  pb::LineParams P = c->line_params(s, dt);
-  const pb::AdjArgs A = adj_args(c, vd ? c->vd : nullptr, tau_bar, q_bar);
-  hipLaunchKernelGGL(pb::adj_ws_kernel, dim3(blocks_for(c->nsys, 64)), dim3(64), 0, c->stream, P, A, c->nsys);
-  HIPCHK(hipGetLastError());
+  if ((r = adj_clear_controls(c, act_bar)) || (r = adj_prepare(c, dt))) return r;
+  const pb::LineParams P = c->line_params(s, dt);
+  if ((r = launch_adj_ws(c, P, adj_args(c, s, vd ? c->vd : nullptr, tau_bar, q_bar)))) return r;
   if ((r = download_ref(c, AU, 9, map9, c->adj_U)) || (r = download_ref(c, ark, 3, kRKMap, c->adj_RK))) return r;
   if (avd) HIPCHK(hipMemcpy(avd, c->adj_vd, 2 * c->plane * sizeof(double2), hipMemcpyDeviceToHost));
   return adj_read_controls(c, nullptr, alpha_bar, act_bar);
@@ -1830,7 +1883,7 @@ int pb_adj_nonlinear_host(pb_ctx *c, const double *U, const double *aprod, doubl
   static const int map9[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
   if ((r = upload_ref(c, U, 4, kStateMap, c->U)) || (r = upload_ref(c, aprod, 9, map9, c->adj_U))) return r;
   c->have_state = true;
-  if ((r = adj_nonlinear_dev(c))) return r;
+  if ((r = adj_nonlinear_dev(c, nullptr))) return r;
   return download_ref(c, astate, 4, kStateMap, c->adj_U);
 }
 
@@ -1873,6 +1926,25 @@ static int restore_state(pb_ctx *c, const double2 *src, bool with_rk) {
   return with_rk ? copy_slots(c, c->RK, src + 4 * c->field, rk, 3, false) : PB_OK;
 }
 
+// substep s of the forward, keeping what its transpose needs: the input
+// state's z inverse (pad fields 0-3 after the z inverse) into pad_dst and
+// its detection planes into vd_dst (when the channel has them)
+static int substep_saving(pb_ctx *c, int s, double2 *pad_dst, double2 *vd_dst) {
+  int r;
+  if (!c->opp_rows.empty()) {
+    if ((r = launch_oppvd(c, c->U))) return r;
+    HIPCHK(hipMemcpyAsync(vd_dst, c->vd, 2 * c->plane * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+  }
+  if (c->usrc)
+    HIPCHK(hipMemcpyAsync(c->usrc, c->U, c->field * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+  if ((r = launch_zinv_any(c))) return r;
+  HIPCHK(hipMemcpyAsync(pad_dst, c->pad, 4 * c->pfield * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+  if ((r = launch_xphys_any(c, 0)) || (r = launch_velmax(c, s == 1 ? (pb::VM_STORE | pb::VM_DT) : 0)) ||
+      (r = launch_zfwd_any(c, 0, c->rescale)))
+    return r;
+  return launch_wavespace(c, s);
+}
+
 int pb_gradient(pb_ctx *c, int32_t nsteps, int32_t ckpt_every, double dt, double wq, double *J, double *dJ_dalpha,
                 double *dJ_da) {
   if (!c) return fail(PB_E_ARG, "null context");
@@ -1881,22 +1953,33 @@ int pb_gradient(pb_ctx *c, int32_t nsteps, int32_t ckpt_every, double dt, double
   if (c->cfg.dt_factor != 1.0) return fail(PB_E_ARG, "the adjoint needs a fixed dt (dt_factor = 1)");
   int r;
   if ((r = check_dt(dt)) || (r = ensure_adj(c))) return r;
-  const int S = c->cfg.rk_step, C = ckpt_every, nck = (nsteps + C - 1) / C;
-  // checkpoints (7 fields each), the final state (7), one segment's substep inputs (4 each)
-  const size_t need = (size_t)(nck + 1) * 7 + (size_t)C * S * 4;
+  const int S = c->cfg.rk_step, C = std::min(ckpt_every, nsteps), nck = (nsteps + C - 1) / C;
+  // saved data, in double2 units: checkpoints (7 fields each) and the final
+  // state (7 fields), then per substep of one segment either the z inverse
+  // of its input state and its detection planes (4 padded fields + 2
+  // planes), or its input state (4 fields) when that does not fit in a
+  // quarter of the free memory (or PB_ADJ_SAVE=state)
+  const size_t fixed = (size_t)(nck + 1) * 7 * c->field;
+  const size_t per_pad = 4 * c->pfield + 2 * c->plane, per_state = 4 * c->field;
+  size_t free_b = 0, total_b = 0;
+  (void)hipMemGetInfo(&free_b, &total_b);
+  const char *mode = std::getenv("PB_ADJ_SAVE");
+  const bool force_state = mode && std::strcmp(mode, "state") == 0;
+  const size_t held = c->adj_nstates * sizeof(double2);
+  const bool pad_path = !force_state && (fixed + (size_t)C * S * per_pad) * sizeof(double2) <= (free_b + held) / 4;
+  const size_t need = fixed + (size_t)C * S * (pad_path ? per_pad : per_state);
   if (c->adj_nstates < need) {
     if (c->adj_states) {
       (void)hipFree(c->adj_states);
-      c->bytes -= c->adj_nstates * c->field * sizeof(double2);
+      c->bytes -= c->adj_nstates * sizeof(double2);
       c->adj_states = nullptr;
       c->adj_nstates = 0;
     }
-    const hipError_t e = hipMalloc((void **)&c->adj_states, need * c->field * sizeof(double2));
+    const hipError_t e = hipMalloc((void **)&c->adj_states, need * sizeof(double2));
     if (e != hipSuccess)
-      return fail(PB_E_HIP, "hipMalloc(%zu) for %zu saved fields: %s", need * c->field * sizeof(double2), need,
-                  hipGetErrorString(e));
+      return fail(PB_E_HIP, "hipMalloc(%zu) for the saved states: %s", need * sizeof(double2), hipGetErrorString(e));
     c->adj_nstates = need;
-    c->bytes += need * c->field * sizeof(double2);
+    c->bytes += need * sizeof(double2);
   }
   double2 *ckpt = c->adj_states, *fin = ckpt + (size_t)nck * 7 * c->field, *seg = fin + 7 * c->field;
   std::vector<pb_substep_out> out(S);
@@ -1924,15 +2007,38 @@ int pb_gradient(pb_ctx *c, int32_t nsteps, int32_t ckpt_every, double dt, double
     for (int k = k0; k < k1; ++k) {
       if ((r = upload_dt(c, dt))) return r;
       for (int s = 1; s <= S; ++s) {
-        if ((r = save_state(c, seg + (size_t)((k - k0) * S + s - 1) * 4 * c->field, false))) return r;
-        if (k == k1 - 1 && s == S) break;  // the last substep's output is not needed
-        if ((r = substep_async(c, s))) return r;
+        const size_t i = (size_t)(k - k0) * S + s - 1;
+        if (pad_path) {
+          double2 *p = seg + i * per_pad;
+          if (k == k1 - 1 && s == S) {  // the last substep: only its input's z inverse and planes
+            if (!c->opp_rows.empty()) {
+              if ((r = launch_oppvd(c, c->U))) return r;
+              HIPCHK(hipMemcpyAsync(p + 4 * c->pfield, c->vd, 2 * c->plane * sizeof(double2),
+                                    hipMemcpyDeviceToDevice, c->stream));
+            }
+            if ((r = launch_zinv_any(c))) return r;
+            HIPCHK(hipMemcpyAsync(p, c->pad, 4 * c->pfield * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+            break;
+          }
+          if ((r = substep_saving(c, s, p, p + 4 * c->pfield))) return r;
+        } else {
+          if ((r = save_state(c, seg + i * per_state, false))) return r;
+          if (k == k1 - 1 && s == S) break;  // the last substep's output is not needed
+          if ((r = substep_async(c, s))) return r;
+        }
       }
     }
     for (int k = k1 - 1; k >= k0; --k)
       for (int s = S; s >= 1; --s) {
-        if ((r = restore_state(c, seg + (size_t)((k - k0) * S + s - 1) * 4 * c->field, false))) return r;
-        if ((r = adj_substep_dev(c, s, dt, s == S ? tb : 0.0, s == S ? qb : 0.0))) return r;
+        const size_t i = (size_t)(k - k0) * S + s - 1;
+        const double a = s == S ? tb : 0.0, b = s == S ? qb : 0.0;
+        if (pad_path) {
+          const double2 *p = seg + i * per_pad;
+          if ((r = adj_substep_dev(c, s, dt, a, b, p, c->opp_rows.empty() ? nullptr : p + 4 * c->pfield))) return r;
+        } else {
+          if ((r = restore_state(c, seg + i * per_state, false))) return r;
+          if ((r = adj_substep_dev(c, s, dt, a, b))) return r;
+        }
       }
   }
   double asum = 0.0;

@@ -1222,11 +1222,32 @@ __global__ void __launch_bounds__(64) wavespace_kernel(LineParams P, WsArgs A, s
 }
 
 // Transpose of the wall-normal stage, one thread per system (solver_adj.h);
-// P.dt is the fixed dt of the substep.
-__global__ void __launch_bounds__(64) adj_ws_kernel(LineParams P, AdjArgs A, size_t nsys) {
+// P.dt is the fixed dt of the substep, A.F its factors.
+// Phase 1: grid.y = 3 chains; phase 2: the transpose of hvhg.
+template <int BW>
+__global__ void __launch_bounds__(64) adj_chain_kernel(LineParams P, AdjArgs A, size_t nsys) {
   const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sys >= nsys) return;
-  adj_wavespace_line(P, A, sys);
+  adj_chain<BW>(P, A, sys, (int)blockIdx.y);
+}
+template <int BW>
+__global__ void __launch_bounds__(64) adj_hvhg_part_kernel(LineParams P, AdjArgs A, size_t nsys) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  adj_hvhg_part<BW>(P, A, sys, (int)blockIdx.y);
+}
+__global__ void adj_hvhg_combine_kernel(AdjArgs A, size_t field) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= field) return;
+  adj_hvhg_combine(A, e % A.plane, (int)(e / A.plane));
+}
+// The factors of substep P.s at dt P.dt for every system; scr: 2 complex
+// scratch columns per system.
+__global__ void __launch_bounds__(64) adj_factor_kernel(LineParams P, AdjFactors F, size_t nsys, int nxh, int nh,
+                                                        double lmfxr, double lmfzr, cplx *scr) {
+  const size_t sys = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  adj_factor_line(P, F, sys, nxh, nh, lmfxr, lmfzr, SV{scr + sys, F.plane}, SV{scr + F.field + sys, F.plane});
 }
 
 // The last transposes of a substep, per element of the [y][kz][kx] field:

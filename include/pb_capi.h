@@ -348,8 +348,12 @@ int pb_adj_substep(pb_ctx *ctx, int32_t s, double dt, const double *state, doubl
  * actuation field (dJ_da, 1 plane complex): forward with a checkpoint every
  * ckpt_every steps, then each segment recomputed from its checkpoint and
  * reversed substep by substep.  *J receives J; the context ends at the
- * state after the window.  Needs dt_factor = 1; allocates its saved states
- * ((nsteps/ckpt_every + 1) * 7 + 12 ckpt_every fields) on first use. */
+ * state after the window.  Needs dt_factor = 1.  Allocates on first use
+ * (nsteps/ckpt_every + 1) * 7 fields of checkpoints and, per substep of one
+ * segment, the z inverse of its input state and its detection planes (4
+ * padded fields + 2 planes: no transform is repeated in the reverse pass),
+ * or, when that exceeds a quarter of the free device memory or the
+ * environment sets PB_ADJ_SAVE=state, its input state (4 fields). */
 int pb_gradient(pb_ctx *ctx, int32_t nsteps, int32_t ckpt_every, double dt, double wq, double *J, double *dJ_dalpha,
                 double *dJ_da);
 

@@ -105,16 +105,27 @@ int emu_adj_wavespace(const pb_config *cfg, int s, double dt, double *AU, double
   pb::LineParams P = params(cfg, S, s, dt);
   const int nxh = (int)cfg->nx / 2, nz = (int)cfg->nz;
   const size_t plane = (size_t)nz * nxh, field = plane * S.n;
-  std::vector<pb::cplx> abuf(6 * field);
-  std::vector<double> dbuf((size_t)S.W * field);
+  if (S.bw < 3 || S.bw > 5) return -2;
+  std::vector<pb::cplx> abuf(pb::ADJ_SCRATCH * field);
+  std::vector<double> fac((size_t)(3 * S.W + 4) * field);
+  pb::AdjFactors F;
+  F.luh = fac.data(); F.lut = F.luh + (size_t)S.W * field; F.lup = F.lut + (size_t)S.W * field;
+  F.hom = F.lup + (size_t)S.W * field; F.plane = plane; F.field = field; F.W = S.W;
+  for (size_t sys = 0; sys < plane; ++sys)
+    pb::adj_factor_line(P, F, sys, nxh, nz / 2, 1.0 / cfg->lmfx, 1.0 / cfg->lmfz, pb::SV{abuf.data() + sys, plane},
+                        pb::SV{abuf.data() + field + sys, plane});
   pb::AdjArgs A;
-  A.AU = (pb::cplx *)AU; A.ARK = (pb::cplx *)ARK; A.abuf = abuf.data(); A.dbuf = dbuf.data();
+  A.AU = (pb::cplx *)AU; A.ARK = (pb::cplx *)ARK; A.abuf = abuf.data(); A.F = F;
   A.vd = (const pb::cplx *)vd; A.avd = (pb::cplx *)avd; A.aact = (pb::cplx *)aact; A.aalpha = aalpha;
   A.mixed = cfg->scalar_bc == PB_SCALAR_MIXED; A.ks = cfg->nu / cfg->pr / cfg->bulk_u;
   A.tau_bar = tau_bar; A.q_bar = q_bar;
   A.plane = plane; A.field = field; A.nxh = nxh; A.nz = nz; A.nh = nz / 2;
   A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha;
-  for (size_t sys = 0; sys < plane; ++sys) pb::adj_wavespace_line(P, A, sys);
+  for (size_t sys = 0; sys < plane; ++sys) {
+    if (S.bw == 3) pb::adj_wavespace_line<3>(P, A, sys);
+    if (S.bw == 4) pb::adj_wavespace_line<4>(P, A, sys);
+    if (S.bw == 5) pb::adj_wavespace_line<5>(P, A, sys);
+  }
   return 0;
 }
 }

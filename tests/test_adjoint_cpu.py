@@ -99,3 +99,26 @@ def test_wavespace_adjoint_dot(emu_lib, name, alpha, sbc, s):
         dl += (ma[2] - mb[2]) * tb + (ma[3] - mb[3]) * qb
     # the per-system terms cancel heavily: bound the rounding by their magnitudes
     assert abs(dl - aal.sum()) <= 1e-12 * (np.abs(aal).sum() + abs(dl)), (dl, aal.sum())
+
+
+def test_dot_test_detects_a_wrong_transpose(emu_lib):
+    """The identity is sharp: the transpose of another substep (other RK
+    coefficients, a plausible slip) misses it at O(1)."""
+    cfg = dataclasses.replace(preset("tiny"), opp_alpha=0.8, opp_yplus=15.0)
+    nxh, nz, ny = cfg.nx // 2, cfg.nz, cfg.ny
+    plane = nz * nxh
+    rng = np.random.default_rng(1)
+    U = crand(rng, 9, ny, nz, nxh)
+    U[5] = 0.0
+    RK = crand(rng, 3, ny, nz, nxh)
+    vd = crand(rng, 2 * plane)
+    Ux, Rx, _ = forward(emu_lib, cfg, 2, U, RK, vd, None, None)
+    z = lambda a: np.zeros_like(a)
+    U0, R0, _ = forward(emu_lib, cfg, 2, z(U), z(RK), z(vd), None, None)
+    AU = crand(rng, 9, ny, nz, nxh)
+    ARK = crand(rng, 3, ny, nz, nxh)
+    lhs = sum(rdot(Ux[f] - U0[f], AU[f]) for f in OUT) + rdot(Rx - R0, ARK)
+    for s_wrong in (1, 3):
+        PU, PR, avd, _, _ = adjoint(emu_lib, cfg, s_wrong, AU, ARK, vd, 0.0, 0.0)
+        rhs = sum(rdot(U[f], PU[f]) for f in PROD) + rdot(RK, PR) + rdot(vd, avd)
+        assert abs(lhs - rhs) > 1e-3 * abs(lhs), (s_wrong, lhs, rhs)
