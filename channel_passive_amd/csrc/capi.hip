@@ -381,8 +381,12 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
       rc = rc ? rc : dalloc((void **)&c->mean_db, 3 * bandb);
     }
     if (!rc && c->mean_lds > 65536 &&
-        hipFuncSetAttribute(reinterpret_cast<const void *>(&pb::mean_kernel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->mean_lds) != hipSuccess)
+        (hipFuncSetAttribute(reinterpret_cast<const void *>(&pb::mean_kernel<3>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->mean_lds) != hipSuccess ||
+         hipFuncSetAttribute(reinterpret_cast<const void *>(&pb::mean_kernel<4>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->mean_lds) != hipSuccess ||
+         hipFuncSetAttribute(reinterpret_cast<const void *>(&pb::mean_kernel<5>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->mean_lds) != hipSuccess))
       c->mean_lds = 0;
   }
   if (rc) { ctx_free(c); return rc; }
@@ -841,8 +845,11 @@ static int launch_wavespace(pb_ctx *c, int s) {
         HIPCHK(hipEventRecord(c->ev_fork, c->stream));
         HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
       }
-      hipLaunchKernelGGL(pb::mean_kernel, dim3(3), dim3(pb::MEAN_THREADS), c->mean_lds, c->linear ? c->stream : c->side,
-                         P, A, c->scal, c->mean_db);
+      hipStream_t ms = c->linear ? c->stream : c->side;
+      const dim3 g3(3), bt(pb::MEAN_THREADS);
+      if (c->bw == 3) hipLaunchKernelGGL(pb::mean_kernel<3>, g3, bt, c->mean_lds, ms, P, A, c->scal, c->mean_db);
+      if (c->bw == 4) hipLaunchKernelGGL(pb::mean_kernel<4>, g3, bt, c->mean_lds, ms, P, A, c->scal, c->mean_db);
+      if (c->bw == 5) hipLaunchKernelGGL(pb::mean_kernel<5>, g3, bt, c->mean_lds, ms, P, A, c->scal, c->mean_db);
       HIPCHK(hipGetLastError());
       if (!c->linear) HIPCHK(hipEventRecord(c->ev_join, c->side));
     }

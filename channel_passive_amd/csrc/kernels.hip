@@ -1279,16 +1279,84 @@ __global__ void wave00_kernel(LineParams P, WsArgs A, const double *scal) {
 // The (0,0) system split by field, one block each: block 0 the mean
 // streamwise velocity, 1 the spanwise velocity, 2 the scalar (the three
 // solves of wave00.f90 are independent).  The block copies its columns of
-// system 0 into LDS, thread 0 runs the recurrences there (same functions,
-// same arithmetic as the one-thread path), and the block writes back what
-// the one-thread path leaves in the state.  Db lives in LDS when
-// db_scratch is null, otherwise in db_scratch + field * W * ny.
+// system 0 into LDS and runs mean_u / mean_w / mean_t of solver.h there with
+// the same arithmetic: the band products and the elementwise updates row-
+// parallel over the block's threads (each row's sum in band_mv's order), the
+// recurrences on one thread each with register windows -- LU(D0)
+// substitutions as deriv_line forms them, the Helmholtz LU fused with the
+// forward substitution as sweep C forms it (crout_sweep with the mean
+// operator: dummy = 1 + beta * 0 = 1, so 1 * D0 - beta D2 is helmholtz_lu's
+// value), the two independent LU(D0) solves of substep 1 on two waves side
+// by side -- and writes back what the one-thread path leaves in the state.
+// U of the Helmholtz factor lives in LDS when db_scratch is null, otherwise
+// in db_scratch + field * W * ny.
 constexpr int MEAN_COLS = 6;
 constexpr int MEAN_THREADS = 256;
+
+// x <- LU(D0)^-1 x, one thread (band_bs with the shared factor)
+template <int BW>
+__device__ void mean_bs_d0(const BandGeom &g, const SharedBand &LU, const SV &x) {
+  CWin<BW> f;
+  f.clear();
+  down_rows<BW>(g.n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double lr[2 * BW + 1];
+    load_lrow<BW, MID>(g, LU, r, lr);
+    const cplx y = fwd_row<BW, MID>(g, r, lr, f, x, x[r]);
+    x[r] = y;
+    f.push(y);
+  });
+  CWin<2 * BW + 1> w;
+  w.clear();
+  up_rows<BW>(g.n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double ur[2 * BW];
+    load_urow<BW, MID>(g, LU, r, ur);
+    const cplx v = bwd_row<BW, MID>(g, r, ur, w, x[r]);
+    x[r] = v;
+    w.push(v);
+  });
+}
+// x <- H^-1 x for the mean-line Helmholtz operator D0 - beta D2 (diagonal
+// wall rows), its LU formed row by row; U kept in Uo.  One thread.
+template <int BW>
+__device__ void mean_helmholtz(const LineParams &Q, double beta, const OwnU &Uo, const SV &x) {
+  const BandGeom g{Q.g.n, BW};
+  const HelmholtzA H{Q.D0, Q.D2, 1.0 + beta * 0.0, beta, g.n, BW};
+  CWin<BW> f;
+  f.clear();
+  crout_sweep<BW>(g, H, Uo, [&](int r, const double(&L)[2 * BW + 1], auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    const cplx y = fwd_lrow<BW, MID>(g, r, L, f, x, x[r]);
+    x[r] = y;
+    f.push(y);
+  });
+  CWin<2 * BW + 1> w;
+  w.clear();
+  up_rows<BW>(g.n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double ur[2 * BW];
+    load_urow<BW, MID>(g, Uo, r, ur);
+    const cplx v = bwd_row<BW, MID>(g, r, ur, w, x[r]);
+    x[r] = v;
+    w.push(v);
+  });
+}
+// b = A x row-parallel over the block (band_mv's order within a row)
+__device__ void mean_mv(const BandGeom &g, const SharedBand &A, const SV &x, const SV &b) {
+  for (int r = threadIdx.x; r < g.n; r += blockDim.x) {
+    const int f = g.first(r), l = g.last(r);
+    cplx v = cscale(x[f], A(r, g.slot(r, f)));
+    for (int c = f + 1; c <= l; ++c) v = cadd(v, cscale(x[c], A(r, g.slot(r, c))));
+    b[r] = v;
+  }
+}
+
+template <int BW>
 __global__ void __launch_bounds__(MEAN_THREADS) mean_kernel(LineParams P, WsArgs A, const double *scal,
                                                             double *db_scratch) {
   extern __shared__ double2 lds[];
-  const int n = P.g.n, W = P.g.W(), f = blockIdx.x;
+  const int n = P.g.n, W = P.g.W(), f = blockIdx.x, tid = threadIdx.x, T = MEAN_THREADS;
   cplx *col = reinterpret_cast<cplx *>(lds);
   double *db = db_scratch ? db_scratch + (size_t)f * W * n : reinterpret_cast<double *>(lds + MEAN_COLS * n);
   auto gU = [&](int slot) { return A.U + (size_t)slot * A.field; };
@@ -1305,25 +1373,95 @@ __global__ void __launch_bounds__(MEAN_THREADS) mean_kernel(LineParams P, WsArgs
   }
   for (int q = 0; q < MEAN_COLS; ++q)
     if (src[q])
-      for (int i = threadIdx.x; i < n; i += MEAN_THREADS) col[q * n + i] = src[q][(size_t)i * A.plane];
+      for (int i = tid; i < n; i += T) col[q * n + i] = src[q][(size_t)i * A.plane];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    LineParams Q = P;
-    Q.dt = scal[0];
-    const OwnBand Db{db, 1, W};
-    auto L = [&](int q) { return SV{col + q * n, 1}; };
-    double o4[4] = {0, 0, 0, 0};
-    if (f == 0) {
-      mean_u(Q, Db, L(0), L(1), L(2), L(3), L(4), o4);
-      if (A.mean_out) { A.mean_out[0] = o4[0]; A.mean_out[2] = o4[2]; }
-    } else if (f == 1) {
-      mean_w(Q, Db, true, L(0), L(1), L(2), L(3), L(4), L(5));
-    } else {
-      const SV us = L(5);
-      mean_t(Q, Db, L(0), L(1), L(2), L(3), L(4), o4, A.usrc ? &us : nullptr, A.src_scale);
-      if (A.mean_out) { A.mean_out[1] = o4[1]; A.mean_out[3] = o4[3]; }
+  LineParams Q = P;
+  Q.dt = scal[0];
+  const BandGeom g{n, BW};
+  const OwnU Uo{db, 1, W};
+  auto L = [&](int q) { return SV{col + q * n, 1}; };
+  const int s = Q.s, S = Q.S;
+  const double dt = Q.dt, nu = Q.nu, pr = Q.pr;
+  // columns of this line: out (the mean profile), prod (the y-derivative
+  // source: uv, vw, v theta), d (D1 LU(D0)^-1 prod), st (the store), b (rhs)
+  const bool isT = f == 2, isW = f == 1;
+  const SV out = isW ? L(1) : L(0), prod = isW ? L(2) : L(1), d = isW ? L(3) : L(2), st = isW ? L(4) : L(3),
+           b = isW ? L(5) : L(4), us = L(5);
+  const bool mixed = isT && A.usrc;
+  const double beta = isT ? dt * Q.coef(1, s) * nu / pr : dt * Q.coef(1, s) * nu;
+  // substep 1: out <- LU(D0)^-1 st on wave 0 beside prod <- LU(D0)^-1 prod on wave 1
+  if (s == 1) {
+    for (int i = tid; i < n; i += T) out[i] = st[i];
+    __syncthreads();
+    if (tid == 0) mean_bs_d0<BW>(g, Q.D0LU, out);
+  }
+  if (tid == 64) mean_bs_d0<BW>(g, Q.D0LU, prod);
+  __syncthreads();
+  if (s == 1) mean_mv(g, Q.D2, out, b);
+  mean_mv(g, Q.D1, prod, d);
+  __syncthreads();
+  const double gamma = dt * Q.coef(2, s);
+  const double alpha = isT ? dt * Q.coef(0, s) * nu / pr : dt * Q.coef(0, s) * nu;
+  const double ug = f == 0 ? Q.u_gal : 0.0, one_im = mixed ? 0.0 : 1.0;
+  for (int i = tid; i < n; i += T) {
+    if (mixed) d[i] = csub(d[i], cscale(us[i], A.src_scale));
+    cplx v = mk(0.0, 0.0);
+    if (i >= 1 && i < n - 1) {
+      const cplx t = s == 1 ? csub(cadd(st[i], cscale(b[i], alpha)), cscale(d[i], gamma)) : csub(st[i], cscale(d[i], gamma));
+      v = f == 0 ? mk(t.re + ug, one_im) : mk(t.re, one_im);
+    }
+    b[i] = v;
+  }
+  __syncthreads();
+  __shared__ double phi_s;
+  if (tid == 0) {
+    mean_helmholtz<BW>(Q, beta, Uo, b);
+    double phi = 0.0;
+    if (f == 0) phi = calc_phi(Q.bulk_u, integrate(Q, b));
+    if (f == 1) {
+      phi = calc_phi(Q.bulk_w, integrate(Q, b));
+      phi = 0.0;  // wave00.f90:396
+    }
+    if (f == 2) phi = mixed ? 0.0 : calc_phi(Q.bulk_t, integrate(Q, b));
+    phi_s = phi;
+    if (A.mean_out && f != 1) {
+      A.mean_out[f == 0 ? 0 : 1] = phi;
+      A.mean_out[f == 0 ? 2 : 3] = 0.0;
+      if (s == S) {
+        // wall_d1 of the corrected column (re + phi im on rows 0, 1, n-2, n-1)
+        cplx c4[4];
+        const int rows[4] = {0, 1, n - 2, n - 1};
+        for (int q = 0; q < 4; ++q) c4[q] = mk(b[rows[q]].re + phi * b[rows[q]].im, 0.0);
+        const int m = BW;
+        const double wd = (((Q.D1(0, m) * c4[0].re + Q.D1(0, m + 1) * c4[1].re) - Q.D1(n - 1, m - 1) * c4[2].re) -
+                           Q.D1(n - 1, m) * c4[3].re);
+        A.mean_out[f == 0 ? 2 : 3] = f == 0 ? 0.5 * nu * wd : 0.5 * nu / pr * wd;
+      }
     }
   }
+  __syncthreads();
+  const double phi = phi_s;
+  for (int i = tid; i < n; i += T) b[i] = mk(b[i].re + phi * b[i].im, 0.0);
+  __syncthreads();
+  mean_mv(g, Q.D0, b, out);
+  __syncthreads();
+  if (f == 0)
+    for (int i = tid; i < n; i += T) out[i] = mk(out[i].re - Q.u_gal, out[i].im);
+  if (s != S) {
+    const double al2 = isT ? dt * Q.coef(0, s + 1) * nu / pr : dt * Q.coef(0, s + 1) * nu;
+    const double zeta = dt * Q.coef(3, s + 1);
+    // st = D2 b row by row, then st = al2 st + out - zeta d (each row reads only its own values)
+    for (int r = tid; r < n; r += T) {
+      const int fr = g.first(r), lr = g.last(r);
+      cplx v = cscale(b[fr], Q.D2(r, g.slot(r, fr)));
+      for (int c = fr + 1; c <= lr; ++c) v = cadd(v, cscale(b[c], Q.D2(r, g.slot(r, c))));
+      st[r] = csub(cadd(cscale(v, al2), out[r]), cscale(d[r], zeta));
+    }
+  } else {
+    for (int i = tid; i < n; i += T) st[i] = out[i];
+  }
+  if (isW)
+    for (int i = tid; i < n; i += T) L(0)[i] = mk(0.0, 0.0);  // mean_w leaves u2 = 0
   __syncthreads();
   // results: f 0 -> slot 0 (U), slot 1 (V = 0, written here because this
   // block is the one that reads slot 1), RK 0; f 1 -> slots 2-4, RK 1;
@@ -1338,9 +1476,9 @@ __global__ void __launch_bounds__(MEAN_THREADS) mean_kernel(LineParams P, WsArgs
   }
   for (int q = 0; q < MEAN_COLS; ++q)
     if (dst[q])
-      for (int i = threadIdx.x; i < n; i += MEAN_THREADS) dst[q][(size_t)i * A.plane] = col[q * n + i];
+      for (int i = tid; i < n; i += T) dst[q][(size_t)i * A.plane] = col[q * n + i];
   if (f == 0)
-    for (int i = threadIdx.x; i < n; i += MEAN_THREADS) gU(1)[(size_t)i * A.plane] = mk(0.0, 0.0);
+    for (int i = tid; i < n; i += T) gU(1)[(size_t)i * A.plane] = mk(0.0, 0.0);
 }
 
 // Block order of the per-system sweeps that read what is stored once per
