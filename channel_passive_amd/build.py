@@ -9,6 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpoongback_hip.so")
 
+RESOURCES = os.path.join(HERE, "kernel_resources.txt")  # the compiler's per-kernel register/scratch report
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -21,11 +22,13 @@ SOURCES = ["capi.hip", "bspline.cpp"]
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [HIPCC, *FLAGS, *SOURCES, "-o", OUT]
+    cmd = [HIPCC, *FLAGS, "-Rpass-analysis=kernel-resource-usage", *SOURCES, "-o", OUT]
     res = subprocess.run(cmd, cwd=CSRC, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed for libpoongback_hip.so")
+    with open(RESOURCES, "w") as f:
+        f.write("\n".join(l for l in res.stderr.splitlines() if "kernel-resource-usage" in l) + "\n")
     if verbose:
         sys.stderr.write(" ".join(cmd) + "\n")
     return OUT

@@ -1990,14 +1990,28 @@ int pb_gradient(pb_ctx *c, int32_t nsteps, int32_t ckpt_every, double dt, double
   }
   double2 *ckpt = c->adj_states, *fin = ckpt + (size_t)nck * 7 * c->field, *seg = fin + 7 * c->field;
   std::vector<pb_substep_out> out(S);
-  // forward: J and the checkpoints
+  // forward: J and the checkpoints; the last segment keeps what its
+  // reverse pass needs right away (it is not recomputed)
   double jsum = 0.0;
+  const int klast = (nck - 1) * C;
   for (int k = 0; k < nsteps; ++k) {
     if (k % C == 0 && (r = save_state(c, ckpt + (size_t)(k / C) * 7 * c->field, true))) return r;
     double dtv = dt;
     if ((r = upload_dt(c, dt))) return r;
-    for (int s = 1; s <= S; ++s)
-      if ((r = substep_async(c, s))) return r;
+    for (int s = 1; s <= S; ++s) {
+      if (k < klast) {
+        r = substep_async(c, s);
+      } else {
+        const size_t i = (size_t)(k - klast) * S + s - 1;
+        if (pad_path) {
+          double2 *p = seg + i * per_pad;
+          r = substep_saving(c, s, p, p + 4 * c->pfield);
+        } else if (!(r = save_state(c, seg + i * per_state, false))) {
+          r = substep_async(c, s);
+        }
+      }
+      if (r) return r;
+    }
     if ((r = read_outputs(c, 1, S, out.data(), &dtv))) return r;
     jsum += out[S - 1].tau_w + wq * out[S - 1].q_w;
   }
@@ -2010,8 +2024,8 @@ int pb_gradient(pb_ctx *c, int32_t nsteps, int32_t ckpt_every, double dt, double
   const double tb = 1.0 / nsteps, qb = wq / nsteps;
   for (int q = nck - 1; q >= 0; --q) {
     const int k0 = q * C, k1 = std::min(nsteps, k0 + C);
-    if ((r = restore_state(c, ckpt + (size_t)q * 7 * c->field, true))) return r;
-    for (int k = k0; k < k1; ++k) {
+    if (q != nck - 1 && (r = restore_state(c, ckpt + (size_t)q * 7 * c->field, true))) return r;
+    for (int k = k0; k < k1 && q != nck - 1; ++k) {
       if ((r = upload_dt(c, dt))) return r;
       for (int s = 1; s <= S; ++s) {
         const size_t i = (size_t)(k - k0) * S + s - 1;
