@@ -544,6 +544,9 @@ struct Ctx {
   double rk[4][3];
   // opposition control: B-spline rows at the two detection planes
   std::vector<double> opp_lo_row, opp_hi_row;  // detection rows r: v(y_d) = r . v
+  // wall actuation (one complex value per (kx,kz), index j*nxh + k), added to
+  // the wall v of both walls; empty: none
+  std::vector<cd> act;
   FFTPlan px, pz;
 };
 
@@ -1089,9 +1092,15 @@ void *orc_create(const pb_config *cfg) {
     else d = std::min(std::fabs(c->y[i] - c->y[i - 1]), std::fabs(c->y[i] - c->y[i + 1]));
     c->dyr[i] = 1.0 / (cfg->dy_factor * d);
   }
-  if (cfg->opp_alpha != 0.0) {
+  // detection rows whenever the plane lies inside the channel, so that a gain
+  // set later (orc_set_control) has them; a nonzero gain requires them
+  if (cfg->opp_alpha != 0.0 || (cfg->opp_ydist > 0.0 && cfg->opp_ydist < 1.0)) {
     if (detection_row(*c, -1.0 + cfg->opp_ydist, c->opp_lo_row) ||
-        detection_row(*c, 1.0 - cfg->opp_ydist, c->opp_hi_row)) { delete c; return nullptr; }
+        detection_row(*c, 1.0 - cfg->opp_ydist, c->opp_hi_row)) {
+      if (cfg->opp_alpha != 0.0) { delete c; return nullptr; }
+      c->opp_lo_row.clear();
+      c->opp_hi_row.clear();
+    }
   }
   c->px = make_plan(c->nx32);
   c->pz = make_plan(c->nz32);
@@ -1535,6 +1544,10 @@ int orc_wavespace(void *h, int s, double dt, double *Ud, double *RKd, const doub
           wlo = mulr(vd[(size_t)j * c->nxh + k], -g.opp_alpha);
           whi = mulr(vd[nl + (size_t)j * c->nxh + k], -g.opp_alpha);
         }
+        if (!c->act.empty()) {
+          wlo = add(wlo, c->act[(size_t)j * c->nxh + k]);
+          whi = add(whi, c->act[(size_t)j * c->nxh + k]);
+        }
         solve_v_f(L, slot(1), slot(2), slot(4), rk(1), slot(3), b1.data(), b2.data(), kxp, kzp, g.nu, wlo, whi);
         solve_u_w(n, slot(0), slot(2), kxp, kzp);
         if (usrc) {  // h_theta += (nu/pr) u / bulk_u
@@ -1635,6 +1648,17 @@ int orc_sample(void *h, int nplanes, int nlines, int s, double dt, double *t_non
 
 // One full substep on reference-layout arrays.  S(ny,nxh,nz,4) state in/out,
 // RK(ny,nxh,nz,3) in/out.  out6 = dt, velmax, phi_u, phi_t, tau_w, q_w.
+// The opposition gain and the wall actuation (nxh*nz complex, [kz][kx], or
+// NULL for none) of later substeps: v(wall) = -alpha v(y_d) + act.
+int orc_set_control(void *h, double alpha, const double *act) {
+  Ctx *c = static_cast<Ctx *>(h);
+  if (alpha != 0.0 && c->opp_lo_row.empty()) return PB_E_ARG;
+  c->cfg.opp_alpha = alpha;
+  c->act.clear();
+  if (act) c->act.assign(reinterpret_cast<const cd *>(act), reinterpret_cast<const cd *>(act) + (size_t)c->nxh * c->nz);
+  return 0;
+}
+
 int orc_substep(void *h, double *Sd, double *RKd, int s, double *dt_inout, double *out6) {
   Ctx *c = static_cast<Ctx *>(h);
   const size_t N = (size_t)c->ny * c->nxh * c->nz;

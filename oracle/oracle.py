@@ -43,6 +43,7 @@ def lib():
         L.orc_wavespace.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp,
                                     ctypes.c_int, ctypes.c_int, _dp]
         L.orc_substep.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_set_control.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
         L.orc_pressure.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.orc_statistics.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_spectra.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
@@ -291,6 +292,15 @@ class Oracle:
         ez = np.zeros((18, c.nz, c.ny))
         lib().orc_spectra(self.h, np.ascontiguousarray(S).ctypes.data_as(_dp), _p(ex), _p(ez))
         return ex, ez
+
+    def set_control(self, alpha: float, act: np.ndarray | None = None):
+        """Opposition gain and wall actuation (nz, nx/2) complex, or None."""
+        a = None if act is None else np.ascontiguousarray(act, dtype=np.complex128)
+        if a is not None:
+            assert a.shape == (self.cfg.nz, self.cfg.nx // 2)
+        r = lib().orc_set_control(self.h, ctypes.c_double(alpha), None if a is None else a.ctypes.data_as(_dp))
+        if r:
+            raise RuntimeError(f"orc_set_control failed: {r}")
 
     def substep(self, S: np.ndarray, RK: np.ndarray, s: int, dt: float):
         out = np.zeros(6)

@@ -139,3 +139,45 @@ def test_statistics_bitwise_cpu(oracle_lib, emu_lib, name):
         assert np.array_equal(out[i * n:(i + 1) * n], ref[k]), k
     assert abs(out[16 * n] - ref["u_center"]) <= 1e-15 * abs(ref["u_center"])
     assert abs(out[16 * n + 1] - ref["t_center"]) <= 1e-15 * max(abs(ref["t_center"]), 1.0)
+
+
+@pytest.mark.parametrize("name,alpha,extra", [("tiny", 0.0, {}), ("tiny", 0.8, {}), ("tiny", 0.6, {"scalar_bc": 1}),
+                                              ("re180_64", 0.5, {})])
+def test_actuated_wavespace_bitwise_cpu(oracle_lib, emu_lib, name, alpha, extra):
+    """Wall actuation: v(wall) = -alpha v(y_d) + a(kx, kz) on both walls.  The
+    device solver (solver_fused.h) with an actuation field equals the oracle
+    with the same field (orc_set_control) bit for bit; the field changes the
+    result, and a zero field gives the unactuated solve."""
+    cfg = dataclasses.replace(preset(name), opp_alpha=alpha, opp_yplus=15.0, **extra)
+    orc = oracle_lib.Oracle(cfg)
+    c = cfg.to_c()
+    rng = np.random.default_rng(3)
+    act = 1e-2 * (rng.standard_normal((cfg.nz, cfg.nxh)) + 1j * rng.standard_normal((cfg.nz, cfg.nxh)))
+    S = make_state(cfg, seed=19)
+    RK = orc.init_stores(S)
+    orc.step(S, RK, cfg.dt)
+    orc.set_control(alpha, act)
+    for s in (1, 2, 3):
+        P, _ = orc.nonlinear(S)
+        vd = orc.opp_planes(S) if alpha else None
+        Ud = np.ascontiguousarray(np.transpose(P[:9], (0, 3, 1, 2)))
+        RKd = np.ascontiguousarray(np.transpose(RK, (0, 3, 1, 2)))
+        us = np.ascontiguousarray(np.transpose(S[0], (2, 0, 1)))
+        Ud0, RKd0 = Ud.copy(), RKd.copy()
+        m_e, m_0 = np.zeros(4), np.zeros(4)
+        args = (None if vd is None else vd.ctypes.data_as(dp), us.ctypes.data_as(dp))
+        assert emu_lib.emu_wavespace_fused_act(ctypes.byref(c), s, ctypes.c_double(cfg.dt), Ud.ctypes.data_as(dp),
+                                               RKd.ctypes.data_as(dp), *args, act.ctypes.data_as(dp),
+                                               m_e.ctypes.data_as(dp)) == 0
+        assert emu_lib.emu_wavespace_fused_act(ctypes.byref(c), s, ctypes.c_double(cfg.dt), Ud0.ctypes.data_as(dp),
+                                               RKd0.ctypes.data_as(dp), *args, None, m_0.ctypes.data_as(dp)) == 0
+        assert not np.array_equal(Ud0[1], Ud[1])
+        m_o = orc.wavespace(s, cfg.dt, P, RK, vd, usrc=S[0])
+        Ue, RKe = np.transpose(Ud, (0, 2, 3, 1)), np.transpose(RKd, (0, 2, 3, 1))
+        for f in (0, 1, 2, 6):
+            assert np.array_equal(Ue[f], P[f]), (s, f, np.abs(Ue[f] - P[f]).max())
+        for i in range(3):
+            assert np.array_equal(RKe[i], RK[i]), (s, i)
+        assert np.array_equal(m_e, m_o)
+        for i, f in enumerate((0, 1, 2, 6)):
+            S[i] = P[f]

@@ -11,8 +11,8 @@ and the wall actuation) and the checkpointed gradient driver.
   the linearisation at x.
 * The gradient of J = mean over the window of tau_w + 0.5 q_w with respect
   to the opposition gain and the actuation field agrees with central finite
-  differences of the GPU forward to 1e-6, through a torch.autograd.Function
-  harness as well.
+  differences of the GPU forward to 1e-6, with those of the oracle's
+  forward to 1e-5, and through a torch.autograd.Function harness as well.
 """
 import dataclasses
 
@@ -204,6 +204,52 @@ def test_gradient_finite_differences(capi, sbc):
         worst = max(worst, abs(rdot(e, g) - fde) / abs(fde))
     print(f"dJ/da . e over 32 directions: worst relative difference {worst:.3e}; |dJ/da| = {np.abs(g).max():.3e}")
     ch.close()
+
+
+def test_gradient_against_oracle_differences(capi, oracle_lib):
+    """pb_gradient against central differences of the oracle's forward (the
+    CPU restatement of the reference's step, with the same opposition gain
+    and wall actuation) over a 4-step window, mixed-type scalar condition:
+    dJ/dalpha and dJ/da . e for 4 random directions within 1e-5.  Steps:
+    1e-5 on the gain and 1e-6 / max|e| on the actuation, where the oracle's
+    own differences have converged to about 1e-7 and the device/oracle
+    rounding difference (about 1e-13 |J|) stays below 1e-6 of them."""
+    cfg = cfg_of("re180_64", 0.8, 1, yplus=10.0)
+    ch = capi.Channel(cfg)
+    ch.set_state(make_state(cfg, seed=21, amp=0.3))
+    ch.init_stores()
+    S0, RK0 = ch.get_state()
+    rng = np.random.default_rng(5)
+    act = 1e-3 * crand(rng, cfg.nz, cfg.nxh)
+    act[0, 0] = 0.0
+    K, dt = 4, cfg.dt
+    ch.set_control(0.8, act)
+    ch.set_state(S0, RK0)
+    J, dJa, g = ch.gradient(K, dt, ckpt_every=3, wq=0.5)
+    ch.close()
+    orc = oracle_lib.Oracle(cfg)
+
+    def Jo(alpha, a):
+        orc.set_control(alpha, a)
+        S, RK = S0.copy(), RK0.copy()
+        j = 0.0
+        for _ in range(K):
+            o = orc.step(S, RK, dt)[-1]
+            j += o["tau_w"] + 0.5 * o["q_w"]
+        return j / K
+
+    assert J == pytest.approx(Jo(0.8, act), rel=1e-10)
+    h = 1e-5
+    fd = (Jo(0.8 + h, act) - Jo(0.8 - h, act)) / (2 * h)
+    print(f"dJ/dalpha = {dJa:.12e}, oracle central difference {fd:.12e}")
+    assert dJa == pytest.approx(fd, rel=1e-5)
+    for q in range(4):
+        e = crand(rng, cfg.nz, cfg.nxh)
+        e[0, 0] = 0.0
+        he = 1e-6 / np.abs(e).max()
+        fde = (Jo(0.8, act + he * e) - Jo(0.8, act - he * e)) / (2 * he)
+        print(f"direction {q}: dJ/da . e = {rdot(e, g):.12e}, oracle central difference {fde:.12e}")
+        assert rdot(e, g) == pytest.approx(fde, rel=1e-5), q
 
 
 def test_gradient_autograd_harness(capi):
