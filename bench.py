@@ -11,6 +11,11 @@ events on the library's stream; the dominant kernel's algorithmic bytes per
 launch divided by its mean event time gives roofline.achieved.  The CPU
 baseline is the C++/OpenMP oracle (the reference algorithm restated) timed
 on a bounded sample of the same workload and extrapolated to one step.
+Beside the headline the line carries three objects of other workloads: the
+gradient of the discrete adjoint against its forward window (`adjoint`),
+2000 steps of the 192 x 129 x 160 production case with statistics every 10
+steps, statistics steps timed apart (`statistics`), and the 24-channel
+ensemble against its channels run one after another (`ensemble`).
 """
 import argparse
 import json
@@ -212,6 +217,63 @@ def adjoint_leg(steps, ckpt_every=20, warmup=5):
             "cost_ratio": t_grad / t_fwd, "J": J, "dJ_dalpha": dJa, "dJ_da_max_abs": float(np.abs(g).max())}
 
 
+def statistics_leg(steps=2000, every=10, warmup=10):
+    """The production-resolution workload with statistics (Re_tau = 180,
+    192x129x160, opposition control alpha = 1 at y+ = 10, mixed-type scalar
+    condition, dt from the CFL rule at CFL 0.8): `steps` steps, every
+    `every`-th one a statistics step -- substep 1 with the pressure and
+    get_statistics_2 (pb_substep_stats2), substeps 2 and 3, then
+    get_statistics_1's profiles (pb_statistics), results copied to the
+    host -- timed apart from the plain steps."""
+    import dataclasses
+    import numpy as np
+    from channel_passive_amd import capi
+    from channel_passive_amd import observables as ob
+    from channel_passive_amd.config import preset
+    from channel_passive_amd.initial import make_state
+
+    cfg = dataclasses.replace(preset("re180_192"), dt_factor=0.0, cfl=0.8)
+    ch = capi.Channel(cfg)
+    ch.set_state(make_state(cfg, seed=0xC0FFEE + 1))
+    ch.init_stores()
+    quad = ob.Quadrature(ch.labs(), ch.bw)
+    dt = cfg.dt
+    for _ in range(warmup):
+        dt = ch.step(dt)[-1]["dt"]
+    t_plain = t_stats = 0.0
+    n_stats, cfl_max, last = 0, 0.0, None
+    for k in range(1, steps + 1):
+        t0 = time.perf_counter()
+        if k % every == 0:
+            o1, _, _, _ = ch.substep_stats2(dt)
+            o2 = ch.substep(2, o1["dt"])
+            o3 = ch.substep(3, o1["dt"])
+            prof = ch.statistics()
+            t_stats += time.perf_counter() - t0
+            n_stats += 1
+            outs = [o1, o2, o3]
+            last = (prof, o3)
+        else:
+            outs = ch.step(dt)
+            t_plain += time.perf_counter() - t0
+        dt = outs[-1]["dt"]
+        cfl_max = max(cfl_max, math.pi * outs[0]["dt"] * outs[0]["velmax"])
+    ch.close()
+    prof, o3 = last
+    n_plain = steps - n_stats
+    tm = ob.theta_mean(prof, quad, "mixed", cfg.u_gal)
+    print(f"[bench] statistics: plain {1e3 * t_plain / n_plain:.3f} ms/step, statistics step "
+          f"{1e3 * t_stats / n_stats:.3f} ms", file=sys.stderr, flush=True)
+    return {"workload": f"Re_tau=180 {cfg.nx}x{cfg.ny}x{cfg.nz}, alpha={cfg.opp_alpha} at y+={cfg.opp_yplus:g}, "
+                        "mixed-type scalar condition, dt from the CFL rule (CFL 0.8)",
+            "steps": steps, "statistics_every": every, "warmup": warmup,
+            "plain_ms_per_step": 1e3 * t_plain / n_plain, "statistics_step_ms": 1e3 * t_stats / n_stats,
+            "statistics_overhead_ms": 1e3 * (t_stats / n_stats - t_plain / n_plain),
+            "total_s": t_plain + t_stats, "cfl_number_max": round(cfl_max, 4),
+            "last_Cf": float(ob.skin_friction(o3["tau_w"], cfg.bulk_u_value())),
+            "last_Nu": float(ob.nusselt(o3["q_w"], cfg.nu, cfg.pr, tm))}
+
+
 ENSEMBLE_HW_QUEUES = "16"
 
 
@@ -247,6 +309,11 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient (discrete adjoint) leg")
     ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--adjoint-only", action="store_true", help="run only the gradient leg and print its JSON")
+    ap.add_argument("--no-statistics", action="store_true",
+                    help="skip the 2000-step run with statistics every 10 steps (192x129x160)")
+    ap.add_argument("--statistics-only", action="store_true",
+                    help="run only the statistics leg and print its JSON")
+    ap.add_argument("--statistics-steps", type=int, default=2000)
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the ensemble comparison and print its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -261,6 +328,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps(adjoint_leg(args.adjoint_steps)))
+        return
+    if args.statistics_only:
+        print(json.dumps(statistics_leg(args.statistics_steps)))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the library runs on one GPU "
@@ -371,6 +441,11 @@ def main():
             out["adjoint"] = adjoint_leg(args.adjoint_steps)
         except Exception as exc:  # reported, not fatal: the headline stands
             out["adjoint"] = {"error": f"{type(exc).__name__}: {exc}"}
+    if not args.no_statistics:
+        try:
+            out["statistics"] = statistics_leg(args.statistics_steps)
+        except Exception as exc:  # reported, not fatal: the headline stands
+            out["statistics"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not args.no_ensemble:
         out["ensemble"] = ensemble_subprocess(args.ensemble_steps)
     print(json.dumps(out))

@@ -1017,7 +1017,7 @@ int pb_substep_stats2(pb_ctx *c, int32_t s, double *dt_inout, pb_substep_out *ou
         (r = alloc((void **)&c->dmsum, (size_t)pb::NMOM * c->ny * sizeof(double))) ||
         (r = alloc((void **)&c->dpv, (size_t)8 * c->ny * sizeof(double))) ||
         (r = alloc((void **)&c->dstat2, (size_t)PB_NSTAT2_LINES * c->ny * sizeof(double))) ||
-        (r = alloc((void **)&c->dstat2_tmp, (size_t)2 * c->ny * sizeof(double2))))
+        (r = alloc((void **)&c->dstat2_tmp, (size_t)2 * pb::STAT2_DDY * c->ny * sizeof(double2))))
       return r;
   }
   if ((r = upload_dt(c, *dt_inout))) return r;
@@ -1040,13 +1040,12 @@ int pb_substep_stats2(pb_ctx *c, int32_t s, double *dt_inout, pb_substep_out *ou
   hipLaunchKernelGGL(pb::stats2_mom_kernel, dim3(c->ny, pb::NMOM), dim3(256), 0, c->stream, (const double *)c->dmom,
                      c->vpart_n / c->ny, c->dmsum);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(pb::stats2_pv_kernel, dim3(blocks_for(c->ny, 64)), dim3(64), 0, c->stream,
+  hipLaunchKernelGGL(pb::stats2_pv_kernel, dim3(c->ny), dim3(pb::PV_THREADS), 0, c->stream,
                      (const pb::cplx *)c->stat_uvw, (const pb::cplx *)(c->U + 5 * c->field), (const pb::cplx *)buf,
                      c->plane, c->ny, c->nxh, c->nh, c->lmfxr, c->lmfzr, c->dpv);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(pb::stats2_final_kernel, dim3(1), dim3(64), 0, c->stream, P, (const double *)c->dmsum,
-                     (const double *)c->dpv, c->rescale, (pb::cplx *)c->dstat2_tmp, (pb::cplx *)c->dstat2_tmp + c->ny,
-                     c->dstat2);
+                     (const double *)c->dpv, c->rescale, (pb::cplx *)c->dstat2_tmp, c->dstat2);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(stat2, c->dstat2, (size_t)PB_NSTAT2_LINES * c->ny * sizeof(double), hipMemcpyDeviceToHost,
                         c->stream));

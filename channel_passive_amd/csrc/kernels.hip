@@ -1553,14 +1553,16 @@ __global__ void __launch_bounds__(64) stats_uv_kernel(const cplx *U, size_t plan
   const cplx *b = U + (size_t)kStatPairs[q][1] * field + (size_t)y * plane;
   stat[(size_t)(6 + q) * ny + y] = mean_uv_row(a, b, nxh, plane);
 }
-// the mean-line profiles and centre values (one thread)
+// the mean-line profiles and centre values: thread q < 3 takes line q (U,
+// W, Theta) with scratch columns q of the two scratch fields
 __global__ void stats_mean_kernel(LineParams P, const cplx *U, cplx *tmp, size_t plane, const double *center,
                                   double u_gal, double *stat) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int q = threadIdx.x;
+  if (blockIdx.x != 0 || q >= 3) return;
   const size_t field = plane * P.g.n;
-  const SV m[3] = {SV{const_cast<cplx *>(U), plane}, SV{const_cast<cplx *>(U) + 2 * field, plane},
-                   SV{const_cast<cplx *>(U) + 6 * field, plane}};
-  mean_line_stats(P, m, SV{tmp, plane}, SV{tmp + field, plane}, center, u_gal, stat, stat + (size_t)16 * P.g.n);
+  const size_t slot = q == 0 ? 0 : (q == 1 ? 2 : 6);
+  mean_line_stats_q(P, SV{const_cast<cplx *>(U) + slot * field, plane}, q, SV{tmp + q, plane},
+                    SV{tmp + field + q, plane}, center, u_gal, stat, stat + (size_t)16 * P.g.n);
 }
 
 // ------------------------------------------ 1-D energy spectra (write_spectra)
@@ -1598,83 +1600,82 @@ __global__ void __launch_bounds__(64) stats2_deriv_kernel(LineParams P, const cp
   deriv_line<BW>(P, SV{const_cast<cplx *>(uvw) + sys, plane}, SV{const_cast<cplx *>(uvw) + field + sys, plane},
                  SV{buf + sys, plane}, SV{buf + field + sys, plane});
 }
-// Parseval sums per y, serial over the (kx, kz) lines in the reference's order
-// (get_uv, get_statistics.f90:982-1017; get_turb_transport_PI1_ij, 817-902):
-// out[0..7][y] = up, vp, wp, pp, 2pdudx, 2pdvdy, 2pdwdz, pdudy+pdvdx
-__global__ void __launch_bounds__(64) stats2_pv_kernel(const cplx *uvw, const cplx *p, const cplx *d, size_t plane,
-                                                       int ny, int nxh, int nh, double lmfxr, double lmfzr,
-                                                       double *out) {
-  const int y = blockIdx.x * blockDim.x + threadIdx.x;
-  if (y >= ny) return;
+// Parseval sums per y (get_uv, get_statistics.f90:982-1017;
+// get_turb_transport_PI1_ij, 817-902): out[0..7][y] = up, vp, wp, pp,
+// 2pdudx, 2pdvdy, 2pdwdz, pdudy+pdvdx.  One block per y: each thread sums
+// the lines l = tid, tid + T, ... (weights 1 at kx = 0, 2 otherwise, the
+// reference's factors), then a tree over the block.  The order of the sum
+// differs from the reference's serial loop (a few ulp of the line).
+constexpr int PV_THREADS = 256;
+__global__ void __launch_bounds__(PV_THREADS) stats2_pv_kernel(const cplx *uvw, const cplx *p, const cplx *d,
+                                                               size_t plane, int ny, int nxh, int nh, double lmfxr,
+                                                               double lmfzr, double *out) {
+  __shared__ double red[8][PV_THREADS];
+  const int y = blockIdx.x, t = threadIdx.x;
   const size_t field = plane * ny, row = (size_t)y * plane;
   const cplx *u = uvw + row, *v = uvw + field + row, *w = uvw + 2 * field + row, *pr = p + row;
   const cplx *du = d + row, *dv = d + field + row;
-  out[0 * ny + y] = mean_uv_row(u, pr, nxh, plane);
-  out[1 * ny + y] = mean_uv_row(v, pr, nxh, plane);
-  out[2 * ny + y] = mean_uv_row(w, pr, nxh, plane);
-  out[3 * ny + y] = mean_uv_row(pr, pr, nxh, plane);
-  double puu = 0.0, pvv = 0.0, pww = 0.0, puv = 0.0;
-  for (size_t l = 0; l < plane; ++l) {
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t l = t; l < plane; l += PV_THREADS) {
     const int k = (int)(l % nxh), j = (int)(l / nxh);
-    const int kzi = (j <= nh) ? j : j + 1 - 2 * nh - 1;
-    const double kxw = lmfxr * k, kzw = lmfzr * kzi;
-    const cplx a = u[l], b = v[l], c = w[l], q = pr[l];
-    if (k == 0) {
-      puu = puu + 2.0 * kxw * (a.re * q.im - a.im * q.re);
-      puv = puv + kxw * (b.re * q.im - b.im * q.re);
-      pww = pww + 2.0 * kzw * (c.re * q.im - c.im * q.re);
-    } else {
-      puu = puu + 4.0 * kxw * (a.re * q.im - a.im * q.re);
-      puv = puv + 2.0 * kxw * (b.re * q.im - b.im * q.re);
-      pww = pww + 4.0 * kzw * (c.re * q.im - c.im * q.re);
-    }
+    const int kzi = (j <= nh) ? j : j + 1 - 2 * nh - 1;  // module_ppdfftw.f90:360-374
+    const double kxw = lmfxr * k, kzw = lmfzr * kzi, wt = (k == 0) ? 1.0 : 2.0;
+    const cplx a = u[l], b = v[l], c = w[l], q = pr[l], da = du[l], db = dv[l];
+    acc[0] += wt * (a.re * q.re + a.im * q.im);
+    acc[1] += wt * (b.re * q.re + b.im * q.im);
+    acc[2] += wt * (c.re * q.re + c.im * q.im);
+    acc[3] += wt * (q.re * q.re + q.im * q.im);
+    acc[4] += 2.0 * wt * kxw * (a.re * q.im - a.im * q.re);
+    acc[5] += 2.0 * wt * (db.re * q.re + db.im * q.im);
+    acc[6] += 2.0 * wt * kzw * (c.re * q.im - c.im * q.re);
+    acc[7] += wt * (kxw * (b.re * q.im - b.im * q.re) + (da.re * q.re + da.im * q.im));
   }
-  for (size_t l = 0; l < plane; ++l) {
-    const cplx a = du[l], b = dv[l], q = pr[l];
-    if (l % nxh == 0) {
-      pvv = pvv + 2.0 * (b.re * q.re + b.im * q.im);
-      puv = puv + (a.re * q.re + a.im * q.im);
-    } else {
-      pvv = pvv + 4.0 * (b.re * q.re + b.im * q.im);
-      puv = puv + 2.0 * (a.re * q.re + a.im * q.im);
-    }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) red[q][t] = acc[q];
+  __syncthreads();
+  for (int h = PV_THREADS / 2; h > 0; h >>= 1) {
+    if (t < h)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) red[q][t] += red[q][t + h];
+    __syncthreads();
   }
-  out[4 * ny + y] = puu;
-  out[5 * ny + y] = pvv;
-  out[6 * ny + y] = pww;
-  out[7 * ny + y] = puv;
+  if (t < 8) out[(size_t)t * ny + y] = red[t][0];
 }
 // the 29 datasets of get_statistics_2 from the moment sums and the Parseval
-// sums (one thread: a few ny-long band solves)
-__global__ void stats2_final_kernel(LineParams P, const double *msum, const double *pv, double rescale, cplx *tmp,
-                                    cplx *tmp2, double *out) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  const int n = P.g.n;
+// sums: the copies spread over the block, the six y derivatives (a band
+// solve and a band product each, ny long) one per thread, each in its own
+// two scratch columns tmp[2 q], tmp[2 q + 1] (ny complex each)
+constexpr int STAT2_DDY = 6;
+__global__ void __launch_bounds__(64) stats2_final_kernel(LineParams P, const double *msum, const double *pv,
+                                                          double rescale, cplx *tmp, double *out) {
+  if (blockIdx.x != 0) return;
+  const int n = P.g.n, t = threadIdx.x;
   const int mom_of[15] = {0, 1, 2, 3, 4, 5, 10, 11, 12, 13, 14, 15, 16, 17, 18};
-  for (int q = 0; q < 15; ++q) {
-    double *o = out + (size_t)q * n;
-    o[0] = 0.0;
-    for (int i = 1; i < n - 1; ++i) o[i] = rescale * msum[(size_t)mom_of[q] * n + i];
-    o[n - 1] = 0.0;
+  for (int i = t; i < 15 * n; i += blockDim.x) {
+    const int q = i / n, y = i % n;
+    out[i] = (y == 0 || y == n - 1) ? 0.0 : rescale * msum[(size_t)mom_of[q] * n + y];
   }
-  const SV a{tmp, 1}, b{tmp2, 1};
-  auto ddy = [&](const double *x, double scale_in, double *o, double scale_out) {
-    for (int i = 0; i < n; ++i) a[i] = mk(x[i] * scale_in, 0.0);
-    band_bs(P.g, P.D0LU, a);
-    band_mv(P.g, P.D1, a, b);
-    for (int i = 0; i < n; ++i) o[i] = scale_out * b[i].re;
-  };
-  const int tri[4] = {6, 8, 7, 9};
-  for (int q = 0; q < 4; ++q) ddy(msum + (size_t)tri[q] * n, rescale, out + (size_t)(15 + q) * n, 1.0);
-  for (int i = 0; i < n; ++i) {
+  for (int i = t; i < n; i += blockDim.x) {
     out[(size_t)19 * n + i] = pv[i];
     out[(size_t)21 * n + i] = pv[(size_t)1 * n + i];
     out[(size_t)23 * n + i] = pv[(size_t)2 * n + i];
     out[(size_t)24 * n + i] = pv[(size_t)3 * n + i];
     for (int q = 0; q < 4; ++q) out[(size_t)(25 + q) * n + i] = pv[(size_t)(4 + q) * n + i];
   }
-  ddy(pv, 1.0, out + (size_t)20 * n, -1.0);
-  ddy(pv + n, 1.0, out + (size_t)22 * n, -2.0);
+  if (t < STAT2_DDY) {
+    // d/dy of the triple moments (-d<uuv>/dy ... into lines 15..18), of <up>
+    // (line 20, times -1) and of <vp> (line 22, times -2)
+    const int tri[4] = {6, 8, 7, 9};
+    const double *x = t < 4 ? msum + (size_t)tri[t] * n : pv + (size_t)(t - 4) * n;
+    const double scale_in = t < 4 ? rescale : 1.0;
+    const double scale_out = t < 4 ? 1.0 : (t == 4 ? -1.0 : -2.0);
+    double *o = out + (size_t)(t < 4 ? 15 + t : (t == 4 ? 20 : 22)) * n;
+    const SV a{tmp + (size_t)(2 * t) * n, 1}, b{tmp + (size_t)(2 * t + 1) * n, 1};
+    for (int i = 0; i < n; ++i) a[i] = mk(x[i] * scale_in, 0.0);
+    band_bs(P.g, P.D0LU, a);
+    band_mv(P.g, P.D1, a, b);
+    for (int i = 0; i < n; ++i) o[i] = scale_out * b[i].re;
+  }
 }
 
 // vorticity into slots 3, 4, 5 (get_vorticity_field, get_statistics.f90:912-948):

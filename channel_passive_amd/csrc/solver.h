@@ -496,14 +496,27 @@ PB_HD void solve_ppe(const LineParams &P, const OwnBand &Db, SV u1, SV u2, SV u3
 
 // <ab>(y) of get_uv (get_statistics.f90:982-1017) for one y: the modes of
 // the plane rows in kz-major, kx-minor order, kx = 0 once, kx > 0 twice.
+// Serial over the plane in memory order (kz rows, kx within a row); weight 1
+// at kx = 0, 2 otherwise.  Within a row the loads of 8 modes are issued
+// before their sums, which keep the serial order.
 PB_HD double mean_uv_row(const cplx *a, const cplx *b, int nxh, size_t plane) {
+  constexpr int NB = 8;
   double acc = 0.0;
-  for (size_t l = 0; l < plane; ++l) {
-    const cplx u = a[l], v = b[l];
-    if (l % nxh == 0)
-      acc = acc + u.re * v.re + u.im * v.im;
-    else
-      acc = acc + 2.0 * (u.re * v.re + u.im * v.im);
+  for (size_t r = 0; r < plane; r += nxh) {
+    const cplx *ar = a + r, *br = b + r;
+    acc = acc + ar[0].re * br[0].re + ar[0].im * br[0].im;
+    int k = 1;
+    for (; k + NB <= nxh; k += NB) {
+      cplx u[NB], v[NB];
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        u[i] = ar[k + i];
+        v[i] = br[k + i];
+      }
+#pragma unroll
+      for (int i = 0; i < NB; ++i) acc = acc + 2.0 * (u[i].re * v[i].re + u[i].im * v[i].im);
+    }
+    for (; k < nxh; ++k) acc = acc + 2.0 * (ar[k].re * br[k].re + ar[k].im * br[k].im);
   }
   return acc;
 }
@@ -514,22 +527,25 @@ PB_HD double mean_vv_row(const cplx *row, int nxh, size_t plane) { return mean_u
 // 2 Theta, 3 dU/dy, 4 dW/dy, 5 dTheta/dy; centre[0] = u_center (u_gal +
 // sum Re(c_k) B_k(0)), centre[1] = t_center.  m[q] are the (0,0) columns of
 // u, w, theta; t1, t2 scratch.
+// One mean line q (0 U, 1 W, 2 Theta): its profile, its y derivative and
+// (U, Theta) its centre value; the three lines are independent.
+PB_HD void mean_line_stats_q(const LineParams &P, const SV &m, int q, const SV &t1, const SV &t2,
+                             const double *center, double u_gal, double *out, double *centre) {
+  const int n = P.g.n;
+  for (int y = 0; y < n; ++y) out[(size_t)q * n + y] = m[y].re + (q == 0 ? u_gal : 0.0);
+  for (int y = 0; y < n; ++y) t1[y] = m[y];
+  band_bs(P.g, P.D0LU, t1);
+  band_mv(P.g, P.D1, t1, t2);
+  for (int y = 0; y < n; ++y) out[(size_t)(3 + q) * n + y] = t2[y].re;
+  if (q != 1) {
+    double a = (q == 0) ? u_gal : 0.0;
+    for (int k = 0; k < n; ++k) a = a + t1[k].re * center[k];
+    centre[q == 0 ? 0 : 1] = a;
+  }
+}
 PB_HD void mean_line_stats(const LineParams &P, const SV m[3], const SV &t1, const SV &t2, const double *center,
                            double u_gal, double *out, double *centre) {
-  const int n = P.g.n;
-  for (int q = 0; q < 3; ++q)
-    for (int y = 0; y < n; ++y) out[(size_t)q * n + y] = m[q][y].re + (q == 0 ? u_gal : 0.0);
-  for (int q = 0; q < 3; ++q) {
-    for (int y = 0; y < n; ++y) t1[y] = m[q][y];
-    band_bs(P.g, P.D0LU, t1);
-    band_mv(P.g, P.D1, t1, t2);
-    for (int y = 0; y < n; ++y) out[(size_t)(3 + q) * n + y] = t2[y].re;
-    if (q != 1) {
-      double a = (q == 0) ? u_gal : 0.0;
-      for (int k = 0; k < n; ++k) a = a + t1[k].re * center[k];
-      centre[q == 0 ? 0 : 1] = a;
-    }
-  }
+  for (int q = 0; q < 3; ++q) mean_line_stats_q(P, m[q], q, t1, t2, center, u_gal, out, centre);
 }
 
 // Detection-plane values of a collocation column, both planes: v(y_d) =
