@@ -815,6 +815,9 @@ static pb::WsArgs ws_args(pb_ctx *c, int s) {
   A.buf = (pb::cplx *)c->buf;
   A.dbuf = c->dbuf;
   A.rbuf = c->rbuf;
+  // the padded pencils are dead during the wall-normal stage: their first
+  // field holds the u1a column of the sweeps
+  A.u1a = (pb::cplx *)c->pad;
   A.vd = c->cfg.opp_alpha != 0.0 ? (const pb::cplx *)c->vd : nullptr;
   A.usrc = c->cfg.scalar_bc == PB_SCALAR_MIXED ? (const pb::cplx *)c->usrc : nullptr;
   A.src_scale = c->cfg.nu / c->cfg.pr / c->cfg.bulk_u;

@@ -27,7 +27,8 @@
 // row m), so every window index is a compile-time constant.
 //
 // Slot use per system (0-based product slots; s = substep):
-//   after A : 0 yB, 1 P2, 2 yE, 3 P4, 4 yF, 6 u1a, 7 y8, 8 u7a
+//   after A : 0 yB, 1 P2, 2 yE, 3 P4, 4 yF, 6 P7, 7 y8, 8 P9; u1a in the
+//             u1a column (P7 and P9 are read only by B_t, which forms u7a)
 //   after B : 0 rhs_phi, 2 rhs_g, 7 rhs_theta, RK = zeta*h (s < S)
 //   after C : 0 y_phip, 2 y_g, 7 y_theta, R0 y_phih; U factors in dbuf, dbuf2
 //   after D : 0 phi_p, 2 g, 6 theta, R1 phi_h, R2 D2 phi_h; RK updated
@@ -108,6 +109,7 @@ struct SysCtx {
   }
   PB_HD SV rk(const WsArgs &A, int slot) const { return SV{A.RK + (size_t)slot * A.field + sys, A.plane}; }
   PB_HD SV buf(const WsArgs &A, int slot) const { return SV{A.buf + (size_t)slot * A.field + sys, A.plane}; }
+  PB_HD SV u1a(const WsArgs &A) const { return SV{A.u1a + sys, A.plane}; }
 };
 
 PB_HD bool make_sys(const WsArgs &A, int n, size_t sys, SysCtx &c) {
@@ -439,32 +441,32 @@ PB_HD void sweep_A1(const LineParams &P, const WsArgs &A, const SysCtx &c) {
 }
 
 // A: hvhg point-wise terms (hvhg.f90:169-204, 256-258) and forward
-// substitution of u3a, u3b, u4b, P8 with LU(D0)
+// substitution of u3a, u3b, u4b, P8 with LU(D0).  u7a (from P7 and P9) is
+// formed where it is used, in B_t: A does not read P7 and P9.
 template <int BW>
 PB_HD void sweep_A(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const double kx = c.kx, kz = c.kz;
   const double kz2 = kz * kz, kx2 = kx * kx, kxkz = kx * kz;
   const cplx ikx = iunit(kx), ikz = iunit(kz);
-  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s4 = c.col(A, 4), s6 = c.col(A, 6),
-     s7 = c.col(A, 7), s8 = c.col(A, 8);
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s4 = c.col(A, 4), s7 = c.col(A, 7),
+     su = c.u1a(A);
   CWin<BW> wB, wE, wF, w8;
   wB.clear(); wE.clear(); wF.clear(); w8.clear();
   down_rows<BW>(c.n, [&](int r, auto kind) __attribute__((always_inline)) {
     constexpr bool MID = decltype(kind)::mid;
-    const cplx p1 = s0[r], p2 = s1[r], p3 = s2[r], p4 = s3[r], p5 = s4[r], p7 = s6[r], p8 = s7[r], p9 = s8[r];
+    const cplx p1 = s0[r], p2 = s1[r], p3 = s2[r], p4 = s3[r], p5 = s4[r], p8 = s7[r];
     double lr[2 * BW + 1];
     load_lrow<BW, MID>(g, P.D0LU, r, lr);
     const cplx u1a = cadd(cscale(csub(p1, p3), kxkz), cscale(p5, kz2 - kx2));
     const cplx u3a = csub(csub(cscale(p1, -kx2), cscale(p5, 2.0 * kxkz)), cscale(p3, kz2));
     const cplx u3b = csub(cmul(ikx, p4), cmul(ikz, p2));
     const cplx u4b = cadd(cmul(ikz, p4), cmul(ikx, p2));
-    const cplx u7a = cadd(cmul(ikx, p7), cmul(ikz, p9));
     const cplx yB = fwd_row<BW, MID>(g, r, lr, wB, s0, u3a);
     const cplx yE = fwd_row<BW, MID>(g, r, lr, wE, s2, u3b);
     const cplx yF = fwd_row<BW, MID>(g, r, lr, wF, s4, u4b);
     const cplx y8 = fwd_row<BW, MID>(g, r, lr, w8, s7, p8);
-    s0[r] = yB; s2[r] = yE; s4[r] = yF; s7[r] = y8; s6[r] = u1a; s8[r] = u7a;
+    s0[r] = yB; s2[r] = yE; s4[r] = yF; s7[r] = y8; su[r] = u1a;
     wB.push(yB); wE.push(yE); wF.push(yF); w8.push(y8);
   });
 }
@@ -511,7 +513,7 @@ PB_HD void sweep_Bg(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const RhsCoef q = rhs_coef(P, c.k2, false);
-  SV s2 = c.col(A, 2), s6 = c.col(A, 6), r1 = c.rk(A, 0), y1 = c.buf(A, 0), y2 = c.buf(A, 1);
+  SV s2 = c.col(A, 2), su = c.u1a(A), r1 = c.rk(A, 0), y1 = c.buf(A, 0), y2 = c.buf(A, 1);
   CWin<2 * BW + 1> cE, c1, c2;
   cE.clear(); c1.clear(); c2.clear();
   up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
@@ -524,7 +526,7 @@ PB_HD void sweep_Bg(const LineParams &P, const WsArgs &A, const SysCtx &c) {
       c2.push(bwd_row<BW, MID>(g, r, ur, c2, y2[r]));
     }
     for_completed_rows<BW>(n, r, [&](int m) {
-      const cplx hg = cadd(s6[m], mv_row<BW>(g, P.D1, m, r, cE));
+      const cplx hg = cadd(su[m], mv_row<BW>(g, P.D1, m, r, cE));
       const cplx d2s = S1 ? mv_row<BW>(g, P.D2, m, r, c1) : mk(0.0, 0.0);
       if (S1) y2[m] = mv_row<BW>(g, P.D2, m, r, c2);
       s2[m] = rhs_row(P, q, m, n, r1, d2s, hg);
@@ -537,7 +539,8 @@ PB_HD void sweep_Bt(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const RhsCoef q = rhs_coef(P, c.k2, true);
-  SV s7 = c.col(A, 7), s8 = c.col(A, 8), r3 = c.rk(A, 2), y3 = c.col(A, 5);
+  SV s6 = c.col(A, 6), s7 = c.col(A, 7), s8 = c.col(A, 8), r3 = c.rk(A, 2), y3 = c.col(A, 5);
+  const cplx ikx = iunit(c.kx), ikz = iunit(c.kz);
   CWin<2 * BW + 1> c8, c3;
   c8.clear(); c3.clear();
   up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
@@ -547,7 +550,8 @@ PB_HD void sweep_Bt(const LineParams &P, const WsArgs &A, const SysCtx &c) {
     c8.push(bwd_row<BW, MID>(g, r, ur, c8, s7[r]));
     if (S1) c3.push(bwd_row<BW, MID>(g, r, ur, c3, y3[r]));
     for_completed_rows<BW>(n, r, [&](int m) {
-      cplx ht = cneg(cadd(s8[m], mv_row<BW>(g, P.D1, m, r, c8)));
+      const cplx u7a = cadd(cmul(ikx, s6[m]), cmul(ikz, s8[m]));
+      cplx ht = cneg(cadd(u7a, mv_row<BW>(g, P.D1, m, r, c8)));
       if (A.usrc) ht = cadd(ht, cscale(A.usrc[c.sys + (size_t)m * A.plane], A.src_scale));  // mixed-type source
       const cplx d2s = S1 ? mv_row<BW>(g, P.D2, m, r, c3) : mk(0.0, 0.0);
       s7[m] = rhs_row(P, q, m, n, r3, d2s, ht);
