@@ -783,10 +783,9 @@ static int launch_sweeps(pb_ctx *c, const pb::LineParams &P, const pb::WsArgs &A
   const bool s1 = s == 1;
   if (s1) SWEEP(pb::SW_A1, c->stream);
   SWEEP(pb::SW_A, c->stream);
-  // reads u1a in slot 6 before D_t writes theta there
   if (s1) SWEEP(pb::SW_BG1, c->stream) else SWEEP(pb::SW_BG, c->stream);
   // theta chain (slots 5, 6, 7, 8, RK3, dbuf2) beside B_v, C_nu, D_nu (slots
-  // 0-4, RK1, RK2, dbuf); joined before E and F overwrite slots 5, 7, 8
+  // 0-4, RK1, RK2, dbuf); joined before F writes slot 7
   hipStream_t th = c->linear ? c->stream : c->side2;
   if (!c->linear) {
     HIPCHK(hipEventRecord(c->ev_fork2, c->stream));

@@ -33,8 +33,7 @@
 //   after C : 0 y_phip, 2 y_g, 7 y_theta, R0 y_phih; U factors in dbuf, dbuf2
 //   after D : 0 phi_p, 2 g, 6 theta, R1 phi_h, R2 D2 phi_h; RK updated
 //   after E : 0 y_vp, R3 y_vh; U factor in dbuf
-//   after F : 7 c_p, R4 c_h (the Poisson solutions; G forms D0 c_p = v_p,
-//             D1 c_p = v_p', D1 c_h = v_h', D0 c_h from them)
+//   after F : 0 v_p, 7 v_p', R4 v_h', R0 D0 c_h
 //   after G : 0 u, 1 v, 2 w, 6 theta; RK2 = store_phi
 // R0..R4 are real columns (the reference's (1+i)-scaled homogeneous
 // solutions are (x, x): only x is stored); R0..R3 live in the memory of the
@@ -805,19 +804,16 @@ PB_HD void sweep_E(const LineParams &P, const WsArgs &A, const PairCtx &pc) {
   else sweep_E_t<BW, false>(P, A, pc);
 }
 
-// F: back substitution with U of the Poisson operator (solve_v_f.f90:258-271):
-// the particular solution c_p -> slot 7 (free since D_t), the homogeneous one
-// c_h -> R4 (by the pair's primary system; the partner still reads R3).
-// Their D0/D1 products are formed in G, which reads c_p and c_h once
-// instead of four product columns; G writes u into slot 0 while its windows
-// still read c_p, so c_p is not kept in slot 0.
+// F: back substitution with U of the Poisson operator and the D0/D1
+// products v_p -> slot 0, v_p' -> slot 7, v_h' -> R4, D0 c_h -> R0
+// (solve_v_f.f90:258-271, 352)
 template <int BW>
 PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const OwnU Uq{A.dbuf + c.prim, A.plane, 2 * BW + 1};
   SV s0 = c.col(A, 0), s7 = c.col(A, 7);
-  RV h5 = c.rcol(A, 3), hc = c.rcol(A, 4);
+  RV h5 = c.rcol(A, 3), h4 = c.rcol(A, 4), h8 = c.rcol(A, 0);
   CWin<2 * BW + 1> wp;
   RWin<2 * BW + 1> wh;
   wp.clear(); wh.clear();
@@ -825,48 +821,30 @@ PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
     constexpr bool MID = decltype(kind)::mid;
     double ur[2 * BW];
     load_urow<BW, MID>(g, Uq, r, ur);
-    const cplx xp = bwd_row<BW, MID>(g, r, ur, wp, s0[r]);
-    const double xh = bwd_row_r<BW, MID>(g, r, ur, wh, h5[r]);
-    wp.push(xp);
-    wh.push(xh);
-    s7[r] = xp;
-    if (c.is_prim()) hc[r] = xh;
+    wp.push(bwd_row<BW, MID>(g, r, ur, wp, s0[r]));
+    wh.push(bwd_row_r<BW, MID>(g, r, ur, wh, h5[r]));
+    for_completed_rows<BW>(n, r, [&](int m) {
+      const cplx vp = mv_row<BW>(g, P.D0, m, r, wp);
+      const cplx dvp = mv_row<BW>(g, P.D1, m, r, wp);
+      const double dvh = mv_row_r<BW>(g, P.D1, m, r, wh);
+      const double d0h = mv_row_r<BW>(g, P.D0, m, r, wh);
+      s0[m] = vp; s7[m] = dvp;
+      if (c.is_prim()) { h4[m] = dvh; h8[m] = d0h; }
+    });
   });
 }
 
-// Sliding windows over a stored column for the band products of rows taken
-// in ascending (asc) or descending order: w.w[q] = x(r + q) with r the first
-// column of the row's band, clamp(m - bw, 0, n - 1 - 2bw) -- the windows of
-// the up sweep that produced x, so each product is the same expression.
-PB_HD int band_first(int m, int bw, int n) { return m < bw ? 0 : (m > n - 1 - bw ? n - 1 - 2 * bw : m - bw); }
-
 // G: influence coefficients and the final combination on mirror pairs
-// (solve_v_f.f90:258-271, 293-355; solve_u_w.f90:27-38): u -> slot 0, v -> 1,
-// w -> 2.  v_p = D0 c_p, v_p' = D1 c_p, v_h' = D1 c_h and D0 c_h are formed
-// here, rows i ascending and j = n-1-i descending, each with its windows.
-template <int BW>
+// (solve_v_f.f90:293-355, solve_u_w.f90:27-38): u -> slot 0, v -> 1, w -> 2
 PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
-  const BandGeom g{c.n, BW};
-  const int n = c.n, s = P.s, S = P.S, rl = n - 1 - 2 * BW;
+  const int n = c.n, s = P.s, S = P.S;
   const double dt = P.dt, nu = P.nu, k2 = c.k2;
-  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), cp = c.col(A, 7);
-  RV h1 = c.rcol(A, 1), h3 = c.rcol(A, 2), hc = c.rcol(A, 4);
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s7 = c.col(A, 7);
+  RV h1 = c.rcol(A, 1), h3 = c.rcol(A, 2), h5 = c.rcol(A, 4), h8 = c.rcol(A, 0);
   SV r2 = c.rk(A, 1);
-  // windows: pa, ha ascending from rows 0..2bw; pd, hd descending from rl..n-1
-  CWin<2 * BW + 1> pa, pd;
-  RWin<2 * BW + 1> ha, hd;
-#pragma unroll
-  for (int q = 0; q <= 2 * BW; ++q) {
-    pa.w[q] = cp[q];
-    ha.w[q] = hc[q];
-    pd.w[q] = cp[rl + q];
-    hd.w[q] = hc[rl + q];
-  }
-  int ra = 0, rd = rl;
   // the real columns stand for the complex (x, x) of the reference
   auto cx = [](double x) { return mk(x, x); };
-  const cplx h0 = cx(mv_row_r<BW>(g, P.D1, 0, 0, ha)), hn = cx(mv_row_r<BW>(g, P.D1, n - 1, rl, hd));
-  const cplx d0 = mv_row<BW>(g, P.D1, 0, 0, pa), dn = mv_row<BW>(g, P.D1, n - 1, rl, pd);
+  const cplx h0 = cx(h5[0]), hn = cx(h5[n - 1]), d0 = s7[0], dn = s7[n - 1];
   const cplx den = csub(cmul(h0, h0), cmul(hn, hn));
   const cplx c1 = cdiv(csub(cmul(hn, dn), cmul(h0, d0)), den);
   const cplx c2 = cdiv(csub(cmul(h0, dn), cmul(hn, d0)), den);
@@ -880,28 +858,9 @@ PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const cplx kxx = cdivr(cmul(ci, mk(c.kx, 0.0)), k2), kzz = cdivr(cmul(ci, mk(c.kz, 0.0)), k2);
   for (int i = 0; i <= (n - 1) / 2; ++i) {
     const int j = n - 1 - i;
-    const int ri = band_first(i, BW, n), rj = band_first(j, BW, n);
-    if (ri > ra) {  // one step up: drop x(ra), append x(ri + 2bw)
-#pragma unroll
-      for (int q = 0; q < 2 * BW; ++q) {
-        pa.w[q] = pa.w[q + 1];
-        ha.w[q] = ha.w[q + 1];
-      }
-      pa.w[2 * BW] = cp[ri + 2 * BW];
-      ha.w[2 * BW] = hc[ri + 2 * BW];
-      ra = ri;
-    }
-    if (rj < rd) {  // one step down: prepend x(rj)
-      pd.push(cp[rj]);
-      hd.push(hc[rj]);
-      rd = rj;
-    }
-    const cplx dvp_i = mv_row<BW>(g, P.D1, i, ri, pa), dvp_j = mv_row<BW>(g, P.D1, j, rj, pd);
-    const cplx vp_i = mv_row<BW>(g, P.D0, i, ri, pa), vp_j = mv_row<BW>(g, P.D0, j, rj, pd);
-    const cplx dvh_i = cx(mv_row_r<BW>(g, P.D1, i, ri, ha)), dvh_j = cx(mv_row_r<BW>(g, P.D1, j, rj, hd));
-    const cplx d0h_i = cx(mv_row_r<BW>(g, P.D0, i, ri, ha)), d0h_j = cx(mv_row_r<BW>(g, P.D0, j, rj, hd));
+    const cplx dvp_i = s7[i], dvp_j = s7[j], dvh_i = cx(h5[i]), dvh_j = cx(h5[j]);
     const cplx sp_i = r2[i], sp_j = r2[j], ph_i = cx(h1[i]), ph_j = cx(h1[j]), d2h_i = cx(h3[i]), d2h_j = cx(h3[j]);
-    const cplx g_i = s2[i], g_j = s2[j];
+    const cplx vp_i = s0[i], vp_j = s0[j], d0h_i = cx(h8[i]), d0h_j = cx(h8[j]), g_i = s2[i], g_j = s2[j];
     auto one = [&](int row, cplx dvp, cplx dvh, cplx dvh_m, cplx sp, cplx ph, cplx ph_m, cplx d2h, cplx d2h_m,
                    cplx vp, cplx d0h, cplx d0h_m, cplx gg) {
       const cplx f = csub(cadd(dvp, cmul(c1, dvh)), cmul(c2, dvh_m));
@@ -999,7 +958,7 @@ PB_HD void run_sweep(const LineParams &P, const WsArgs &A, double *dbuf2, size_t
   if (SW == SW_BV) sweep_Bv<BW>(P, A, c);
   if (SW == SW_DNU) sweep_Dnu<BW>(P, A, c);
   if (SW == SW_F) sweep_F<BW>(P, A, c);
-  if (SW == SW_G) sweep_G<BW>(P, A, c);
+  if (SW == SW_G) sweep_G(P, A, c);
 }
 
 // the sweeps of substep s in a serial order valid for the stream schedule
