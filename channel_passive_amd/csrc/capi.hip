@@ -782,19 +782,21 @@ static int launch_sweeps(pb_ctx *c, const pb::LineParams &P, const pb::WsArgs &A
   }
   const bool s1 = s == 1;
   if (s1) SWEEP(pb::SW_A1, c->stream);
-  SWEEP(pb::SW_A, c->stream);
-  if (s1) SWEEP(pb::SW_BG1, c->stream) else SWEEP(pb::SW_BG, c->stream);
-  // theta chain (slots 5, 6, 7, 8, RK3, dbuf2) beside B_v, C_nu, D_nu (slots
-  // 0-4, RK1, RK2, dbuf); joined before F writes slot 7
+  // theta chain (slots 5-8, RK3, dbuf2) beside A, B_g, B_v, C_nu, D_nu
+  // (slots 0-4, the u1a column, buffers 0-1, RK1, RK2, dbuf); joined before
+  // F writes slot 7
   hipStream_t th = c->linear ? c->stream : c->side2;
   if (!c->linear) {
     HIPCHK(hipEventRecord(c->ev_fork2, c->stream));
     HIPCHK(hipStreamWaitEvent(c->side2, c->ev_fork2, 0));
   }
+  SWEEP(pb::SW_AT, th);
   if (s1) SWEEP(pb::SW_BT1, th) else SWEEP(pb::SW_BT, th);
   SWEEP(pb::SW_CT, th);
   SWEEP(pb::SW_DT, th);
   if (!c->linear) HIPCHK(hipEventRecord(c->ev_join2, c->side2));
+  SWEEP(pb::SW_A, c->stream);
+  if (s1) SWEEP(pb::SW_BG1, c->stream) else SWEEP(pb::SW_BG, c->stream);
   SWEEP(pb::SW_BV, c->stream);
   SWEEP(pb::SW_CNU, c->stream);
   SWEEP(pb::SW_DNU, c->stream);

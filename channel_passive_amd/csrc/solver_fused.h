@@ -27,8 +27,8 @@
 // row m), so every window index is a compile-time constant.
 //
 // Slot use per system (0-based product slots; s = substep):
-//   after A : 0 yB, 1 P2, 2 yE, 3 P4, 4 yF, 6 P7, 7 y8, 8 P9; u1a in the
-//             u1a column (P7 and P9 are read only by B_t, which forms u7a)
+//   after A, A_t: 0 yB, 1 P2, 2 yE, 3 P4, 4 yF, 6 P7, 7 y8, 8 P9; u1a in
+//             the u1a column (P7 and P9 are read only by B_t, which forms u7a)
 //   after B : 0 rhs_phi, 2 rhs_g, 7 rhs_theta, RK = zeta*h (s < S)
 //   after C : 0 y_phip, 2 y_g, 7 y_theta, R0 y_phih; U factors in dbuf, dbuf2
 //   after D : 0 phi_p, 2 g, 6 theta, R1 phi_h, R2 D2 phi_h; RK updated
@@ -441,21 +441,21 @@ PB_HD void sweep_A1(const LineParams &P, const WsArgs &A, const SysCtx &c) {
 }
 
 // A: hvhg point-wise terms (hvhg.f90:169-204, 256-258) and forward
-// substitution of u3a, u3b, u4b, P8 with LU(D0).  u7a (from P7 and P9) is
-// formed where it is used, in B_t: A does not read P7 and P9.
+// substitution of u3a, u3b, u4b with LU(D0).  u7a (from P7 and P9) is formed
+// where it is used, in B_t, and P8's forward substitution is the theta
+// chain's own first sweep (A_t): A reads only P1-P5.
 template <int BW>
 PB_HD void sweep_A(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const double kx = c.kx, kz = c.kz;
   const double kz2 = kz * kz, kx2 = kx * kx, kxkz = kx * kz;
   const cplx ikx = iunit(kx), ikz = iunit(kz);
-  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s4 = c.col(A, 4), s7 = c.col(A, 7),
-     su = c.u1a(A);
-  CWin<BW> wB, wE, wF, w8;
-  wB.clear(); wE.clear(); wF.clear(); w8.clear();
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s4 = c.col(A, 4), su = c.u1a(A);
+  CWin<BW> wB, wE, wF;
+  wB.clear(); wE.clear(); wF.clear();
   down_rows<BW>(c.n, [&](int r, auto kind) __attribute__((always_inline)) {
     constexpr bool MID = decltype(kind)::mid;
-    const cplx p1 = s0[r], p2 = s1[r], p3 = s2[r], p4 = s3[r], p5 = s4[r], p8 = s7[r];
+    const cplx p1 = s0[r], p2 = s1[r], p3 = s2[r], p4 = s3[r], p5 = s4[r];
     double lr[2 * BW + 1];
     load_lrow<BW, MID>(g, P.D0LU, r, lr);
     const cplx u1a = cadd(cscale(csub(p1, p3), kxkz), cscale(p5, kz2 - kx2));
@@ -465,9 +465,26 @@ PB_HD void sweep_A(const LineParams &P, const WsArgs &A, const SysCtx &c) {
     const cplx yB = fwd_row<BW, MID>(g, r, lr, wB, s0, u3a);
     const cplx yE = fwd_row<BW, MID>(g, r, lr, wE, s2, u3b);
     const cplx yF = fwd_row<BW, MID>(g, r, lr, wF, s4, u4b);
-    const cplx y8 = fwd_row<BW, MID>(g, r, lr, w8, s7, p8);
-    s0[r] = yB; s2[r] = yE; s4[r] = yF; s7[r] = y8; su[r] = u1a;
-    wB.push(yB); wE.push(yE); wF.push(yF); w8.push(y8);
+    s0[r] = yB; s2[r] = yE; s4[r] = yF; su[r] = u1a;
+    wB.push(yB); wE.push(yE); wF.push(yF);
+  });
+}
+
+// A_t: forward substitution of P8 with LU(D0) (hvhg.f90:256-258's operand),
+// the first sweep of the theta chain
+template <int BW>
+PB_HD void sweep_At(const LineParams &P, const WsArgs &A, const SysCtx &c) {
+  const BandGeom g{c.n, BW};
+  SV s7 = c.col(A, 7);
+  CWin<BW> w8;
+  w8.clear();
+  down_rows<BW>(c.n, [&](int r, auto kind) __attribute__((always_inline)) {
+    constexpr bool MID = decltype(kind)::mid;
+    double lr[2 * BW + 1];
+    load_lrow<BW, MID>(g, P.D0LU, r, lr);
+    const cplx y8 = fwd_row<BW, MID>(g, r, lr, w8, s7, s7[r]);
+    s7[r] = y8;
+    w8.push(y8);
   });
 }
 
@@ -921,12 +938,13 @@ PB_HD void deriv_line(const LineParams &P, const SV &x1, const SV &x2, const SV 
 
 // One sweep: index is a system (kz-major, kx-minor) or, for the
 // factorising sweeps C_nu, C_theta and E, a pair (kz index 0 .. nz/2).
-// A substep runs, for all systems, A1 (s == 1 only), A, B_g, then the theta
-// chain B_t, C_t, D_t and, in any order relative to it, B_phi, C_nu, D_nu;
-// then E, F, G (on s == 1 B_g1 and B_t1 stand for B_g and B_t).
+// A substep runs, for all systems, A1 (s == 1 only), then the theta chain
+// A_t, B_t, C_t, D_t and, in any order relative to it, A, B_g, B_phi, C_nu,
+// D_nu; then E, F, G (on s == 1 B_g1 and B_t1 stand for B_g and B_t).
 enum SweepId {
   SW_A1, SW_A, SW_BT, SW_CT, SW_DT, SW_BG, SW_BV, SW_CNU, SW_DNU, SW_E, SW_F, SW_G,
   SW_BT1, SW_BG1,  // substep 1: B with D2 LU(D0)^-1 of the stores
+  SW_AT,           // theta chain: P8's forward substitution
   SW_COUNT
 };
 
@@ -950,6 +968,7 @@ PB_HD void run_sweep(const LineParams &P, const WsArgs &A, double *dbuf2, size_t
   if (!make_sys(A, P.g.n, idx, c)) return;
   if (SW == SW_A1) sweep_A1<BW>(P, A, c);
   if (SW == SW_A) sweep_A<BW>(P, A, c);
+  if (SW == SW_AT) sweep_At<BW>(P, A, c);
   if (SW == SW_BT) sweep_Bt<BW, false>(P, A, c);
   if (SW == SW_BT1) sweep_Bt<BW, true>(P, A, c);
   if (SW == SW_DT) sweep_Dt<BW>(P, A, c, dbuf2);
@@ -966,7 +985,7 @@ inline int sweep_sequence(int s, int *ids) {
   int k = 0;
   const bool s1 = s == 1;
   if (s1) ids[k++] = SW_A1;
-  const int rest[] = {SW_A, s1 ? SW_BG1 : SW_BG, s1 ? SW_BT1 : SW_BT, SW_CT, SW_DT, SW_BV,
+  const int rest[] = {SW_AT, s1 ? SW_BT1 : SW_BT, SW_CT, SW_DT, SW_A, s1 ? SW_BG1 : SW_BG, SW_BV,
                       SW_CNU, SW_DNU, SW_E, SW_F, SW_G};
   for (int q : rest) ids[k++] = q;
   return k;

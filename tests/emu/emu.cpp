@@ -139,6 +139,7 @@ static void run_id(int id, const pb::LineParams &P, const pb::WsArgs &A, double 
     case pb::SW_BG1: pb::run_sweep<BW, pb::SW_BG1>(P, A, dbuf2, sys); break;
     case pb::SW_BT1: pb::run_sweep<BW, pb::SW_BT1>(P, A, dbuf2, sys); break;
     case pb::SW_A: pb::run_sweep<BW, pb::SW_A>(P, A, dbuf2, sys); break;
+    case pb::SW_AT: pb::run_sweep<BW, pb::SW_AT>(P, A, dbuf2, sys); break;
     case pb::SW_BT: pb::run_sweep<BW, pb::SW_BT>(P, A, dbuf2, sys); break;
     case pb::SW_CT: pb::run_sweep<BW, pb::SW_CT>(P, A, dbuf2, sys); break;
     case pb::SW_DT: pb::run_sweep<BW, pb::SW_DT>(P, A, dbuf2, sys); break;
