@@ -782,9 +782,8 @@ static int launch_sweeps(pb_ctx *c, const pb::LineParams &P, const pb::WsArgs &A
   }
   const bool s1 = s == 1;
   if (s1) SWEEP(pb::SW_A1, c->stream);
-  // theta chain (slots 5-8, RK3, dbuf2) beside A, B_g, B_v, C_nu, D_nu
-  // (slots 0-4, the u1a column, buffers 0-1, RK1, RK2, dbuf); joined before
-  // F writes slot 7
+  // theta chain (slots 5-8, RK3, dbuf2) beside all other sweeps (slots 0-4,
+  // the extra column, buffers 0-1, RK1, RK2, dbuf); joined at the end
   hipStream_t th = c->linear ? c->stream : c->side2;
   if (!c->linear) {
     HIPCHK(hipEventRecord(c->ev_fork2, c->stream));
@@ -800,10 +799,10 @@ static int launch_sweeps(pb_ctx *c, const pb::LineParams &P, const pb::WsArgs &A
   SWEEP(pb::SW_BV, c->stream);
   SWEEP(pb::SW_CNU, c->stream);
   SWEEP(pb::SW_DNU, c->stream);
-  if (!c->linear) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
   SWEEP(pb::SW_E, c->stream);
   SWEEP(pb::SW_F, c->stream);
   SWEEP(pb::SW_G, c->stream);
+  if (!c->linear) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
 #undef SWEEP
   return PB_OK;
 }
@@ -817,8 +816,8 @@ static pb::WsArgs ws_args(pb_ctx *c, int s) {
   A.dbuf = c->dbuf;
   A.rbuf = c->rbuf;
   // the padded pencils are dead during the wall-normal stage: their first
-  // field holds the u1a column of the sweeps
-  A.u1a = (pb::cplx *)c->pad;
+  // field holds the sweeps' extra column
+  A.xcol = (pb::cplx *)c->pad;
   A.vd = c->cfg.opp_alpha != 0.0 ? (const pb::cplx *)c->vd : nullptr;
   A.usrc = c->cfg.scalar_bc == PB_SCALAR_MIXED ? (const pb::cplx *)c->usrc : nullptr;
   A.src_scale = c->cfg.nu / c->cfg.pr / c->cfg.bulk_u;

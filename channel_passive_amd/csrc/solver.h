@@ -572,7 +572,8 @@ struct WsArgs {
   cplx *buf;      // 2 scratch columns per system
   double *dbuf;   // W*ny doubles per system
   double *rbuf;   // one real scratch column per system (fused sweeps only)
-  cplx *u1a;      // one complex scratch column per system (fused sweeps only): hvhg's u1a, sweep A -> B_g
+  cplx *xcol;     // one complex scratch column per system (fused sweeps only): hvhg's u1a from
+                  // sweep A to B_g, then v_p' from F to G
   const cplx *vd; // detection-plane v (2 planes) or nullptr
   // mixed-type scalar condition: u of the substep's input state (one
   // field, device layout) and the source factor (nu/pr)/bulk_u; nullptr for

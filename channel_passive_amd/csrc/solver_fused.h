@@ -28,12 +28,13 @@
 //
 // Slot use per system (0-based product slots; s = substep):
 //   after A, A_t: 0 yB, 1 P2, 2 yE, 3 P4, 4 yF, 6 P7, 7 y8, 8 P9; u1a in
-//             the u1a column (P7 and P9 are read only by B_t, which forms u7a)
+//             the extra column X (P7 and P9 are read only by B_t, which forms u7a)
 //   after B : 0 rhs_phi, 2 rhs_g, 7 rhs_theta, RK = zeta*h (s < S)
 //   after C : 0 y_phip, 2 y_g, 7 y_theta, R0 y_phih; U factors in dbuf, dbuf2
 //   after D : 0 phi_p, 2 g, 6 theta, R1 phi_h, R2 D2 phi_h; RK updated
 //   after E : 0 y_vp, R3 y_vh; U factor in dbuf
-//   after F : 0 v_p, 7 v_p', R4 v_h', R0 D0 c_h
+//   after F : 0 v_p, X v_p', R4 v_h', R0 D0 c_h (E, F and G touch no slot of
+//             the theta chain, which joins at the end of the stage)
 //   after G : 0 u, 1 v, 2 w, 6 theta; RK2 = store_phi
 // R0..R4 are real columns (the reference's (1+i)-scaled homogeneous
 // solutions are (x, x): only x is stored); R0..R3 live in the memory of the
@@ -109,7 +110,7 @@ struct SysCtx {
   }
   PB_HD SV rk(const WsArgs &A, int slot) const { return SV{A.RK + (size_t)slot * A.field + sys, A.plane}; }
   PB_HD SV buf(const WsArgs &A, int slot) const { return SV{A.buf + (size_t)slot * A.field + sys, A.plane}; }
-  PB_HD SV u1a(const WsArgs &A) const { return SV{A.u1a + sys, A.plane}; }
+  PB_HD SV xcol(const WsArgs &A) const { return SV{A.xcol + sys, A.plane}; }
 };
 
 PB_HD bool make_sys(const WsArgs &A, int n, size_t sys, SysCtx &c) {
@@ -450,7 +451,7 @@ PB_HD void sweep_A(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const double kx = c.kx, kz = c.kz;
   const double kz2 = kz * kz, kx2 = kx * kx, kxkz = kx * kz;
   const cplx ikx = iunit(kx), ikz = iunit(kz);
-  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s4 = c.col(A, 4), su = c.u1a(A);
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s3 = c.col(A, 3), s4 = c.col(A, 4), su = c.xcol(A);
   CWin<BW> wB, wE, wF;
   wB.clear(); wE.clear(); wF.clear();
   down_rows<BW>(c.n, [&](int r, auto kind) __attribute__((always_inline)) {
@@ -530,7 +531,7 @@ PB_HD void sweep_Bg(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const RhsCoef q = rhs_coef(P, c.k2, false);
-  SV s2 = c.col(A, 2), su = c.u1a(A), r1 = c.rk(A, 0), y1 = c.buf(A, 0), y2 = c.buf(A, 1);
+  SV s2 = c.col(A, 2), su = c.xcol(A), r1 = c.rk(A, 0), y1 = c.buf(A, 0), y2 = c.buf(A, 1);
   CWin<2 * BW + 1> cE, c1, c2;
   cE.clear(); c1.clear(); c2.clear();
   up_rows<BW>(n, [&](int r, auto kind) __attribute__((always_inline)) {
@@ -822,14 +823,14 @@ PB_HD void sweep_E(const LineParams &P, const WsArgs &A, const PairCtx &pc) {
 }
 
 // F: back substitution with U of the Poisson operator and the D0/D1
-// products v_p -> slot 0, v_p' -> slot 7, v_h' -> R4, D0 c_h -> R0
+// products v_p -> slot 0, v_p' -> column X, v_h' -> R4, D0 c_h -> R0
 // (solve_v_f.f90:258-271, 352)
 template <int BW>
 PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const OwnU Uq{A.dbuf + c.prim, A.plane, 2 * BW + 1};
-  SV s0 = c.col(A, 0), s7 = c.col(A, 7);
+  SV s0 = c.col(A, 0), sx = c.xcol(A);
   RV h5 = c.rcol(A, 3), h4 = c.rcol(A, 4), h8 = c.rcol(A, 0);
   CWin<2 * BW + 1> wp;
   RWin<2 * BW + 1> wh;
@@ -845,7 +846,7 @@ PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
       const cplx dvp = mv_row<BW>(g, P.D1, m, r, wp);
       const double dvh = mv_row_r<BW>(g, P.D1, m, r, wh);
       const double d0h = mv_row_r<BW>(g, P.D0, m, r, wh);
-      s0[m] = vp; s7[m] = dvp;
+      s0[m] = vp; sx[m] = dvp;
       if (c.is_prim()) { h4[m] = dvh; h8[m] = d0h; }
     });
   });
@@ -856,12 +857,12 @@ PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
 PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const int n = c.n, s = P.s, S = P.S;
   const double dt = P.dt, nu = P.nu, k2 = c.k2;
-  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), s7 = c.col(A, 7);
+  SV s0 = c.col(A, 0), s1 = c.col(A, 1), s2 = c.col(A, 2), sx = c.xcol(A);
   RV h1 = c.rcol(A, 1), h3 = c.rcol(A, 2), h5 = c.rcol(A, 4), h8 = c.rcol(A, 0);
   SV r2 = c.rk(A, 1);
   // the real columns stand for the complex (x, x) of the reference
   auto cx = [](double x) { return mk(x, x); };
-  const cplx h0 = cx(h5[0]), hn = cx(h5[n - 1]), d0 = s7[0], dn = s7[n - 1];
+  const cplx h0 = cx(h5[0]), hn = cx(h5[n - 1]), d0 = sx[0], dn = sx[n - 1];
   const cplx den = csub(cmul(h0, h0), cmul(hn, hn));
   const cplx c1 = cdiv(csub(cmul(hn, dn), cmul(h0, d0)), den);
   const cplx c2 = cdiv(csub(cmul(h0, dn), cmul(hn, d0)), den);
@@ -875,7 +876,7 @@ PB_HD void sweep_G(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const cplx kxx = cdivr(cmul(ci, mk(c.kx, 0.0)), k2), kzz = cdivr(cmul(ci, mk(c.kz, 0.0)), k2);
   for (int i = 0; i <= (n - 1) / 2; ++i) {
     const int j = n - 1 - i;
-    const cplx dvp_i = s7[i], dvp_j = s7[j], dvh_i = cx(h5[i]), dvh_j = cx(h5[j]);
+    const cplx dvp_i = sx[i], dvp_j = sx[j], dvh_i = cx(h5[i]), dvh_j = cx(h5[j]);
     const cplx sp_i = r2[i], sp_j = r2[j], ph_i = cx(h1[i]), ph_j = cx(h1[j]), d2h_i = cx(h3[i]), d2h_j = cx(h3[j]);
     const cplx vp_i = s0[i], vp_j = s0[j], d0h_i = cx(h8[i]), d0h_j = cx(h8[j]), g_i = s2[i], g_j = s2[j];
     auto one = [&](int row, cplx dvp, cplx dvh, cplx dvh_m, cplx sp, cplx ph, cplx ph_m, cplx d2h, cplx d2h_m,

@@ -78,7 +78,7 @@ int emu_wavespace_act(const pb_config *cfg, int s, double dt, double *U, double 
   std::vector<double> dbuf((size_t)S.W * field);
   pb::WsArgs A;
   A.U = (pb::cplx *)U; A.RK = (pb::cplx *)RK; A.buf = buf.data(); A.dbuf = dbuf.data(); A.rbuf = nullptr;
-  A.u1a = nullptr;
+  A.xcol = nullptr;
   A.vd = (const pb::cplx *)vd; A.plane = plane; A.field = field; A.nxh = nxh; A.nz = nz; A.nh = nz / 2;
   A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha; A.mean_out = mean4;
   A.usrc = cfg->scalar_bc == PB_SCALAR_MIXED ? (const pb::cplx *)usrc : nullptr;
@@ -162,10 +162,10 @@ extern "C" int emu_wavespace_fused_act(const pb_config *cfg, int s, double dt, d
   const size_t plane = (size_t)nz * nxh, field = plane * S.n;
   std::vector<pb::cplx> buf(2 * field);
   std::vector<double> dbuf((size_t)S.W * field), dbuf2((size_t)S.W * field), rbuf(field);
-  std::vector<pb::cplx> u1a(field);
+  std::vector<pb::cplx> xcol(field);
   pb::WsArgs A;
   A.U = (pb::cplx *)U; A.RK = (pb::cplx *)RK; A.buf = buf.data(); A.dbuf = dbuf.data(); A.rbuf = rbuf.data();
-  A.u1a = u1a.data();
+  A.xcol = xcol.data();
   A.vd = (const pb::cplx *)vd; A.plane = plane; A.field = field; A.nxh = nxh; A.nz = nz; A.nh = nz / 2;
   A.lmfxr = 1.0 / cfg->lmfx; A.lmfzr = 1.0 / cfg->lmfz; A.opp_alpha = cfg->opp_alpha; A.mean_out = mean4;
   A.usrc = cfg->scalar_bc == PB_SCALAR_MIXED ? (const pb::cplx *)usrc : nullptr;
