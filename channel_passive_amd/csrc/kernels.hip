@@ -1542,12 +1542,14 @@ __global__ void __launch_bounds__(64) vv_kernel(const cplx *U, size_t plane, int
 }
 // statistics profiles (get_statistics_1): one thread per (y, product) for the
 // ten Parseval sums, serial over the plane's modes; stat: 16 lines of ny,
-// then u_center, t_center
+// then u_center, t_center.  The ten products of one y are neighbouring
+// threads, so a wave's loads touch the rows of about six y (each row a
+// separate memory page) instead of 64.
 __constant__ int kStatPairs[10][2] = {{0, 0}, {1, 1}, {2, 2}, {6, 6}, {0, 1}, {0, 2}, {1, 2}, {0, 6}, {6, 2}, {1, 6}};
 __global__ void __launch_bounds__(64) stats_uv_kernel(const cplx *U, size_t plane, int ny, int nxh, double *stat) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 10 * ny) return;
-  const int q = t / ny, y = t % ny;
+  const int q = t % 10, y = t / 10;
   const size_t field = plane * ny;
   const cplx *a = U + (size_t)kStatPairs[q][0] * field + (size_t)y * plane;
   const cplx *b = U + (size_t)kStatPairs[q][1] * field + (size_t)y * plane;
