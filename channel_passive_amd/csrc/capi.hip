@@ -352,8 +352,11 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
   rc = rc ? rc : dalloc((void **)&c->buf, 2 * c->field * sizeof(double2));
   rc = rc ? rc : dalloc((void **)&c->vd, 2 * c->plane * sizeof(double2));
   if (cfg->scalar_bc == PB_SCALAR_MIXED) rc = rc ? rc : dalloc((void **)&c->usrc, c->field * sizeof(double2));
-  rc = rc ? rc : dalloc((void **)&c->dbuf, (size_t)c->W * c->field * sizeof(double));
-  rc = rc ? rc : dalloc((void **)&c->dbuf2, (size_t)c->W * c->field * sizeof(double));
+  // U factors: W per row and system, in the plane layout (statement-by-statement
+  // solver) or in blocks of 64 systems (fused sweeps), whichever is larger
+  const size_t nfac = std::max((size_t)c->W * c->field, pb::ublk_elems(c->plane, c->ny, c->W));
+  rc = rc ? rc : dalloc((void **)&c->dbuf, nfac * sizeof(double));
+  rc = rc ? rc : dalloc((void **)&c->dbuf2, nfac * sizeof(double));
   rc = rc ? rc : dalloc((void **)&c->rbuf, c->field * sizeof(double));
   rc = rc ? rc : dalloc((void **)&c->vpart, (size_t)c->ny * c->nz32 * 16 * sizeof(double));  // <= 16 waves/block
   rc = rc ? rc : dalloc((void **)&c->scal, SCAL_N * sizeof(double));

@@ -263,7 +263,7 @@ PB_HD void for_completed_rows(int n, int r, F &&emit) {
   }
 }
 
-// A system's own band factor: U entries of row p at dbuf[(p*W + slot)*plane + sys]
+// A system's own band factor: U entries of row p at p[(p*W + slot)*s]
 struct OwnU {
   double *p;
   size_t s;
@@ -271,6 +271,15 @@ struct OwnU {
   PB_HD double operator()(int r, int slot) const { return p[(size_t)(r * W + slot) * s]; }
   PB_HD void set(int r, int slot, double v) const { p[(size_t)(r * W + slot) * s] = v; }
 };
+// The sweeps' U factors in blocks of UBLK consecutive systems (one wave):
+// [block][row][slot][UBLK].  A slot's values of a wave are one 512-byte
+// segment, and all factors of a wave lie within n*W*512 bytes (a few memory
+// pages) rather than one page per (row, slot).
+constexpr int UBLK = 64;
+PB_HD size_t ublk_elems(size_t plane, int n, int W) { return (plane + UBLK - 1) / UBLK * UBLK * (size_t)n * W; }
+PB_HD OwnU own_u(double *base, size_t sys, int n, int W) {
+  return OwnU{base + (sys / UBLK) * ((size_t)n * W * UBLK) + sys % UBLK, UBLK, W};
+}
 
 // Matrix entries formed on the fly.
 struct HelmholtzA {  // (1 + beta k2) D0 - beta D2 with diagonal-only wall rows
@@ -631,7 +640,7 @@ PB_HD void sweep_Cnu_t(const LineParams &P, const WsArgs &A, const PairCtx &pc) 
   const int n = c.n;
   const double be = P.dt * P.coef(1, P.s) * P.nu;
   const HelmholtzA H{P.D0, P.D2, 1.0 + be * c.k2, be, n, BW};
-  const OwnU U{A.dbuf + c.prim, A.plane, 2 * BW + 1};
+  const OwnU U = own_u(A.dbuf, c.prim, c.n, 2 * BW + 1);
   SV s0 = c.col(A, 0), s2 = c.col(A, 2), t0 = pc.b.col(A, 0), t2 = pc.b.col(A, 2);
   RV h1 = c.rcol(A, 0);
   CWin<BW> wg, wp, vg, vp;
@@ -668,7 +677,7 @@ PB_HD void sweep_Ct_t(const LineParams &P, const WsArgs &A, const PairCtx &pc, d
   const int n = c.n;
   const double be = P.dt * P.coef(1, P.s) * P.nu / P.pr;
   const HelmholtzA H{P.D0, P.D2, 1.0 + be * c.k2, be, n, BW};
-  const OwnU U{dbuf2 + c.prim, A.plane, 2 * BW + 1};
+  const OwnU U = own_u(dbuf2, c.prim, c.n, 2 * BW + 1);
   SV s7 = c.col(A, 7), t7 = pc.b.col(A, 7);
   CWin<BW> wt, vt;
   wt.clear(); vt.clear();
@@ -700,7 +709,7 @@ PB_HD void sweep_Dnu(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n, s = P.s, S = P.S;
   const double dt = P.dt, nu = P.nu, k2 = c.k2;
-  const OwnU U{A.dbuf + c.prim, A.plane, 2 * BW + 1};
+  const OwnU U = own_u(A.dbuf, c.prim, c.n, 2 * BW + 1);
   double al_g = 0, du_g = 0, al_p = 0, du_p = 0;
   if (s != S) {
     al_g = dt * P.coef(0, s + 1) * nu;
@@ -745,7 +754,7 @@ template <int BW>
 PB_HD void sweep_Dt(const LineParams &P, const WsArgs &A, const SysCtx &c, double *dbuf2) {
   const BandGeom g{c.n, BW};
   const int n = c.n, s = P.s, S = P.S;
-  const OwnU U{dbuf2 + c.prim, A.plane, 2 * BW + 1};
+  const OwnU U = own_u(dbuf2, c.prim, c.n, 2 * BW + 1);
   double al = 0, du = 0;
   if (s != S) {
     al = P.dt * P.coef(0, s + 1) * P.nu / P.pr;
@@ -781,7 +790,7 @@ PB_HD void sweep_E_t(const LineParams &P, const WsArgs &A, const PairCtx &pc) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
   const PoissonA Q{P.D0, P.D2, c.k2, n, BW};
-  const OwnU Uq{A.dbuf + c.prim, A.plane, 2 * BW + 1};
+  const OwnU Uq = own_u(A.dbuf, c.prim, c.n, 2 * BW + 1);
   const bool opp = A.opp_alpha != 0.0 && A.vd;
   SV s0 = c.col(A, 0), t0 = d.col(A, 0);
   RV h1 = c.rcol(A, 1), h5 = c.rcol(A, 3);
@@ -829,7 +838,7 @@ template <int BW>
 PB_HD void sweep_F(const LineParams &P, const WsArgs &A, const SysCtx &c) {
   const BandGeom g{c.n, BW};
   const int n = c.n;
-  const OwnU Uq{A.dbuf + c.prim, A.plane, 2 * BW + 1};
+  const OwnU Uq = own_u(A.dbuf, c.prim, c.n, 2 * BW + 1);
   SV s0 = c.col(A, 0), sx = c.xcol(A);
   RV h5 = c.rcol(A, 3), h4 = c.rcol(A, 4), h8 = c.rcol(A, 0);
   CWin<2 * BW + 1> wp;

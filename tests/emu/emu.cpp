@@ -2,6 +2,7 @@
 // (channel_passive_amd/csrc/bspline.cpp, solver.h), used only by tests to
 // check the kernel arithmetic bit-for-bit against the oracle without a GPU.
 // The product never loads this library.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -161,7 +162,8 @@ extern "C" int emu_wavespace_fused_act(const pb_config *cfg, int s, double dt, d
   const int nxh = (int)cfg->nx / 2, nz = (int)cfg->nz;
   const size_t plane = (size_t)nz * nxh, field = plane * S.n;
   std::vector<pb::cplx> buf(2 * field);
-  std::vector<double> dbuf((size_t)S.W * field), dbuf2((size_t)S.W * field), rbuf(field);
+  const size_t nfac = std::max((size_t)S.W * field, pb::ublk_elems(plane, S.n, S.W));
+  std::vector<double> dbuf(nfac), dbuf2(nfac), rbuf(field);
   std::vector<pb::cplx> xcol(field);
   pb::WsArgs A;
   A.U = (pb::cplx *)U; A.RK = (pb::cplx *)RK; A.buf = buf.data(); A.dbuf = dbuf.data(); A.rbuf = rbuf.data();
