@@ -346,22 +346,24 @@ int pb_create(const pb_config *cfg, pb_ctx **out) {
     return fail(PB_E_HIP, "no HIP device / stream");
   }
   // Placement of a large context: while the context allocates, a block of up
-  // to 100 GiB is held and then released, so that the arrays do not take the
-  // first free region of the device.  Measured on fresh devices at
-  // 1024x385x768: the first process's wall-normal stage ran at 36.0-36.1 ms
-  // without the block (five of five) and 33.7 ms with it (two of two); later
-  // processes 32.4-35.5 ms either way.  The sweeps walk columns whose rows are
-  // one plane apart, every row a new memory page, and are bound by address
-  // translation as much as by bandwidth, so the physical placement matters.
-  // PB_RESERVE=0 turns it off.
+  // to 160 GiB (as much as leaves room for the context) is held and then
+  // released, so that the arrays lie above the device's first free region.
+  // Measured on fresh devices at 1024x385x768, the first process's
+  // wall-normal stage: 36.0-36.1 ms without the block (five of five), 33.6-
+  // 33.7 ms with 100 GiB (three of three), 32.1 and 32.7 ms with 160 and 180
+  // GiB; later processes 31.7-35.5 ms.  The sweeps walk columns whose rows
+  // are one plane apart, every row a new memory page, and are bound by
+  // address translation as much as by bandwidth, so the physical placement
+  // matters.  PB_RESERVE=<GiB> caps the block (0: off).
   void *reserve = nullptr;
   {
     const size_t est = (size_t)40 * c->field * sizeof(double2), GiB = (size_t)1 << 30;
     const char *e = getenv("PB_RESERVE");
+    const size_t cap = (e ? (size_t)std::strtoull(e, nullptr, 10) : 160) * GiB;
     size_t fr = 0, tot = 0;
-    if (est >= 32 * GiB && !(e && e[0] == '0') && hipMemGetInfo(&fr, &tot) == hipSuccess &&
-        fr > est + 108 * GiB) {
-      if (hipMalloc(&reserve, 100 * GiB) != hipSuccess) {
+    if (est >= 32 * GiB && cap > 0 && hipMemGetInfo(&fr, &tot) == hipSuccess && fr > est + 72 * GiB) {
+      const size_t want = std::min(cap, (fr - est - 8 * GiB) / GiB * GiB);
+      if (hipMalloc(&reserve, want) != hipSuccess) {
         reserve = nullptr;
         (void)hipGetLastError();
       }
