@@ -8,6 +8,7 @@
 // order in which independent values are produced changes:
 //
 //   A  (down) hvhg point-wise terms + forward substitution of 4 RHS with LU(D0)
+//             (P8's in its own kernel A_t, the first of the theta chain)
 //   B  (up)   back substitution + D1/D2 products -> h_g, h_v, h_theta, the
 //             Helmholtz right-hand sides and the zeta*h part of the stores
 //   C  (down) Crout LU of H_nu and H_theta row by row (matrix entries formed
